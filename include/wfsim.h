@@ -1,0 +1,158 @@
+/*
+ * wfsim.h -- C ABI of the MI355X (gfx950) propagation library.
+ *
+ * Drop-in boundary for the WaveFunctionSimPython propagation path.  The
+ * reference has no FFI: its boundary is a Python call surface plus the
+ * positional argument lists of CuPy RawKernels.  Each entry point below
+ * states which reference interface it replaces (paths relative to the
+ * reference repository).
+ *
+ * Conventions
+ *  - All data pointers are DEVICE pointers owned by the caller (e.g. torch
+ *    tensors).  The library owns only plans (twiddle/kinetic tables and
+ *    scratch buffers).
+ *  - Grids are C-order (x slowest, z fastest), interleaved complex.  The
+ *    state is complex64 (WFS_C64) or complex128 (WFS_C128); the potential V
+ *    is always complex64 (sources/cuda_kernels/potential_operator.cu:7).
+ *  - Every call is ordered on the given stream (hipStream_t passed as void*;
+ *    NULL = the null stream) and returns 0 on success, a hipError_t value on a
+ *    HIP failure, or a WFS_ERR_* code; wfs_last_error() describes the last
+ *    failure of the calling thread.
+ *  - A plan is used by one host thread at a time.
+ */
+#ifndef WFSIM_H
+#define WFSIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFS_C64 0
+#define WFS_C128 1
+
+#define WFS_METHOD_FOURIER 0
+#define WFS_METHOD_POWER_SERIES 1
+
+#define WFS_ERR_INVALID 1000
+#define WFS_ERR_UNSUPPORTED 1001
+#define WFS_ERR_ALLOC 1002
+
+typedef struct wfs_plan wfs_plan;
+
+/* Sub-box of the grid for an observable: voxel origin and sample counts. */
+typedef struct wfs_box {
+  int bottom[3];
+  int count[3];
+} wfs_box;
+
+/* Library version string. */
+const char* wfs_version(void);
+
+/* Description of the last error raised on the calling thread ("" if none). */
+const char* wfs_last_error(void);
+
+/*
+ * Create a propagation plan.
+ * Replaces the per-run set-up of SimState.initialize_state
+ * (sources/sim_state.py:257-294): kernel JIT, wave numbers
+ * (2*pi*fftfreq, :275-278) and ping-pong buffers (:291-294).
+ *   n[3]      grid size per axis (8,16,32,64,96,128,192,256,512)
+ *   dx[3]     voxel size in Bohr radii
+ *   dt        time step in hbar/Hartree
+ *   mass      particle mass in electron masses
+ *   precision WFS_C64 or WFS_C128 (double_precision_calculation)
+ *   method    WFS_METHOD_FOURIER or WFS_METHOD_POWER_SERIES
+ *   order     Taylor order p of the power series (order_of_approximation)
+ */
+int wfs_plan_create(const int n[3], const double dx[3], double dt, double mass, int precision, int method,
+                    int order, wfs_plan** out);
+
+/* Release a plan and its device tables. */
+int wfs_plan_destroy(wfs_plan* plan);
+
+/*
+ * Advance psi in place by nsteps time steps.
+ * Replaces SimState.evolve_state (sources/sim_state.py:754-762) with its two
+ * branches _fft_time_evolution (:641-697, kinetic_operator.cu:4-24,
+ * potential_operator.cu:4-16) and _power_series_time_evolution (:721-752,
+ * power_series_operator.cu:3-65).  psi is in position space on entry and exit;
+ * consecutive split-step half kicks are merged internally.
+ *   psi  device state, complex64 or complex128 per plan precision
+ *   V    device complex64 potential
+ */
+int wfs_step(wfs_plan* plan, void* psi, const void* V, int nsteps, void* stream);
+
+/*
+ * Unitary 3-D DFT in place: direction -1 = forward (cupy.fft.fftn(norm="ortho"),
+ * sources/sim_state.py:646,677), +1 = inverse (ifftn, :665,696).
+ */
+int wfs_fft3d(wfs_plan* plan, void* data, int direction, void* stream);
+
+/*
+ * Simpson-weighted moments of |psi|^2 over each box, accumulated in FP64.
+ * Replaces volume_probability_kernel (sources/cuda_kernels/volume_probability.cu:4-47)
+ * and expected_location_kernel (expected_location.cu:3-72) as launched by
+ * VolumeProbability.calculate / ExpectedLocation.calculate
+ * (sources/measurement.py:153-175, 354-384).
+ * out (device, 7 doubles per box): P, E[x], E[y], E[z], E[x^2], E[y^2], E[z^2]
+ * (each already multiplied by dx*dy*dz; not divided by P).
+ */
+int wfs_observables(wfs_plan* plan, const void* psi, const wfs_box* boxes, int nboxes, double* out,
+                    void* stream);
+
+/*
+ * Gaussian wave packet (sources/cuda_kernels/gaussian_wave_packet.cu:6-43,
+ * common.cu:565-605), as launched by GaussianWavePacket.init_wave_packet
+ * (sources/wave_packet.py:97-151).  Writes (overwrites) psi.
+ */
+int wfs_init_gaussian(const int n[3], const double dx[3], int precision, const double sigma[3],
+                      const double r0[3], const double k0[3], void* psi, void* stream);
+
+/*
+ * Complex absorbing potential: V += i * pot (absorbing_potential.cu:3-307,
+ * launched by AbsorbingBoundaryCondition.add_potential, sources/potential.py:108-158).
+ */
+int wfs_add_cap(const int n[3], const double dx[3], const double bottom[3], const double top[3], double offset,
+                const double pot_pos[3], const double pot_neg[3], const double exp_pos[3],
+                const double exp_neg[3], void* V, void* stream);
+
+/*
+ * Potential wall with 0-4 slits: V += pot (potential_wall.cu:4-83, launched by
+ * PotentialWall.add_potential, sources/potential.py:201-242).  normal must be
+ * unit length.
+ */
+int wfs_add_wall(const int n[3], const double dx[3], double potential, const double center[3],
+                 const double normal[3], double plateau, double slope, double slope_exponent, int slit_count,
+                 double slit_spacing, double slit_width, double slit_rotation, void* V, void* stream);
+
+/*
+ * Harmonic potential V += 1/2 m w^2 |r - c|^2 (formula of the uncalled helper
+ * sources/potential.py:414-423; exposed here for the harmonic workloads).
+ */
+int wfs_add_harmonic(const int n[3], const double dx[3], double omega, double mass, const double center[3],
+                     void* V, void* stream);
+
+/* cos^2 optical lattice V += depth * sum_d cos^2(pi r_d / period) (no reference counterpart). */
+int wfs_add_lattice(const int n[3], const double dx[3], double depth, double period, void* V, void* stream);
+
+/*
+ * Kinetic half-step operator table out = exp(-i |k|^2 dt / (4 m)) on the full
+ * grid, k = 2*pi*fftfreq.  The intent of operators.init_kinetic_operator
+ * (sources/operators.py:9-34, whose body references undefined names) with the
+ * phase of kinetic_operator.cu:20-22.  out: state precision.
+ */
+int wfs_kinetic_operator(const int n[3], const double dx[3], double dt, double mass, int precision, void* out,
+                         void* stream);
+
+/*
+ * Potential operator table out = cexp_i(-dt V) elementwise (count points).
+ * The intent of operators.init_potential_operator (sources/operators.py:41-55)
+ * with the phase of potential_operator.cu:10-15.  V complex64; out: state precision.
+ */
+int wfs_potential_operator(long count, const void* V, double dt, int precision, void* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WFSIM_H */

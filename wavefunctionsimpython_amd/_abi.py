@@ -1,0 +1,254 @@
+"""ctypes binding of ``libwfsim.so`` (the C ABI declared in ``include/wfsim.h``).
+
+The library is the only compute path of this package.  Loading fails loudly
+when it has not been built (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C wavefunctionsimpython_amd/csrc``); there is no CPU fallback.
+
+Every launch helper checks on the host that tensor devices, dtypes, shapes and
+contiguity match what the kernels and their grids assume before any pointer
+crosses the boundary.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libwfsim.so")
+
+WFS_C64 = 0
+WFS_C128 = 1
+WFS_METHOD_FOURIER = 0
+WFS_METHOD_POWER_SERIES = 1
+SUPPORTED_LENGTHS = (8, 16, 32, 64, 96, 128, 192, 256, 512)
+
+# Every function declared in include/wfsim.h, with its ctypes signature.
+_c_int3 = ctypes.c_int * 3
+_c_dbl3 = ctypes.c_double * 3
+_vp = ctypes.c_void_p
+
+
+class WfsBox(ctypes.Structure):
+    _fields_ = [("bottom", ctypes.c_int * 3), ("count", ctypes.c_int * 3)]
+
+
+SIGNATURES = {
+    "wfs_version": (ctypes.c_char_p, []),
+    "wfs_last_error": (ctypes.c_char_p, []),
+    "wfs_plan_create": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "wfs_plan_destroy": (ctypes.c_int, [_vp]),
+    "wfs_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "wfs_fft3d": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    "wfs_observables": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(WfsBox), ctypes.c_int, _vp, _vp]),
+    "wfs_init_gaussian": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_int, _c_dbl3, _c_dbl3, _c_dbl3, _vp, _vp]),
+    "wfs_add_cap": (ctypes.c_int, [_c_int3, _c_dbl3, _c_dbl3, _c_dbl3, ctypes.c_double, _c_dbl3, _c_dbl3,
+                                   _c_dbl3, _c_dbl3, _vp, _vp]),
+    "wfs_add_wall": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, _c_dbl3, _c_dbl3, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "wfs_add_harmonic": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, _c_dbl3, _vp, _vp]),
+    "wfs_add_lattice": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "wfs_kinetic_operator": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            _vp, _vp]),
+    "wfs_potential_operator": (ctypes.c_int, [ctypes.c_long, _vp, ctypes.c_double, ctypes.c_int, _vp, _vp]),
+}
+
+
+class WfsError(RuntimeError):
+    """A non-zero status returned by the HIP library."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib() -> ctypes.CDLL:
+    """Load libwfsim.so once; raise if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is not built: run `make -C wavefunctionsimpython_amd/csrc` "
+                    "(the package has no CPU fallback)")
+            handle = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().wfs_last_error().decode(errors="replace")
+        raise WfsError(f"{what} failed with status {rc}: {msg}")
+
+
+def i3(v) -> ctypes.Array:
+    return _c_int3(*[int(x) for x in v])
+
+
+def d3(v) -> ctypes.Array:
+    return _c_dbl3(*[float(x) for x in v])
+
+
+def stream_handle(stream=None) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _require_device(t: torch.Tensor, name: str) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be a device tensor, got {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous (C-order)")
+
+
+def check_state(t: torch.Tensor, shape, precision: int, name: str = "psi") -> None:
+    _require_device(t, name)
+    want = torch.complex128 if precision == WFS_C128 else torch.complex64
+    if t.dtype != want:
+        raise TypeError(f"{name} must be {want}, got {t.dtype}")
+    if tuple(t.shape) != tuple(int(s) for s in shape):
+        raise ValueError(f"{name} shape {tuple(t.shape)} != grid {tuple(shape)}")
+
+
+def check_potential(t: torch.Tensor, shape, name: str = "V") -> None:
+    _require_device(t, name)
+    if t.dtype != torch.complex64:
+        raise TypeError(f"{name} must be complex64 (the potential is always complex64), got {t.dtype}")
+    if tuple(t.shape) != tuple(int(s) for s in shape):
+        raise ValueError(f"{name} shape {tuple(t.shape)} != grid {tuple(shape)}")
+
+
+def ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Plan:
+    """Owner of one ``wfs_plan`` (twiddle and kinetic tables, scratch buffers)."""
+
+    def __init__(self, n, dx, dt: float, mass: float, precision: int, method: int, order: int = 10):
+        n = tuple(int(v) for v in n)
+        for v in n:
+            if v not in SUPPORTED_LENGTHS:
+                raise ValueError(f"grid length {v} not supported; choose from {SUPPORTED_LENGTHS}")
+        self.n = n
+        self.dx = tuple(float(v) for v in dx)
+        self.dt = float(dt)
+        self.mass = float(mass)
+        self.precision = int(precision)
+        self.method = int(method)
+        self.order = int(order)
+        self._h = ctypes.c_void_p()
+        check(lib().wfs_plan_create(i3(n), d3(dx), self.dt, self.mass, self.precision, self.method,
+                                    self.order, ctypes.byref(self._h)), "wfs_plan_create")
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def step(self, psi: torch.Tensor, V: torch.Tensor, nsteps: int = 1, stream=None) -> None:
+        check_state(psi, self.n, self.precision)
+        check_potential(V, self.n)
+        if nsteps < 0:
+            raise ValueError("nsteps must be >= 0")
+        check(lib().wfs_step(self._h, ptr(psi), ptr(V), int(nsteps), stream_handle(stream)), "wfs_step")
+
+    def fft3d(self, data: torch.Tensor, direction: int, stream=None) -> None:
+        check_state(data, self.n, self.precision, "data")
+        check(lib().wfs_fft3d(self._h, ptr(data), int(direction), stream_handle(stream)), "wfs_fft3d")
+
+    def observables(self, psi: torch.Tensor, boxes, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Moments of |psi|^2 per box; returns a device float64 tensor (nboxes, 7)."""
+        check_state(psi, self.n, self.precision)
+        nb = len(boxes)
+        arr = (WfsBox * max(nb, 1))()
+        for i, (bottom, count) in enumerate(boxes):
+            for a in range(3):
+                b, c = int(bottom[a]), int(count[a])
+                if b < 0 or c < 1 or b + c > self.n[a]:
+                    raise ValueError(f"box {i} axis {a}: [{b}, {b + c}) lies outside the grid of {self.n[a]}")
+                arr[i].bottom[a] = b
+                arr[i].count[a] = c
+        if out is None:
+            out = torch.empty((nb, 7), dtype=torch.float64, device=psi.device)
+        elif out.dtype != torch.float64 or out.numel() < 7 * nb or not out.is_contiguous() or out.device != psi.device:
+            raise ValueError("out must be a contiguous float64 device tensor with 7 entries per box")
+        if nb:
+            check(lib().wfs_observables(self._h, ptr(psi), arr, nb, ptr(out), stream_handle(stream)),
+                  "wfs_observables")
+        return out
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            lib().wfs_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _grid_numel(n) -> int:
+    return int(n[0]) * int(n[1]) * int(n[2])
+
+
+def init_gaussian(n, dx, precision: int, sigma, r0, k0, psi: torch.Tensor, stream=None) -> None:
+    check_state(psi, n, precision)
+    check(lib().wfs_init_gaussian(i3(n), d3(dx), int(precision), d3(sigma), d3(r0), d3(k0), ptr(psi),
+                                  stream_handle(stream)), "wfs_init_gaussian")
+
+
+def add_cap(n, dx, bottom, top, offset, pot_pos, pot_neg, exp_pos, exp_neg, V: torch.Tensor, stream=None) -> None:
+    check_potential(V, n)
+    check(lib().wfs_add_cap(i3(n), d3(dx), d3(bottom), d3(top), float(offset), d3(pot_pos), d3(pot_neg),
+                            d3(exp_pos), d3(exp_neg), ptr(V), stream_handle(stream)), "wfs_add_cap")
+
+
+def add_wall(n, dx, potential, center, normal, plateau, slope, slope_exponent, slit_count, slit_spacing,
+             slit_width, slit_rotation, V: torch.Tensor, stream=None) -> None:
+    check_potential(V, n)
+    check(lib().wfs_add_wall(i3(n), d3(dx), float(potential), d3(center), d3(normal), float(plateau),
+                             float(slope), float(slope_exponent), int(slit_count), float(slit_spacing),
+                             float(slit_width), float(slit_rotation), ptr(V), stream_handle(stream)),
+          "wfs_add_wall")
+
+
+def add_harmonic(n, dx, omega, mass, center, V: torch.Tensor, stream=None) -> None:
+    check_potential(V, n)
+    check(lib().wfs_add_harmonic(i3(n), d3(dx), float(omega), float(mass), d3(center), ptr(V),
+                                 stream_handle(stream)), "wfs_add_harmonic")
+
+
+def add_lattice(n, dx, depth, period, V: torch.Tensor, stream=None) -> None:
+    check_potential(V, n)
+    check(lib().wfs_add_lattice(i3(n), d3(dx), float(depth), float(period), ptr(V), stream_handle(stream)),
+          "wfs_add_lattice")
+
+
+def kinetic_operator(n, dx, dt, mass, precision: int, out: torch.Tensor, stream=None) -> None:
+    check_state(out, n, precision, "out")
+    check(lib().wfs_kinetic_operator(i3(n), d3(dx), float(dt), float(mass), int(precision), ptr(out),
+                                     stream_handle(stream)), "wfs_kinetic_operator")
+
+
+def potential_operator(V: torch.Tensor, dt, precision: int, out: torch.Tensor, stream=None) -> None:
+    _require_device(V, "V")
+    if V.dtype != torch.complex64:
+        raise TypeError("V must be complex64")
+    check_state(out, tuple(V.shape), precision, "out")
+    check(lib().wfs_potential_operator(int(V.numel()), ptr(V), float(dt), int(precision), ptr(out),
+                                       stream_handle(stream)), "wfs_potential_operator")

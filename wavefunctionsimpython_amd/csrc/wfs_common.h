@@ -1,0 +1,224 @@
+// Shared device helpers for the wfsim HIP library (gfx950).
+//
+// Complex values are interleaved (re, im) pairs stored as HIP float2/double2
+// so that a grid point moves in one 8-byte (complex64) or 16-byte
+// (complex128) access.  Grids are C-order: x slowest, z fastest
+// (sources/cuda_kernels/common.cu:294-299).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <utility>
+
+namespace wfs {
+
+template <typename R> struct cplx_t;
+template <> struct cplx_t<float> { using type = float2; };
+template <> struct cplx_t<double> { using type = double2; };
+template <typename R> using cx = typename cplx_t<R>::type;
+
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> mk(R x, R y) { cx<R> r; r.x = x; r.y = y; return r; }
+
+template <typename T>
+__host__ __device__ __forceinline__ T cadd(T a, T b) { a.x += b.x; a.y += b.y; return a; }
+template <typename T>
+__host__ __device__ __forceinline__ T csub(T a, T b) { a.x -= b.x; a.y -= b.y; return a; }
+template <typename T>
+__host__ __device__ __forceinline__ T cmul(T a, T b) {
+  T r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+// a * conj(b)
+template <typename T>
+__host__ __device__ __forceinline__ T cmulc(T a, T b) {
+  T r;
+  r.x = a.x * b.x + a.y * b.y;
+  r.y = a.y * b.x - a.x * b.y;
+  return r;
+}
+template <typename T, typename S>
+__host__ __device__ __forceinline__ T cscale(T a, S s) { a.x *= s; a.y *= s; return a; }
+
+// ---------------------------------------------------------------------------
+// Compile-time cos/sin of 2*pi*e/M for the in-register DFT twiddles.
+// Exact values are returned for multiples of pi/4; otherwise the argument is
+// reduced to [-pi/4, pi/4] by octant and a Taylor series (error < 1e-17 on
+// that interval after 14 terms) is summed in double.
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kSqrtHalf = 0.707106781186547524400844362104849039;
+
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / double((2 * k) * (2 * k + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / double((2 * k - 1) * (2 * k));
+    sum += term;
+  }
+  return sum;
+}
+// returns {cos, sin} of 2*pi*e/M for 0 <= e < M
+struct cs_pair { double c, s; };
+constexpr cs_pair cos_sin_2pi(long e, long M) {
+  // 8*e/M = octant + frac, angle = (octant + frac) * pi/4
+  long num = 8 * e;           // angle in units of pi/4 = num / M
+  long oct = num / M;         // 0..7
+  long rem = num - oct * M;   // remainder in [0, M)
+  // reduce to the nearest multiple of pi/4: angle = q*pi/4 + d, |d| <= pi/8
+  long q = oct;
+  double d = double(rem) / double(M) * (kPi / 4.0);
+  if (2 * rem > M) { q = oct + 1; d = -double(M - rem) / double(M) * (kPi / 4.0); }
+  q &= 7;
+  const double cq[8] = {1.0, kSqrtHalf, 0.0, -kSqrtHalf, -1.0, -kSqrtHalf, 0.0, kSqrtHalf};
+  const double sq[8] = {0.0, kSqrtHalf, 1.0, kSqrtHalf, 0.0, -kSqrtHalf, -1.0, -kSqrtHalf};
+  if (rem == 0 || 2 * rem == 2 * M) return {cq[q], sq[q]};
+  double cd = taylor_cos(d), sd = taylor_sin(d);
+  return {cq[q] * cd - sq[q] * sd, sq[q] * cd + cq[q] * sd};
+}
+
+// multiply by exp(SIGN * 2*pi*i * E / M), E known at compile time
+template <int M, int E, int SIGN, typename T>
+__device__ __forceinline__ T twiddle_ct(T a) {
+  constexpr int e = ((E % M) + M) % M;
+  using R = decltype(a.x);
+  if constexpr (e == 0) {
+    return a;
+  } else if constexpr (2 * e == M) {
+    a.x = -a.x; a.y = -a.y; return a;
+  } else if constexpr (4 * e == M) {        // exp(SIGN*i*pi/2) = SIGN*i
+    T r;
+    if constexpr (SIGN > 0) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+    return r;
+  } else if constexpr (4 * e == 3 * M) {    // exp(SIGN*3i*pi/2) = -SIGN*i
+    T r;
+    if constexpr (SIGN > 0) { r.x = a.y; r.y = -a.x; } else { r.x = -a.y; r.y = a.x; }
+    return r;
+  } else {
+    constexpr cs_pair cs = cos_sin_2pi(e, M);
+    T w;
+    w.x = R(cs.c);
+    w.y = R(SIGN > 0 ? cs.s : -cs.s);
+    return cmul(a, w);
+  }
+}
+
+// compile-time loop
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// ---------------------------------------------------------------------------
+// In-register DFT of length M, natural order in and out:
+//   a[k] <- sum_n a[n] exp(SIGN * 2*pi*i * n*k / M)
+// Built by decimation in time, M = R0 * Q (R0 in {4, 3, 2}), fully unrolled.
+// ---------------------------------------------------------------------------
+template <int M, int SIGN>
+struct DFT;
+
+template <int SIGN>
+struct DFT<1, SIGN> {
+  template <typename T> __device__ __forceinline__ static void run(T*) {}
+};
+
+template <int SIGN>
+struct DFT<2, SIGN> {
+  template <typename T> __device__ __forceinline__ static void run(T* a) {
+    T t0 = cadd(a[0], a[1]);
+    T t1 = csub(a[0], a[1]);
+    a[0] = t0; a[1] = t1;
+  }
+};
+
+template <int SIGN>
+struct DFT<3, SIGN> {
+  template <typename T> __device__ __forceinline__ static void run(T* a) {
+    using R = decltype(a[0].x);
+    const R c = R(-0.5);
+    const R s = R(0.866025403784438646763723170752936183);  // sqrt(3)/2
+    T t = cadd(a[1], a[2]);
+    T d = csub(a[1], a[2]);
+    T m;  // a0 + c*(a1+a2)
+    m.x = a[0].x + c * t.x;
+    m.y = a[0].y + c * t.y;
+    // SIGN*i*s*(a1 - a2)
+    T q;
+    if constexpr (SIGN > 0) { q.x = -s * d.y; q.y = s * d.x; } else { q.x = s * d.y; q.y = -s * d.x; }
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, q);
+    a[2] = csub(m, q);
+  }
+};
+
+template <int SIGN>
+struct DFT<4, SIGN> {
+  template <typename T> __device__ __forceinline__ static void run(T* a) {
+    T s02 = cadd(a[0], a[2]);
+    T d02 = csub(a[0], a[2]);
+    T s13 = cadd(a[1], a[3]);
+    T d13 = csub(a[1], a[3]);
+    // SIGN*i*d13
+    T j;
+    if constexpr (SIGN > 0) { j.x = -d13.y; j.y = d13.x; } else { j.x = d13.y; j.y = -d13.x; }
+    a[0] = cadd(s02, s13);
+    a[2] = csub(s02, s13);
+    a[1] = cadd(d02, j);
+    a[3] = csub(d02, j);
+  }
+};
+
+template <int M>
+struct radix_of {
+  static constexpr int value = (M % 4 == 0) ? 4 : (M % 3 == 0) ? 3 : 2;
+};
+
+template <int M, int SIGN>
+struct DFT {
+  static constexpr int R0 = radix_of<M>::value;
+  static constexpr int Q = M / R0;
+  static_assert(R0 * Q == M, "unsupported DFT length");
+  template <typename T> __device__ __forceinline__ static void run(T* a) {
+    T y[R0][Q];
+    static_for<R0>([&](auto r) {
+      static_for<Q>([&](auto q) { y[r][q] = a[R0 * q + r]; });
+      DFT<Q, SIGN>::run(y[r]);
+    });
+    static_for<Q>([&](auto k1) {
+      T t[R0];
+      static_for<R0>([&](auto r) {
+        t[r] = twiddle_ct<M, decltype(r)::value * decltype(k1)::value, SIGN>(y[r][k1]);
+      });
+      DFT<R0, SIGN>::run(t);
+      static_for<R0>([&](auto k2) { a[k1 + Q * k2] = t[k2]; });
+    });
+  }
+};
+
+// error reporting shared by all translation units
+void set_last_error(const char* fmt, ...);
+
+}  // namespace wfs
+
+#define WFS_HIP_CHECK(expr)                                                    \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      ::wfs::set_last_error("%s failed: %s (%s:%d)", #expr,                    \
+                            hipGetErrorString(_e), __FILE__, __LINE__);        \
+      return int(_e);                                                          \
+    }                                                                          \
+  } while (0)

@@ -1,0 +1,236 @@
+// Axis passes of the 3-D FFT with fused phase multiplies (gfx950).
+//
+// A length-N transform along one axis is split N = N1 * N2 (N1 % N2 == 0):
+//   X[k1 + N1 k2] = sum_n2 W_N2^{n2 k2} W_N^{n2 k1} sum_n1 x[N2 n1 + n2] W_N1^{n1 k1}
+// Thread t (0 <= t < N2) of a column holds a[i] = x[t + N2 i], i < N1, does the
+// inner length-N1 DFT in registers, multiplies by the table twiddle W_N^{t k1},
+// exchanges through LDS, and finishes with N1/N2 length-N2 DFTs.  With the
+// outputs written back to a[m + (N1/N2) k2] the result satisfies
+// a[i] = X[t + N2 i]: the register layout of the output equals that of the
+// input, so a forward transform, a pointwise phase and an inverse transform
+// chain in registers with no extra memory traffic.
+//
+// Passes (one kernel launch each, in place, unnormalised transforms):
+//   ROWS layout  (contiguous z axis):  FWD, INV, IVF = inverse -> exp(-i dt V) -> forward
+//   COLS layout  (strided x / y axes): FWD, INV, FKI = forward -> K(kx,ky,kz) -> inverse
+// where K holds the kinetic phase and the 1/(Nx Ny Nz) normalisation of the
+// round trip.  The reference performs the same arithmetic as four cuFFT calls
+// and three elementwise kernels per step (sources/sim_state.py:641-697).
+#include "wfs_internal.h"
+
+namespace wfs {
+
+template <typename R, int N, int N2, int LAYOUT, int MODE, int G>
+__global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ data, const PassParams p) {
+  constexpr int N1 = N / N2;
+  static_assert(N1 * N2 == N && N1 % N2 == 0, "bad split");
+  constexpr int M = N1 / N2;            // length-N2 DFTs per thread in step 2
+  using C = cx<R>;
+  // LDS exchange buffer: element (k1, n2) of group g
+  constexpr int ROW_STRIDE = N1 * (N2 + 1);   // padded: conflict-free step-2 reads
+  constexpr int LDS_ELEMS = (LAYOUT == LAYOUT_ROWS) ? G * ROW_STRIDE : G * N;
+  __shared__ C lds[LDS_ELEMS];
+
+  const int tid = threadIdx.x;
+  int t, g;
+  if constexpr (LAYOUT == LAYOUT_ROWS) { t = tid % N2; g = tid / N2; }
+  else { g = tid % G; t = tid / G; }
+
+  long base;
+  long stride;
+  bool active;
+  long col = 0;  // COLS: column index inside one outer slab
+  if constexpr (LAYOUT == LAYOUT_ROWS) {
+    const long row = long(blockIdx.x) * G + g;
+    active = row < p.rows;
+    base = row * N;
+    stride = 1;
+    col = row;
+  } else {
+    col = long(blockIdx.x) * G + g;
+    active = col < p.inner;
+    base = long(blockIdx.y) * N * p.inner + col;
+    stride = p.inner;
+  }
+
+  auto lidx = [&](int k1, int n2) -> int {
+    if constexpr (LAYOUT == LAYOUT_ROWS) return g * ROW_STRIDE + k1 * (N2 + 1) + n2;
+    else return (k1 * N2 + n2) * G + g;
+  };
+
+  const C* __restrict__ tw = reinterpret_cast<const C*>(p.tw);
+  C w[N1];
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(t * k1) % N];
+
+  C a[N1];
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[i] = data[base + long(t + N2 * i) * stride];
+  } else {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[i] = mk<R>(R(0), R(0));
+  }
+
+  // one length-N transform of the column held in a[] (SIGN -1 forward, +1 inverse)
+  auto transform = [&](auto sign_c) {
+    constexpr int SIGN = decltype(sign_c)::value;
+    DFT<N1, SIGN>::run(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = (SIGN < 0) ? cmul(a[k1], w[k1]) : cmulc(a[k1], w[k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) lds[lidx(k1, t)] = a[k1];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      C b[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[n2] = lds[lidx(t + N2 * m, n2)];
+      DFT<N2, SIGN>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) a[m + M * k2] = b[k2];
+    }
+  };
+
+  if constexpr (MODE == MODE_FWD || MODE == MODE_INV) {
+    transform(std::integral_constant<int, (MODE == MODE_FWD) ? -1 : +1>{});
+    if (p.scale != 1.0) {
+      const R sc = R(p.scale);
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a[i] = cscale(a[i], sc);
+    }
+  } else if constexpr (MODE == MODE_FKI) {
+    // x pass in full k-space: column = (ky, kz)
+    transform(std::integral_constant<int, -1>{});
+    const C* __restrict__ kx = reinterpret_cast<const C*>(p.kx);
+    const C* __restrict__ ky = reinterpret_cast<const C*>(p.ky);
+    const C* __restrict__ kz = reinterpret_cast<const C*>(p.kz);
+    const long cc = active ? col : 0;
+    const C kyz = cmul(ky[cc / p.nz], kz[cc % p.nz]);
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], cmul(kx[t + N2 * i], kyz));
+    __syncthreads();
+    transform(std::integral_constant<int, +1>{});
+  } else if constexpr (MODE == MODE_IVF) {
+    // z pass at the turning point: inverse, potential phase, forward
+    transform(std::integral_constant<int, +1>{});
+    const float2* __restrict__ V = p.V;
+    const R dt = R(p.dt);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        const float2 v = V[base + t + N2 * i];
+        // cexp_i(-dt*V) = (cos(-dt Vr), sin(-dt Vr)) * exp(dt Vi)
+        //   sources/cuda_kernels/common.cu:98-111, potential_operator.cu:10-15
+        R s, c;
+        const R ang = -dt * R(v.x);
+        if constexpr (sizeof(R) == 4) sincosf(ang, &s, &c); else sincos(ang, &s, &c);
+        const R amp = (sizeof(R) == 4) ? R(expf(float(dt * R(v.y)))) : R(exp(double(dt * R(v.y))));
+        a[i] = cmul(a[i], mk<R>(c * amp, s * amp));
+      }
+    }
+    __syncthreads();
+    transform(std::integral_constant<int, -1>{});
+  }
+
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) data[base + long(t + N2 * i) * stride] = a[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side dispatch
+// ---------------------------------------------------------------------------
+
+template <int N> struct split_of;
+template <> struct split_of<8>   { static constexpr int n2 = 2; };
+template <> struct split_of<16>  { static constexpr int n2 = 4; };
+template <> struct split_of<32>  { static constexpr int n2 = 4; };
+template <> struct split_of<64>  { static constexpr int n2 = 8; };
+template <> struct split_of<96>  { static constexpr int n2 = 4; };
+template <> struct split_of<128> { static constexpr int n2 = 8; };
+template <> struct split_of<192> { static constexpr int n2 = 8; };
+template <> struct split_of<256> { static constexpr int n2 = 16; };
+template <> struct split_of<512> { static constexpr int n2 = 16; };
+
+// Columns (COLS) or rows (ROWS) per block: about 256 threads, with the LDS
+// exchange buffer kept within 64 KiB so that at least two blocks fit a CU.
+template <typename R, int N, int N2, int LAYOUT>
+constexpr int pick_groups() {
+  int g = 256 / N2;
+  if (g > 64) g = 64;
+  auto lds = [](int gg) {
+    const long e = (LAYOUT == LAYOUT_ROWS) ? long(gg) * (N / N2) * (N2 + 1) : long(gg) * N;
+    return e * long(sizeof(cx<R>));
+  };
+  while (g > 4 && lds(g) > 65536) g /= 2;
+  return g;
+}
+
+template <typename R, int N, int LAYOUT, int MODE>
+static int launch_n(void* data, const PassParams& p, long outer, int /*threads*/, hipStream_t s) {
+  constexpr int N2 = split_of<N>::n2;
+  constexpr int G = pick_groups<R, N, N2, LAYOUT>();
+  const long ncols = (LAYOUT == LAYOUT_ROWS) ? p.rows : p.inner;
+  dim3 grid(unsigned((ncols + G - 1) / G), unsigned(LAYOUT == LAYOUT_ROWS ? 1 : outer), 1);
+  hipLaunchKernelGGL((fft_pass_kernel<R, N, N2, LAYOUT, MODE, G>), grid, dim3(N2 * G), 0, s,
+                     reinterpret_cast<cx<R>*>(data), p);
+  return int(hipGetLastError());
+}
+
+template <typename R, int LAYOUT, int MODE>
+static int launch_mode(int n, void* data, const PassParams& p, long outer, int threads, hipStream_t s) {
+  switch (n) {
+    case 8:   return launch_n<R, 8, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 16:  return launch_n<R, 16, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 32:  return launch_n<R, 32, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 64:  return launch_n<R, 64, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 96:  return launch_n<R, 96, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 128: return launch_n<R, 128, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 192: return launch_n<R, 192, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 256: return launch_n<R, 256, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 512: return launch_n<R, 512, LAYOUT, MODE>(data, p, outer, threads, s);
+    default:
+      set_last_error("unsupported FFT length %d (supported: 8,16,32,64,96,128,192,256,512)", n);
+      return WFS_ERR_UNSUPPORTED;
+  }
+}
+
+template <typename R>
+int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams& p, long outer, int threads,
+                      hipStream_t s) {
+  if (layout == LAYOUT_ROWS) {
+    if (mode == MODE_FWD) return launch_mode<R, LAYOUT_ROWS, MODE_FWD>(n, data, p, outer, threads, s);
+    if (mode == MODE_INV) return launch_mode<R, LAYOUT_ROWS, MODE_INV>(n, data, p, outer, threads, s);
+    if (mode == MODE_IVF) return launch_mode<R, LAYOUT_ROWS, MODE_IVF>(n, data, p, outer, threads, s);
+  } else {
+    if (mode == MODE_FWD) return launch_mode<R, LAYOUT_COLS, MODE_FWD>(n, data, p, outer, threads, s);
+    if (mode == MODE_INV) return launch_mode<R, LAYOUT_COLS, MODE_INV>(n, data, p, outer, threads, s);
+    if (mode == MODE_FKI) return launch_mode<R, LAYOUT_COLS, MODE_FKI>(n, data, p, outer, threads, s);
+  }
+  set_last_error("invalid FFT pass layout/mode %d/%d", layout, mode);
+  return WFS_ERR_INVALID;
+}
+
+// This file is compiled twice: WFS_FFT_PREC=0 (complex64) and 1 (complex128).
+#if WFS_FFT_PREC == 0
+template int launch_fft_pass_t<float>(int, int, int, void*, const PassParams&, long, int, hipStream_t);
+
+bool fft_length_supported(int n) {
+  switch (n) {
+    case 8: case 16: case 32: case 64: case 96: case 128: case 192: case 256: case 512: return true;
+    default: return false;
+  }
+}
+
+int launch_fft_pass(int precision, int n, int layout, int mode, void* data, const PassParams& p, long outer,
+                    int threads, hipStream_t s) {
+  if (precision == WFS_C128) return launch_fft_pass_t<double>(n, layout, mode, data, p, outer, threads, s);
+  return launch_fft_pass_t<float>(n, layout, mode, data, p, outer, threads, s);
+}
+#else
+template int launch_fft_pass_t<double>(int, int, int, void*, const PassParams&, long, int, hipStream_t);
+#endif
+
+}  // namespace wfs

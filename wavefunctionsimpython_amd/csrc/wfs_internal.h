@@ -1,0 +1,43 @@
+// Internal declarations shared by the wfsim translation units.
+#pragma once
+
+#include "wfs_common.h"
+#include "../../include/wfsim.h"
+
+namespace wfs {
+
+enum { LAYOUT_ROWS = 0, LAYOUT_COLS = 1 };
+enum { MODE_FWD = 0, MODE_INV = 1, MODE_FKI = 2, MODE_IVF = 3 };
+
+// Parameters of one axis pass (see wfs_fft.hip).
+struct PassParams {
+  const void* tw = nullptr;     // forward twiddles exp(-2 pi i j / N), j < N, in the state precision
+  long rows = 0;                // ROWS layout: number of contiguous rows of length N
+  long inner = 0;               // COLS layout: stride of the axis = columns per outer slab
+  const void* kx = nullptr;     // FKI: per-axis kinetic factors (normalisation folded in)
+  const void* ky = nullptr;
+  const void* kz = nullptr;
+  long nz = 1;                  // FKI: column index -> (ky, kz) = (c / nz, c % nz)
+  const float2* V = nullptr;    // IVF: complex64 potential, same layout as the state
+  double dt = 0.0;              // IVF: time step
+  double scale = 1.0;           // FWD/INV: output scale (1 = none)
+};
+
+bool fft_length_supported(int n);
+template <typename R>
+int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams& p, long outer, int threads,
+                      hipStream_t s);
+int launch_fft_pass(int precision, int n, int layout, int mode, void* data, const PassParams& p, long outer,
+                    int threads, hipStream_t s);
+
+// power series: one Horner stage  u_out = psi + (-i dt / n) H u_in
+int launch_horner_stage(int precision, int nx, int ny, int nz, const void* u_in, void* u_out, const void* psi,
+                        const float2* V, double dt_over_n, double inv_2m, double inv_dx2, int tile_x,
+                        hipStream_t s);
+
+// observables over one box: writes 7 doubles to out (device)
+int launch_box_moments(int precision, int nx, int ny, int nz, const double dx[3], const void* psi,
+                       const wfs_box& box, double* partials, long max_partials, double* out, hipStream_t s);
+long box_moments_partials_needed(const wfs_box& box);
+
+}  // namespace wfs

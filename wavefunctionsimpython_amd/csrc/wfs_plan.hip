@@ -1,0 +1,287 @@
+// Plans, the time-step driver and the remaining C ABI entry points.
+#include "wfs_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+struct wfs_plan {
+  int n[3];
+  double dx[3];
+  double dt, mass;
+  int precision, method, order;
+  size_t elem;            // bytes per state element
+  void* tw[3];            // forward twiddles per axis, state precision
+  void* kfull[3];         // exp(-i k_d^2 dt / 2m) / N_d
+  void* khalf[3];         // exp(-i k_d^2 dt / 4m) / N_d
+  void* scratch[2];       // power-series ping-pong buffers (lazy)
+  double* obs_work;       // observable partial records
+  long obs_work_len;
+  int threads_rows;       // threads per block of the contiguous-axis pass
+  int threads_cols;       // threads per block of the strided-axis passes
+};
+
+namespace wfs {
+
+static thread_local char g_last_error[512] = "";
+
+void set_last_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+namespace {
+
+// numpy.fft.fftfreq index -> signed frequency index
+long freq_index(long j, long n) { return (j <= (n - 1) / 2) ? j : j - n; }
+
+int upload(void** dst, const void* host, size_t bytes) {
+  hipError_t e = hipMalloc(dst, bytes);
+  if (e != hipSuccess) { set_last_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e)); return WFS_ERR_ALLOC; }
+  e = hipMemcpy(*dst, host, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { set_last_error("hipMemcpy failed: %s", hipGetErrorString(e)); return int(e); }
+  return 0;
+}
+
+// complex table in the state precision from FP64 (re, im) pairs
+int upload_table(void** dst, const std::vector<double>& re, const std::vector<double>& im, int precision) {
+  const size_t n = re.size();
+  if (precision == WFS_C128) {
+    std::vector<double2> h(n);
+    for (size_t j = 0; j < n; ++j) { h[j].x = re[j]; h[j].y = im[j]; }
+    return upload(dst, h.data(), n * sizeof(double2));
+  }
+  std::vector<float2> h(n);
+  for (size_t j = 0; j < n; ++j) { h[j].x = float(re[j]); h[j].y = float(im[j]); }
+  return upload(dst, h.data(), n * sizeof(float2));
+}
+
+// cos/sin(2 pi j / n) with exact values at multiples of pi/2 (long double)
+void unit_root(long j, long n, double& c, double& s) {
+  j %= n;
+  if ((4 * j) % n == 0) {
+    const long q = (4 * j) / n;
+    const double cq[4] = {1.0, 0.0, -1.0, 0.0}, sq[4] = {0.0, 1.0, 0.0, -1.0};
+    c = cq[q]; s = sq[q];
+    return;
+  }
+  const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+  c = double(cosl(a));
+  s = double(sinl(a));
+}
+
+// exp(-i k^2 dt / (f m)) / n with the phase reduced modulo 2 pi in long double
+void kinetic_table(long n, double dx, double dt, double mass, double f, std::vector<double>& re,
+                   std::vector<double>& im) {
+  re.resize(n); im.resize(n);
+  const long double two_pi = 2.0L * 3.14159265358979323846264338327950288L;
+  for (long j = 0; j < n; ++j) {
+    const long double k = two_pi * (long double)freq_index(j, n) / ((long double)n * (long double)dx);
+    long double th = -k * k * (long double)dt / ((long double)f * (long double)mass);
+    th = fmodl(th, two_pi);
+    re[j] = double(cosl(th) / (long double)n);
+    im[j] = double(sinl(th) / (long double)n);
+  }
+}
+
+int ensure_scratch(wfs_plan* p) {
+  if (p->scratch[0]) return 0;
+  const size_t bytes = size_t(p->n[0]) * p->n[1] * p->n[2] * p->elem;
+  for (int i = 0; i < 2; ++i) {
+    hipError_t e = hipMalloc(&p->scratch[i], bytes);
+    if (e != hipSuccess) { set_last_error("hipMalloc(%zu) for power-series scratch failed", bytes); return WFS_ERR_ALLOC; }
+  }
+  return 0;
+}
+
+// one axis pass; axis 0 = x (strided, whole grid), 1 = y (strided per x slab), 2 = z (rows)
+int run_pass(const wfs_plan* p, int axis, int mode, void* data, const void* V, const void* const* ktab,
+             double scale, hipStream_t s) {
+  PassParams pp;
+  pp.tw = p->tw[axis];
+  pp.scale = scale;
+  long outer = 1;
+  int layout;
+  if (axis == 2) {
+    layout = LAYOUT_ROWS;
+    pp.rows = long(p->n[0]) * p->n[1];
+  } else if (axis == 1) {
+    layout = LAYOUT_COLS;
+    outer = p->n[0];
+    pp.inner = p->n[2];
+  } else {
+    layout = LAYOUT_COLS;
+    outer = 1;
+    pp.inner = long(p->n[1]) * p->n[2];
+  }
+  if (mode == MODE_FKI) {
+    pp.kx = ktab[0]; pp.ky = ktab[1]; pp.kz = ktab[2];
+    pp.nz = p->n[2];
+  }
+  if (mode == MODE_IVF) {
+    pp.V = static_cast<const float2*>(V);
+    pp.dt = p->dt;
+  }
+  const int threads = (layout == LAYOUT_ROWS) ? p->threads_rows : p->threads_cols;
+  return launch_fft_pass(p->precision, p->n[axis], layout, mode, data, pp, outer, threads, s);
+}
+
+#define WFS_TRY(expr)              \
+  do {                             \
+    int _rc = (expr);              \
+    if (_rc != 0) return _rc;      \
+  } while (0)
+
+int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
+  const void* const* kh = p->khalf;
+  const void* const* kf = p->kfull;
+  WFS_TRY(run_pass(p, 2, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
+  WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
+  WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, kh, 1.0, s));
+  for (int step = 1; step <= nsteps; ++step) {
+    WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s));
+    WFS_TRY(run_pass(p, 2, MODE_IVF, psi, V, nullptr, 1.0, s));
+    WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
+    WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, step == nsteps ? kh : kf, 1.0, s));
+  }
+  WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s));
+  WFS_TRY(run_pass(p, 2, MODE_INV, psi, nullptr, nullptr, 1.0, s));
+  return 0;
+}
+
+int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
+  WFS_TRY(ensure_scratch(p));
+  const double inv_2m = 1.0 / (2.0 * p->mass);
+  const double inv_dx2 = 1.0 / (p->dx[0] * p->dx[0]);   // x spacing on every axis (power_series_operator.cu:58)
+  const size_t bytes = size_t(p->n[0]) * p->n[1] * p->n[2] * p->elem;
+  for (int step = 0; step < nsteps; ++step) {
+    const void* u_in = psi;
+    int which = 0;
+    for (int nn = p->order; nn >= 1; --nn) {
+      void* out = (nn == 1 && u_in != psi) ? psi : p->scratch[which];
+      WFS_TRY(launch_horner_stage(p->precision, p->n[0], p->n[1], p->n[2], u_in, out, psi,
+                                  static_cast<const float2*>(V), p->dt / double(nn), inv_2m, inv_dx2, 0, s));
+      u_in = out;
+      which ^= 1;
+    }
+    if (u_in != psi) WFS_HIP_CHECK(hipMemcpyAsync(psi, u_in, bytes, hipMemcpyDeviceToDevice, s));
+  }
+  return 0;
+}
+
+}  // namespace
+}  // namespace wfs
+
+using namespace wfs;
+
+extern "C" const char* wfs_version(void) { return "wfsim-gfx950 0.1.0"; }
+
+extern "C" const char* wfs_last_error(void) { return g_last_error; }
+
+extern "C" int wfs_plan_destroy(wfs_plan* p) {
+  if (!p) return 0;
+  for (int a = 0; a < 3; ++a) {
+    if (p->tw[a]) (void)hipFree(p->tw[a]);
+    if (p->kfull[a]) (void)hipFree(p->kfull[a]);
+    if (p->khalf[a]) (void)hipFree(p->khalf[a]);
+  }
+  for (int i = 0; i < 2; ++i)
+    if (p->scratch[i]) (void)hipFree(p->scratch[i]);
+  if (p->obs_work) (void)hipFree(p->obs_work);
+  delete p;
+  return 0;
+}
+
+extern "C" int wfs_plan_create(const int n[3], const double dx[3], double dt, double mass, int precision,
+                               int method, int order, wfs_plan** out) {
+  if (!out || !n || !dx) { set_last_error("wfs_plan_create: null argument"); return WFS_ERR_INVALID; }
+  *out = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    if (!fft_length_supported(n[a])) {
+      set_last_error("wfs_plan_create: axis %d length %d not supported (8,16,32,64,96,128,192,256,512)", a, n[a]);
+      return WFS_ERR_UNSUPPORTED;
+    }
+    if (!(dx[a] > 0.0)) { set_last_error("wfs_plan_create: dx[%d] must be positive", a); return WFS_ERR_INVALID; }
+  }
+  if (precision != WFS_C64 && precision != WFS_C128) { set_last_error("wfs_plan_create: bad precision %d", precision); return WFS_ERR_INVALID; }
+  if (method != WFS_METHOD_FOURIER && method != WFS_METHOD_POWER_SERIES) { set_last_error("wfs_plan_create: bad method %d", method); return WFS_ERR_INVALID; }
+  if (method == WFS_METHOD_POWER_SERIES && order < 1) { set_last_error("wfs_plan_create: order must be >= 1"); return WFS_ERR_INVALID; }
+  if (!(mass > 0.0)) { set_last_error("wfs_plan_create: mass must be positive"); return WFS_ERR_INVALID; }
+
+  wfs_plan* p = new wfs_plan();
+  std::memset(p, 0, sizeof(*p));
+  for (int a = 0; a < 3; ++a) { p->n[a] = n[a]; p->dx[a] = dx[a]; }
+  p->dt = dt; p->mass = mass; p->precision = precision; p->method = method; p->order = order;
+  p->elem = precision == WFS_C128 ? 16 : 8;
+  p->threads_rows = 256;
+  p->threads_cols = 256;
+  int rc = 0;
+  for (int a = 0; a < 3 && rc == 0; ++a) {
+    std::vector<double> re(n[a]), im(n[a]);
+    for (long j = 0; j < n[a]; ++j) {
+      double c, s;
+      unit_root(j, n[a], c, s);
+      re[j] = c; im[j] = -s;   // forward twiddle exp(-2 pi i j / N)
+    }
+    rc = upload_table(&p->tw[a], re, im, precision);
+    if (rc) break;
+    kinetic_table(n[a], dx[a], dt, mass, 2.0, re, im);
+    rc = upload_table(&p->kfull[a], re, im, precision);
+    if (rc) break;
+    kinetic_table(n[a], dx[a], dt, mass, 4.0, re, im);
+    rc = upload_table(&p->khalf[a], re, im, precision);
+  }
+  if (rc == 0) {
+    wfs_box whole{{0, 0, 0}, {n[0], n[1], n[2]}};
+    p->obs_work_len = box_moments_partials_needed(whole);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->obs_work), size_t(p->obs_work_len) * sizeof(double));
+    if (e != hipSuccess) { set_last_error("hipMalloc for observable workspace failed"); rc = WFS_ERR_ALLOC; }
+  }
+  if (rc) { wfs_plan_destroy(p); return rc; }
+  *out = p;
+  return 0;
+}
+
+extern "C" int wfs_step(wfs_plan* p, void* psi, const void* V, int nsteps, void* stream) {
+  if (!p || !psi || !V) { set_last_error("wfs_step: null argument"); return WFS_ERR_INVALID; }
+  if (nsteps < 0) { set_last_error("wfs_step: negative step count"); return WFS_ERR_INVALID; }
+  if (nsteps == 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p->method == WFS_METHOD_FOURIER) return fourier_steps(p, psi, V, nsteps, s);
+  return power_series_steps(p, psi, V, nsteps, s);
+}
+
+extern "C" int wfs_fft3d(wfs_plan* p, void* data, int direction, void* stream) {
+  if (!p || !data || (direction != -1 && direction != 1)) { set_last_error("wfs_fft3d: bad argument"); return WFS_ERR_INVALID; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int mode = direction < 0 ? MODE_FWD : MODE_INV;
+  for (int axis = 2; axis >= 0; --axis) {
+    const int rc = run_pass(p, axis, mode, data, nullptr, nullptr, 1.0 / std::sqrt(double(p->n[axis])), s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int wfs_observables(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, double* out,
+                               void* stream) {
+  if (!p || !psi || (nboxes > 0 && (!boxes || !out)) || nboxes < 0) { set_last_error("wfs_observables: bad argument"); return WFS_ERR_INVALID; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int b = 0; b < nboxes; ++b) {
+    const wfs_box& bx = boxes[b];
+    for (int a = 0; a < 3; ++a) {
+      if (bx.bottom[a] < 0 || bx.count[a] < 1 || long(bx.bottom[a]) + bx.count[a] > p->n[a]) {
+        set_last_error("wfs_observables: box %d axis %d [%d, %d) outside the grid of %d voxels", b, a,
+                       bx.bottom[a], bx.bottom[a] + bx.count[a], p->n[a]);
+        return WFS_ERR_INVALID;
+      }
+    }
+    const int rc = launch_box_moments(p->precision, p->n[0], p->n[1], p->n[2], p->dx, psi, bx, p->obs_work,
+                                      p->obs_work_len, out + 7 * b, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
