@@ -1,0 +1,237 @@
+"""Throughput benchmark of the propagation path on one MI355X.
+
+Workload (the 256^3 double-slit configuration): 256^3 complex64 state in an
+80 Bohr box (dx = 0.3125); a 5 Ha wall, 2 Bohr thick, with two 1.5 Bohr slits
+6 Bohr apart; a cubic complex absorbing layer of 8 Bohr outside the +-32 Bohr
+observation box (outer potential -100 Ha, the reference default); Gaussian
+packet sigma = 4 Bohr at x = -16 Bohr with k0 = (2, 0, 0); dt = 0.005.
+Observables (norm, <r>, sigma_r over the observation box and the two half
+boxes x < 0, x > 0) every 10 steps.  One "step" is one time step of the
+split-step Fourier method over the whole grid; inputs are resident in HBM
+before the timed region starts.
+
+Prints one JSON line with value = grid-point-steps per second (all GPUs),
+the roofline of the dominant kernel (HIP events around every launch of a
+separate instrumented window on the same stream) and a CPU baseline (the
+complex128 oracle of the reference's 4-FFT step, timed on a few steps on the
+host cores).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+N = 256
+L = 80.0
+DT = 0.005
+OBS_EVERY = 10
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_POINT = {      # algorithmic HBM bytes per grid point per launch, complex64 state
+    "z_fwd": 16, "z_inv": 16, "z_inv_v_fwd": 24, "y_fwd": 16, "y_inv": 16,
+    "x_fwd_k_inv": 16, "x_fwd": 16, "x_inv": 16, "horner_stage": 32,
+}
+
+
+def build_workload(dev, n=N):
+    import torch
+    from wavefunctionsimpython_amd import _abi
+    shape = (n, n, n)
+    dx = (L / n,) * 3
+    psi = torch.empty(shape, dtype=torch.complex64, device=dev)
+    _abi.init_gaussian(shape, dx, _abi.WFS_C64, (4.0, 4.0, 4.0), (-16.0, 0.0, 0.0), (2.0, 0.0, 0.0), psi)
+    V = torch.zeros(shape, dtype=torch.complex64, device=dev)
+    _abi.add_cap(shape, dx, (-32.0,) * 3, (32.0,) * 3, 0.0, (-100.0,) * 3, (-100.0,) * 3, (3.0,) * 3, (3.0,) * 3, V)
+    _abi.add_wall(shape, dx, 5.0, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), 2.0, 0.0, 1.0, 2, 6.0, 1.5, 0.0, V)
+    # observation box voxels (int truncation of (p + L/2)/dx), odd Simpson counts
+    b = int((-32.0 + L / 2) / dx[0])
+    t = int((32.0 + L / 2) / dx[0])
+    cnt = t + 1 - b
+    cnt -= (cnt + 1) % 2
+    mid = n // 2
+    boxes = [((b, b, b), (cnt, cnt, cnt)),
+             ((b, b, b), (mid - b - (mid - b + 1) % 2, cnt, cnt)),
+             ((mid, b, b), (cnt - (mid - b) - ((cnt - (mid - b)) + 1) % 2, cnt, cnt))]
+    torch.cuda.synchronize()
+    return shape, dx, psi, V, boxes
+
+
+def run_chunks(plan, psi, V, boxes, steps, out):
+    k = 0
+    chunk = 0
+    while k < steps:
+        s = min(OBS_EVERY, steps - k)
+        plan.observables(psi, boxes, out=out[chunk % out.shape[0]])
+        plan.step(psi, V, s)
+        k += s
+        chunk += 1
+
+
+def gpu_bench(steps, warmup):
+    import torch
+    from wavefunctionsimpython_amd import _abi
+    dev = torch.device("cuda:0")
+    shape, dx, psi, V, boxes = build_workload(dev)
+    plan = _abi.Plan(shape, dx, DT, 1.0, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER)
+    out = torch.zeros((64, len(boxes), 7), dtype=torch.float64, device=dev)
+    run_chunks(plan, psi, V, boxes, warmup, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_chunks(plan, psi, V, boxes, steps, out)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    norm = float(out[0, 0, 0].item())
+
+    # instrumented window: HIP events around every launch, same stream, same workload
+    plan.set_profiling(True)
+    run_chunks(plan, psi, V, boxes, 40, out)
+    torch.cuda.synchronize()
+    prof = plan.profile(reset=True)
+    plan.set_profiling(False)
+    kernels = {}
+    npts = shape[0] * shape[1] * shape[2]
+    for name, (launches, ms) in prof.items():
+        avg_ms = ms / launches
+        rec = {"launches": launches, "avg_us": round(avg_ms * 1e3, 2)}
+        if name in BYTES_PER_POINT:
+            gbs = BYTES_PER_POINT[name] * npts / (avg_ms * 1e-3) / 1e9
+            rec["algorithmic_bytes"] = BYTES_PER_POINT[name] * npts
+            rec["achieved_GBs"] = round(gbs, 1)
+        kernels[name] = rec
+    dominant = max((k for k in prof if k in BYTES_PER_POINT), key=lambda k: prof[k][1])
+
+    # power series, p = 10, same grid and potential
+    ps_plan = _abi.Plan(shape, dx, DT, 1.0, _abi.WFS_C64, _abi.WFS_METHOD_POWER_SERIES, 10)
+    ps_psi = psi.clone()
+    ps_plan.step(ps_psi, V, 2)
+    torch.cuda.synchronize()
+    ps_steps = max(4, steps // 20)
+    t2 = time.perf_counter()
+    ps_plan.step(ps_psi, V, ps_steps)
+    torch.cuda.synchronize()
+    ps_elapsed = time.perf_counter() - t2
+    ps_plan.set_profiling(True)
+    ps_plan.step(ps_psi, V, 2)
+    torch.cuda.synchronize()
+    ps_prof = ps_plan.profile(reset=True)
+    hl, hms = ps_prof["horner_stage"]
+    power_series = {
+        "value": npts * ps_steps / ps_elapsed, "unit": "grid-pt-steps/s", "order": 10,
+        "ms_per_step": ps_elapsed / ps_steps * 1e3,
+        "horner_stage_avg_us": round(hms / hl * 1e3, 2),
+        "horner_stage_GBs": round(32 * npts / (hms / hl * 1e-3) / 1e9, 1),
+    }
+    return {
+        "elapsed": elapsed, "npts": npts, "norm": norm, "kernels": kernels, "dominant": dominant,
+        "power_series": power_series, "V_host": V.cpu().numpy(),
+    }
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_256.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == f"{kernel}_bytes_per_launch":
+                try:
+                    return float(parts[1])
+                except ValueError:
+                    return None
+    return None
+
+
+def cpu_baseline(V, seconds_target=15.0):
+    """Complex128 oracle of the reference's 4-FFT split-step on the host cores.
+
+    V is the workload's complex64 potential (host copy); the initial state is
+    the same Gaussian packet evaluated by the oracle.
+    """
+    import numpy as np
+    import scipy
+    import oracle
+    cores = len(os.sched_getaffinity(0))
+    n = N
+    shape = (n, n, n)
+    dx = (L / n,) * 3
+    psi = oracle.gaussian_wave_packet(shape, dx, (4, 4, 4), (-16, 0, 0), (2, 0, 0), np.complex128)
+    phases = (oracle.kinetic_half_phase(shape, dx, DT, 1.0), oracle.potential_phase(V, DT))
+    psi = oracle.split_step(psi, V, dx, DT, 1.0, nsteps=1, workers=cores, phases=phases)   # warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        psi = oracle.split_step(psi, V, dx, DT, 1.0, nsteps=1, workers=cores, phases=phases)
+        steps += 1
+        if time.perf_counter() - t0 > seconds_target or steps >= 50:
+            break
+    el = time.perf_counter() - t0
+    return {
+        "value": n ** 3 * steps / el, "unit": "grid-pt-steps/s", "cores": cores, "kind": "port",
+        "sample": f"{steps} split-steps (4 scipy FFTs each, complex128) of the 256^3 double-slit workload "
+                  f"after 1 warm-up step, scipy {scipy.__version__}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="only run 5 device copies of a 128 MiB buffer (HBM counter calibration) and exit")
+    args = ap.parse_args()
+    if args.profile_copy:
+        import torch
+        src = torch.ones(N ** 3, dtype=torch.complex64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU decomposition is "
+              "described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py is a one-GPU benchmark; run it without torch.distributed", file=sys.stderr)
+        sys.exit(2)
+    g = gpu_bench(args.steps, args.warmup)
+    value = g["npts"] * args.steps / g["elapsed"]
+    dom = g["dominant"]
+    k = g["kernels"][dom]
+    achieved = k["achieved_GBs"]
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom), "kernel": dom,
+                "algorithmic_bytes_per_launch": k["algorithmic_bytes"]}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(g["V_host"])
+    line = {
+        "metric": "grid-pt-steps/s at 256^3 (split-step Fourier, complex64)",
+        "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": g["elapsed"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "256^3 double slit + CAP, split-step Fourier, observables every 10 steps",
+                   "grid": [N, N, N], "box_bohr": L, "dt": DT, "observe_every": OBS_EVERY,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": g["kernels"],
+        "power_series": g["power_series"],
+        "norm_after_warmup": g["norm"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

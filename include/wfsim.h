@@ -151,6 +151,29 @@ int wfs_kinetic_operator(const int n[3], const double dx[3], double dt, double m
  */
 int wfs_potential_operator(long count, const void* V, double dt, int precision, void* out, void* stream);
 
+/*
+ * Per-kernel timing (no reference counterpart: the reference times whole
+ * iterations with time.time(), sources/core_sim.py:43-49).  While enabled,
+ * every kernel launched for the plan is bracketed by a pair of HIP events on
+ * its stream.  wfs_plan_profile() waits for the recorded events and returns,
+ * per kernel class, the launch count and the summed duration in ms since the
+ * last reset (reset != 0 clears the sums after reading).
+ * Kernel classes: WFS_K_* below.
+ */
+#define WFS_K_Z_FWD 0
+#define WFS_K_Z_INV 1
+#define WFS_K_Z_IVF 2
+#define WFS_K_Y_FWD 3
+#define WFS_K_Y_INV 4
+#define WFS_K_X_FKI 5
+#define WFS_K_X_FWD 6
+#define WFS_K_X_INV 7
+#define WFS_K_HORNER 8
+#define WFS_K_OBSERVABLES 9
+#define WFS_K_COUNT 10
+int wfs_plan_set_profiling(wfs_plan* plan, int enable);
+int wfs_plan_profile(wfs_plan* plan, int kernel_class, int reset, long* launches, double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif

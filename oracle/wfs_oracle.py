@@ -136,15 +136,19 @@ def potential_phase(V: np.ndarray, dt: float) -> np.ndarray:
     return (np.cos(ang.real) + 1j * np.sin(ang.real)) * np.exp(-ang.imag)
 
 
-def split_step(psi, V, dx, dt, mass, nsteps: int = 1, workers: int | None = None):
+def split_step(psi, V, dx, dt, mass, nsteps: int = 1, workers: int | None = None, phases=None):
     """Strang step psi <- F^-1 K/2 F . V . F^-1 K/2 F psi, repeated nsteps times.
 
     Follows sources/sim_state.py:641-697 step for step: four unitary FFTs, two
-    half-kinetic multiplies and one potential multiply per step.
+    half-kinetic multiplies and one potential multiply per step.  ``phases``
+    may carry precomputed (kinetic_half_phase, potential_phase) arrays.
     """
     psi = np.asarray(psi, dtype=np.complex128).copy()
-    kh = kinetic_half_phase(psi.shape, dx, dt, mass)
-    pv = potential_phase(np.asarray(V), dt)
+    if phases is None:
+        kh = kinetic_half_phase(psi.shape, dx, dt, mass)
+        pv = potential_phase(np.asarray(V), dt)
+    else:
+        kh, pv = phases
     for _ in range(nsteps):
         psi = fftn_ortho(psi, workers)
         psi *= kh

@@ -55,7 +55,14 @@ SIGNATURES = {
     "wfs_kinetic_operator": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                             _vp, _vp]),
     "wfs_potential_operator": (ctypes.c_int, [ctypes.c_long, _vp, ctypes.c_double, ctypes.c_int, _vp, _vp]),
+    "wfs_plan_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "wfs_plan_profile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_long),
+                                        ctypes.POINTER(ctypes.c_double)]),
 }
+
+# kernel classes of wfs_plan_profile
+KERNEL_CLASSES = ("z_fwd", "z_inv", "z_inv_v_fwd", "y_fwd", "y_inv", "x_fwd_k_inv", "x_fwd", "x_inv",
+                  "horner_stage", "observables")
 
 
 class WfsError(RuntimeError):
@@ -188,6 +195,21 @@ class Plan:
         if nb:
             check(lib().wfs_observables(self._h, ptr(psi), arr, nb, ptr(out), stream_handle(stream)),
                   "wfs_observables")
+        return out
+
+    def set_profiling(self, enable: bool) -> None:
+        check(lib().wfs_plan_set_profiling(self._h, 1 if enable else 0), "wfs_plan_set_profiling")
+
+    def profile(self, reset: bool = True) -> dict:
+        """{kernel class: (launches, total ms)} of the event-timed launches so far."""
+        out = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            n = ctypes.c_long()
+            ms = ctypes.c_double()
+            check(lib().wfs_plan_profile(self._h, k, 1 if reset else 0, ctypes.byref(n), ctypes.byref(ms)),
+                  "wfs_plan_profile")
+            if n.value:
+                out[name] = (int(n.value), float(ms.value))
         return out
 
     def close(self) -> None:

@@ -21,6 +21,12 @@ struct wfs_plan {
   long obs_work_len;
   int threads_rows;       // threads per block of the contiguous-axis pass
   int threads_cols;       // threads per block of the strided-axis passes
+  // per-kernel timing (wfs_plan_set_profiling)
+  bool profiling;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>>* pending;   // [WFS_K_COUNT]
+  std::vector<hipEvent_t>* event_pool;
+  long launches[WFS_K_COUNT];
+  double total_ms[WFS_K_COUNT];
 };
 
 namespace wfs {
@@ -98,8 +104,48 @@ int ensure_scratch(wfs_plan* p) {
   return 0;
 }
 
+// event pair around one launch when profiling is on
+struct ProfileScope {
+  wfs_plan* p;
+  int kclass;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfileScope(wfs_plan* plan, int kc, hipStream_t st) : p(plan), kclass(kc), s(st) {
+    if (!p->profiling) return;
+    a = take();
+    b = take();
+    if (a && b) (void)hipEventRecord(a, s);
+  }
+  hipEvent_t take() {
+    if (!p->event_pool->empty()) {
+      hipEvent_t e = p->event_pool->back();
+      p->event_pool->pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  ~ProfileScope() {
+    if (!p->profiling) return;
+    if (a && b) {
+      (void)hipEventRecord(b, s);
+      p->pending[kclass].push_back({a, b});
+    } else {
+      if (a) p->event_pool->push_back(a);
+      if (b) p->event_pool->push_back(b);
+    }
+  }
+};
+
+int kernel_class_of(int axis, int mode) {
+  if (axis == 2) return mode == MODE_FWD ? WFS_K_Z_FWD : mode == MODE_INV ? WFS_K_Z_INV : WFS_K_Z_IVF;
+  if (axis == 1) return mode == MODE_FWD ? WFS_K_Y_FWD : WFS_K_Y_INV;
+  return mode == MODE_FKI ? WFS_K_X_FKI : mode == MODE_FWD ? WFS_K_X_FWD : WFS_K_X_INV;
+}
+
 // one axis pass; axis 0 = x (strided, whole grid), 1 = y (strided per x slab), 2 = z (rows)
-int run_pass(const wfs_plan* p, int axis, int mode, void* data, const void* V, const void* const* ktab,
+int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* V, const void* const* ktab,
              double scale, hipStream_t s) {
   PassParams pp;
   pp.tw = p->tw[axis];
@@ -127,6 +173,7 @@ int run_pass(const wfs_plan* p, int axis, int mode, void* data, const void* V, c
     pp.dt = p->dt;
   }
   const int threads = (layout == LAYOUT_ROWS) ? p->threads_rows : p->threads_cols;
+  ProfileScope scope(p, kernel_class_of(axis, mode), s);
   return launch_fft_pass(p->precision, p->n[axis], layout, mode, data, pp, outer, threads, s);
 }
 
@@ -163,6 +210,7 @@ int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStr
     int which = 0;
     for (int nn = p->order; nn >= 1; --nn) {
       void* out = (nn == 1 && u_in != psi) ? psi : p->scratch[which];
+      ProfileScope scope(p, WFS_K_HORNER, s);
       WFS_TRY(launch_horner_stage(p->precision, p->n[0], p->n[1], p->n[2], u_in, out, psi,
                                   static_cast<const float2*>(V), p->dt / double(nn), inv_2m, inv_dx2, 0, s));
       u_in = out;
@@ -192,6 +240,15 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
   for (int i = 0; i < 2; ++i)
     if (p->scratch[i]) (void)hipFree(p->scratch[i]);
   if (p->obs_work) (void)hipFree(p->obs_work);
+  if (p->pending) {
+    for (int k = 0; k < WFS_K_COUNT; ++k)
+      for (auto& pr : p->pending[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    delete[] p->pending;
+  }
+  if (p->event_pool) {
+    for (auto e : *p->event_pool) (void)hipEventDestroy(e);
+    delete p->event_pool;
+  }
   delete p;
   return 0;
 }
@@ -219,6 +276,8 @@ extern "C" int wfs_plan_create(const int n[3], const double dx[3], double dt, do
   p->elem = precision == WFS_C128 ? 16 : 8;
   p->threads_rows = 256;
   p->threads_cols = 256;
+  p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
+  p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;
   for (int a = 0; a < 3 && rc == 0; ++a) {
     std::vector<double> re(n[a]), im(n[a]);
@@ -279,9 +338,35 @@ extern "C" int wfs_observables(wfs_plan* p, const void* psi, const wfs_box* boxe
         return WFS_ERR_INVALID;
       }
     }
+    ProfileScope scope(p, WFS_K_OBSERVABLES, s);
     const int rc = launch_box_moments(p->precision, p->n[0], p->n[1], p->n[2], p->dx, psi, bx, p->obs_work,
                                       p->obs_work_len, out + 7 * b, s);
     if (rc) return rc;
   }
+  return 0;
+}
+
+extern "C" int wfs_plan_set_profiling(wfs_plan* p, int enable) {
+  if (!p) { set_last_error("wfs_plan_set_profiling: null plan"); return WFS_ERR_INVALID; }
+  p->profiling = enable != 0;
+  return 0;
+}
+
+extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, double* total_ms) {
+  if (!p || kc < 0 || kc >= WFS_K_COUNT) { set_last_error("wfs_plan_profile: bad argument"); return WFS_ERR_INVALID; }
+  auto& v = p->pending[kc];
+  for (auto& pr : v) {
+    WFS_HIP_CHECK(hipEventSynchronize(pr.second));
+    float ms = 0.0f;
+    WFS_HIP_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    p->total_ms[kc] += double(ms);
+    p->launches[kc] += 1;
+    p->event_pool->push_back(pr.first);
+    p->event_pool->push_back(pr.second);
+  }
+  v.clear();
+  if (launches) *launches = p->launches[kc];
+  if (total_ms) *total_ms = p->total_ms[kc];
+  if (reset) { p->launches[kc] = 0; p->total_ms[kc] = 0.0; }
   return 0;
 }

@@ -1,0 +1,374 @@
+"""Simulation state and the time step (sources/sim_state.py).
+
+``SimState`` keeps the reference's configuration surface, geometry rules and
+getters.  The state psi (complex64 or complex128) and the potential V
+(complex64) are device tensors; every step runs in the HIP library through one
+``wfs_step`` call, which merges consecutive split-step half kicks when several
+steps are requested at once.
+"""
+
+from __future__ import annotations
+
+import io
+import os
+from enum import Enum
+from typing import Dict
+
+import numpy as np
+import torch
+
+from . import _abi
+from . import math_utils
+from . import potential
+from . import wave_packet
+from .config_read_helper import try_read_param
+
+_RED = "\033[31m"
+_GREEN = "\033[32m"
+_RESET = "\033[0m"
+
+
+class SimulationMethod(Enum):
+    """sources/sim_state.py:16-18."""
+    FOURIER = 0
+    POWER_SERIES = 1
+
+
+class SimState:
+    """Grid, time step, observation box, psi and V (sources/sim_state.py:21-762)."""
+
+    def __init__(self, config: Dict, device: torch.device | None = None):
+        self.__device = device or torch.device("cuda", torch.cuda.current_device() if torch.cuda.is_available() else 0)
+        self.__cache_dir = try_read_param(config, "paths.cache_dir", "./cache")
+        self.__output_dir = try_read_param(config, "paths.output_dir", "./output")
+        self.__use_cache = True
+        self.__enable_visual_output = True
+
+        # method (sources/sim_state.py:70-77): fft/fourier -> FOURIER, fdtd/power_series -> POWER_SERIES,
+        # anything else -> FOURIER
+        method_name = str(try_read_param(config, "simulation.method", "fourier")).lower()
+        if method_name in ("fdtd", "power_series"):
+            self.__simulation_method = SimulationMethod.POWER_SERIES
+        else:
+            self.__simulation_method = SimulationMethod.FOURIER
+        self.__order_of_approximation = int(try_read_param(config, "simulation.order_of_approximation", 10))
+        self.__is_dynamic_potential_mode = bool(try_read_param(config, "simulation.is_dynamic_potential_mode", True))
+        self.__enable_wave_function_saving = bool(
+            try_read_param(config, "simulation.enable_wave_function_saving", False))
+        self.__wave_function_save_iteration_interval = int(
+            try_read_param(config, "simulation.wave_function_save_iteration_interval", 1))
+        self.__double_precision = bool(try_read_param(config, "simulation.double_precision_calculation", False))
+
+        self.__wave_packet = wave_packet.GaussianWavePacket(config)
+
+        self.__simulated_volume_dimensions_bohr_radii_3 = np.array(
+            try_read_param(config, "volume.simulated_volume_dimensions_bohr_radii_3", [100.0, 100.0, 100.0]),
+            dtype=np.float64)
+        self.__number_of_voxels_3 = np.array(
+            try_read_param(config, "volume.number_of_voxels_3", [256, 256, 256]), dtype=np.int64)
+        self.__delta_x_bohr_radii_3 = self.__simulated_volume_dimensions_bohr_radii_3 / self.__number_of_voxels_3
+        dxm = float(np.max(self.__delta_x_bohr_radii_3))
+        self.__upper_limit_on_delta_time_h_per_hartree = 4.0 / np.pi * (3.0 * dxm * dxm) / 3.0   # :95-100
+        # required key: raises KeyError like sources/sim_state.py:101-103
+        self.__delta_time_h_bar_per_hartree = float(config["simulation"]["delta_time_h_bar_per_hartree"])
+
+        # observation box (sources/sim_state.py:105-161)
+        L = self.__simulated_volume_dimensions_bohr_radii_3
+        N = self.__number_of_voxels_3
+        b = np.array(try_read_param(config, "volume.observation_box_bottom_corner_bohr_radii_3",
+                                    [-25.0, -25.0, -25.0]), dtype=np.float64)
+        t = np.array(try_read_param(config, "volume.observation_box_top_corner_bohr_radii_3",
+                                    [25.0, 25.0, 25.0]), dtype=np.float64)
+        for i in range(3):
+            if b[i] > t[i]:
+                b[i], t[i] = t[i], b[i]
+        for i in range(3):
+            if b[i] < -L[i] * 0.5:
+                b[i] = -L[i] * 0.5
+            if t[i] > L[i] * 0.5:
+                t[i] = L[i] * 0.5
+        self.__observation_box_bottom_corner_bohr_radii_3 = b
+        self.__observation_box_top_corner_bohr_radii_3 = t
+        self.__observation_box_bottom_corner_voxel_3 = np.array(
+            math_utils.transform_center_origin_to_corner_origin_system(b, L) / self.__delta_x_bohr_radii_3,
+            dtype=np.int32)
+        self.__observation_box_top_corner_voxel_3 = np.array(
+            math_utils.transform_center_origin_to_corner_origin_system(t, L) / self.__delta_x_bohr_radii_3,
+            dtype=np.int32)
+        for i in range(3):
+            if self.__observation_box_top_corner_voxel_3[i] >= N[i]:
+                self.__observation_box_top_corner_voxel_3[i] = N[i] - 1
+            if self.__observation_box_bottom_corner_voxel_3[i] < 0:
+                self.__observation_box_bottom_corner_voxel_3[i] = 0
+        self.__observation_box_dimensions_bohr_radii_3 = t - b
+        self.__observation_box_dimensions_voxel_3 = (self.__observation_box_top_corner_voxel_3
+                                                     - self.__observation_box_bottom_corner_voxel_3)
+
+        # potentials (sources/sim_state.py:167-183); lists are per instance
+        self.__pre_initialized_potential = potential.PreInitializedPotential(config)
+        if self.__pre_initialized_potential.is_enabled():
+            self.__is_dynamic_potential_mode = False
+        self.__absorbing_boundary_condition = potential.AbsorbingBoundaryCondition(
+            config, bottom_corner=self.__observation_box_bottom_corner_bohr_radii_3,
+            top_corner=self.__observation_box_top_corner_bohr_radii_3)
+        self.__potential_walls = [potential.PotentialWall(w) for w in try_read_param(config, "potential.walls", [])]
+        self.__harmonic = potential.HarmonicPotential(config, self.__wave_packet.get_particle_mass_electron_rest_mass())
+        self.__lattice = potential.OpticalLattice(config)
+
+        self.__wave_tensor: torch.Tensor | None = None
+        self.__localised_potential_hartree: torch.Tensor | None = None
+        self.__localised_potential_to_visualize_hartree: torch.Tensor | None = None
+        self.__plan: _abi.Plan | None = None
+
+    # ------------------------------------------------------------------ set-up
+    def _shape(self):
+        return tuple(int(v) for v in self.__number_of_voxels_3)
+
+    def _build_potential(self):
+        shape = self._shape()
+        V = torch.zeros(shape, dtype=torch.complex64, device=self.__device)
+        V_vis = torch.zeros(shape, dtype=torch.complex64, device=self.__device)
+        if self.__pre_initialized_potential.is_enabled():
+            self.__pre_initialized_potential.add_potential(V)
+        self.__absorbing_boundary_condition.add_potential(V, self.__delta_x_bohr_radii_3)
+        for wall in self.__potential_walls:
+            wall.add_potential(V, self.__delta_x_bohr_radii_3)
+            if wall.visible:
+                wall.add_potential(V_vis, self.__delta_x_bohr_radii_3)
+        self.__harmonic.add_potential(V, self.__delta_x_bohr_radii_3)
+        self.__lattice.add_potential(V, self.__delta_x_bohr_radii_3)
+        return V, V_vis
+
+    def initialize_state(self):
+        """Initial psi and V, plan creation, potential dump (sources/sim_state.py:185-300)."""
+        print(self.get_sim_state_description_text(use_colors=True))
+        shape = self._shape()
+        full_init = True
+        if self.__use_cache:
+            try:
+                arr = np.load(os.path.join(self.__cache_dir, "gaussian_wave_packet.npy"), allow_pickle=False)
+                want = np.complex128 if self.__double_precision else np.complex64
+                if arr.shape == shape and arr.dtype == want:
+                    self.__wave_tensor = torch.from_numpy(arr).to(self.__device)
+                    full_init = False
+            except OSError:
+                print("No cached gaussian_wave_packet.npy found.")
+        if full_init:
+            self.__wave_tensor = self.__wave_packet.init_wave_packet(
+                self.__delta_x_bohr_radii_3, shape, self.__double_precision, self.__device)
+            os.makedirs(self.__cache_dir, exist_ok=True)
+            np.save(os.path.join(self.__cache_dir, "gaussian_wave_packet.npy"), self.__wave_tensor.cpu().numpy())
+
+        full_init = True
+        if self.__use_cache:
+            try:
+                v = np.load(os.path.join(self.__cache_dir, "localized_potential.npy"), allow_pickle=False)
+                vv = np.load(os.path.join(self.__cache_dir, "localized_potential_to_visualize.npy"), allow_pickle=False)
+                if v.shape == shape and v.dtype == np.complex64:
+                    self.__localised_potential_hartree = torch.from_numpy(v).to(self.__device)
+                    self.__localised_potential_to_visualize_hartree = torch.from_numpy(vv).to(self.__device)
+                    full_init = False
+            except OSError:
+                print("No cached localized_potential.npy found.")
+        if full_init:
+            V, V_vis = self._build_potential()
+            self.__localised_potential_hartree = V
+            self.__localised_potential_to_visualize_hartree = V_vis
+            os.makedirs(self.__cache_dir, exist_ok=True)
+            np.save(os.path.join(self.__cache_dir, "localized_potential.npy"), V.cpu().numpy())
+            np.save(os.path.join(self.__cache_dir, "localized_potential_to_visualize.npy"), V_vis.cpu().numpy())
+
+        self.__plan = _abi.Plan(
+            shape, self.__delta_x_bohr_radii_3, self.__delta_time_h_bar_per_hartree,
+            self.get_particle_mass(), _abi.WFS_C128 if self.__double_precision else _abi.WFS_C64,
+            _abi.WFS_METHOD_FOURIER if self.__simulation_method == SimulationMethod.FOURIER
+            else _abi.WFS_METHOD_POWER_SERIES, self.__order_of_approximation)
+
+        # raw float32 dump of V (sources/sim_state.py:296-300)
+        os.makedirs(os.path.join(self.__output_dir, "potential_raw"), exist_ok=True)
+        self.__localised_potential_hartree.cpu().numpy().view(np.float32).tofile(
+            os.path.join(self.__output_dir, "potential_raw/potential.bin"))
+
+    # ------------------------------------------------------------------ getters
+    def get_delta_time_h_bar_per_hartree(self):
+        return self.__delta_time_h_bar_per_hartree
+
+    def get_delta_x_bohr_radii_3(self):
+        return self.__delta_x_bohr_radii_3
+
+    def set_wave_function(self, wave_func):
+        want = torch.complex128 if self.__double_precision else torch.complex64
+        if not isinstance(wave_func, torch.Tensor):
+            wave_func = torch.from_numpy(np.ascontiguousarray(wave_func))
+        wave_func = wave_func.to(device=self.__device, dtype=want).contiguous()
+        if tuple(wave_func.shape) != self._shape():
+            raise ValueError(f"wave function shape {tuple(wave_func.shape)} != grid {self._shape()}")
+        self.__wave_tensor = wave_func
+
+    def get_wave_function(self) -> torch.Tensor:
+        return self.__wave_tensor
+
+    def get_potential(self) -> torch.Tensor:
+        """Device complex64 potential used by the step."""
+        return self.__localised_potential_hartree
+
+    def set_potential(self, V: torch.Tensor):
+        V = V.to(device=self.__device, dtype=torch.complex64).contiguous()
+        if tuple(V.shape) != self._shape():
+            raise ValueError("potential shape does not match the grid")
+        self.__localised_potential_hartree = V
+
+    def get_plan(self) -> _abi.Plan:
+        return self.__plan
+
+    def is_wave_function_saving(self):
+        return self.__enable_wave_function_saving
+
+    def get_wave_function_save_interval(self):
+        return self.__wave_function_save_iteration_interval
+
+    def get_simulation_method(self):
+        return self.__simulation_method
+
+    def get_order_of_approximation(self):
+        return self.__order_of_approximation
+
+    def is_dynamic_potential_mode(self):
+        return self.__is_dynamic_potential_mode
+
+    def is_use_cache(self):
+        return self.__use_cache
+
+    def set_use_cache(self, uc: bool):
+        self.__use_cache = uc
+
+    def get_cache_dir(self):
+        return self.__cache_dir
+
+    def get_output_dir(self):
+        return self.__output_dir
+
+    def get_number_of_voxels_3(self):
+        return self.__number_of_voxels_3
+
+    def get_simulated_volume_dimensions_bohr_radii_3(self):
+        return self.__simulated_volume_dimensions_bohr_radii_3
+
+    def get_observation_box_bottom_corner_voxel_3(self):
+        return self.__observation_box_bottom_corner_voxel_3
+
+    def get_observation_box_top_corner_voxel_3(self):
+        return self.__observation_box_top_corner_voxel_3
+
+    def get_observation_box_bottom_corner_bohr_radii_3(self):
+        return self.__observation_box_bottom_corner_bohr_radii_3
+
+    def get_observation_box_top_corner_bohr_radii_3(self):
+        return self.__observation_box_top_corner_bohr_radii_3
+
+    def is_double_precision(self) -> bool:
+        return self.__double_precision
+
+    def get_device(self) -> torch.device:
+        return self.__device
+
+    def transform_physical_coordinate_to_voxel_3(self, pos_bohr_radii_3):
+        """((pos + L/2) / dx) truncated to unsigned (sources/sim_state.py:354-358)."""
+        return (math_utils.transform_center_origin_to_corner_origin_system(
+            pos_bohr_radii_3, self.__simulated_volume_dimensions_bohr_radii_3)
+            / self.__delta_x_bohr_radii_3).astype(np.uint64)
+
+    def transform_voxel_to_physical_coordinate_3(self, voxel_3):
+        """sources/sim_state.py:360-364."""
+        return math_utils.transform_corner_origin_to_center_origin_system(
+            np.asarray(voxel_3), self.__number_of_voxels_3) * self.__delta_x_bohr_radii_3
+
+    def get_view_into_wave_function(self):
+        """psi restricted to the observation box (sources/sim_state.py:366-383)."""
+        return math_utils.cut_bounding_box(self.__wave_tensor, self.__observation_box_bottom_corner_voxel_3,
+                                           self.__observation_box_top_corner_voxel_3)
+
+    def get_view_into_potential(self):
+        return math_utils.cut_bounding_box(self.__localised_potential_to_visualize_hartree.real,
+                                           self.__observation_box_bottom_corner_voxel_3,
+                                           self.__observation_box_top_corner_voxel_3)
+
+    def get_view_into_complex_potential(self):
+        return math_utils.cut_bounding_box(self.__localised_potential_to_visualize_hartree,
+                                           self.__observation_box_bottom_corner_voxel_3,
+                                           self.__observation_box_top_corner_voxel_3)
+
+    def get_particle_mass(self):
+        return self.__wave_packet.get_particle_mass_electron_rest_mass()
+
+    def get_wave_packet(self):
+        return self.__wave_packet
+
+    def get_dxdydz(self):
+        d = self.__delta_x_bohr_radii_3
+        return d[0] * d[1] * d[2]
+
+    # ------------------------------------------------------------------ dynamics
+    def update_potential(self):
+        """Dynamic-potential mode (sources/sim_state.py:466-509).
+
+        Wall centres advect by v*dt and an x centre below zero is clamped to
+        zero (the reference's ":472-473" test clamp); V is then rebuilt from
+        walls and the absorbing boundary.  When no centre changed the rebuild
+        would reproduce V bit for bit, so it is skipped.
+        """
+        if not self.__is_dynamic_potential_mode:
+            return
+        changed = False
+        for w in self.__potential_walls:
+            old = w.center_bohr_radii_3.copy()
+            w.center_bohr_radii_3 = w.center_bohr_radii_3 + (w.velocity_bohr_radius_hartree_per_h_bar_3
+                                                             * self.__delta_time_h_bar_per_hartree)
+            if w.center_bohr_radii_3[0] < 0.0:
+                w.center_bohr_radii_3[0] = 0.0
+            if not np.array_equal(old, w.center_bohr_radii_3):
+                changed = True
+        if not changed and self.__localised_potential_hartree is not None and getattr(self, "_regenerated", False):
+            return
+        self.__localised_potential_hartree, self.__localised_potential_to_visualize_hartree = \
+            potential.generate_potential_from_walls_and_drain(
+                V=self.__localised_potential_hartree, V_vis=self.__localised_potential_to_visualize_hartree,
+                delta_x_3=self.__delta_x_bohr_radii_3, absorbing_pontetial=self.__absorbing_boundary_condition,
+                walls=self.__potential_walls)
+        self.__harmonic.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
+        self.__lattice.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
+        self._regenerated = True
+
+    def potential_is_static(self) -> bool:
+        """True when update_potential() can no longer change V."""
+        if not self.__is_dynamic_potential_mode:
+            return True
+        if not getattr(self, "_regenerated", False):
+            return False
+        for w in self.__potential_walls:
+            if np.any(w.velocity_bohr_radius_hartree_per_h_bar_3 != 0.0):
+                return False
+        return True
+
+    def evolve_state(self, steps: int = 1):
+        """Advance psi by ``steps`` time steps in place (sources/sim_state.py:754-762)."""
+        if self.__plan is None:
+            raise RuntimeError("initialize_state() must be called before evolve_state()")
+        self.__plan.step(self.__wave_tensor, self.__localised_potential_hartree, int(steps))
+
+    # ------------------------------------------------------------------ text
+    def get_sim_state_description_text(self, use_colors=False):
+        """Summary of the configured system (sources/sim_state.py:548-630, abridged)."""
+        g = _GREEN if use_colors else ""
+        r = _RESET if use_colors else ""
+        red = _RED if use_colors else ""
+        text = io.StringIO()
+        text.write(g + "Simulated system state:\n" + r)
+        text.write(f"Mass of the particle is {self.get_particle_mass()} electron rest mass.\n")
+        text.write(f"Number of samples per axis is {tuple(int(v) for v in self.__number_of_voxels_3)}.\n")
+        text.write(f"Space resolution is delta_r = {tuple(float(v) for v in self.__delta_x_bohr_radii_3)} Bohr radii.\n")
+        text.write(f"The maximal viable time resolution < {self.__upper_limit_on_delta_time_h_per_hartree} h-bar / Hartree\n")
+        text.write(f"Time resolution is delta = {self.__delta_time_h_bar_per_hartree} h-bar / Hartree.\n")
+        if self.__delta_time_h_bar_per_hartree > 0.5 * self.__upper_limit_on_delta_time_h_per_hartree:
+            text.write(red + "WARNING: delta_time exceeds theoretical limit!\n" + r)
+        return text.getvalue()
