@@ -83,6 +83,15 @@ int wfs_plan_destroy(wfs_plan* plan);
 int wfs_step(wfs_plan* plan, void* psi, const void* V, int nsteps, void* stream);
 
 /*
+ * Store the potential of subsequent wfs_step calls made with V == NULL.
+ * For the split-step the plan keeps the phase table cexp_i(-dt V) (the intent
+ * of operators.init_potential_operator, sources/operators.py:41-55); callers
+ * whose V changes (SimState.update_potential, sources/sim_state.py:466-509)
+ * call it again.  wfs_step with V != NULL refreshes the table itself.
+ */
+int wfs_set_potential(wfs_plan* plan, const void* V, void* stream);
+
+/*
  * Unitary 3-D DFT in place: direction -1 = forward (cupy.fft.fftn(norm="ortho"),
  * sources/sim_state.py:646,677), +1 = inverse (ifftn, :665,696).
  */
@@ -172,6 +181,16 @@ int wfs_potential_operator(long count, const void* V, double dt, int precision, 
 #define WFS_K_OBSERVABLES 9
 #define WFS_K_COUNT 10
 int wfs_plan_set_profiling(wfs_plan* plan, int enable);
+
+/*
+ * FFT tile autotuner (no reference counterpart).  Times each tile variant
+ * (0, 1, 2 = half, default, double the columns per block) of every
+ * split-step pass on plan-owned scratch buffers and keeps the fastest per
+ * kernel class.  Variants give bit-identical results.  Synchronises stream.
+ */
+int wfs_plan_autotune(wfs_plan* plan, void* stream);
+int wfs_plan_set_variant(wfs_plan* plan, int kernel_class, int variant);
+int wfs_plan_get_variant(wfs_plan* plan, int kernel_class);
 int wfs_plan_profile(wfs_plan* plan, int kernel_class, int reset, long* launches, double* total_ms);
 
 #ifdef __cplusplus

@@ -1,0 +1,143 @@
+"""CPU-only checks: the C ABI library loads and exports what include/wfsim.h
+declares, and host-side logic (config lookup, signals, time integrals) behaves
+like the reference."""
+
+import os
+import re
+import signal
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from wavefunctionsimpython_amd import _abi, math_utils
+from wavefunctionsimpython_amd.config_read_helper import try_read_param
+from wavefunctionsimpython_amd.iter_data import IterData
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "wfsim.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wfs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = _abi.lib()
+    names = _header_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _abi.SIGNATURES, f"{name} has no ctypes signature"
+    assert set(_abi.SIGNATURES) == set(names)
+    assert lib.wfs_version().startswith(b"wfsim-gfx950")
+
+
+def test_library_is_built_for_gfx950():
+    """The fat binary embeds gfx950 code objects (our own build)."""
+    data = open(_abi.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_bad_arguments_fail_loudly_without_a_gpu_call():
+    lib = _abi.lib()
+    import ctypes
+    h = ctypes.c_void_p()
+    rc = lib.wfs_plan_create(_abi.i3((100, 64, 64)), _abi.d3((1, 1, 1)), 0.01, 1.0, 0, 0, 10, ctypes.byref(h))
+    assert rc == 1001 and b"length 100 not supported" in lib.wfs_last_error()
+    rc = lib.wfs_step(None, None, None, 1, None)
+    assert rc == 1000
+    with pytest.raises(ValueError):
+        _abi.Plan((100, 64, 64), (1, 1, 1), 0.01, 1.0, 0, 0)
+
+
+def test_try_read_param_semantics(capsys):
+    cfg = {"a": {"b": {"c": 3}}, "x": 1}
+    assert try_read_param(cfg, "a.b.c", 0) == 3
+    assert try_read_param(cfg, "a.q.c", 7) == 7
+    out = capsys.readouterr().out
+    assert 'Did not found token "q" under "a"' in out and "default value of 7" in out
+    assert try_read_param(cfg, "x", 5) == 1
+    assert IterData({}).total_iteration_count == 1000
+    assert IterData({"simulation": {"total_iteration_count": 12}}).total_iteration_count == 12
+
+
+def test_delta_time_is_required():
+    from wavefunctionsimpython_amd.sim_state import SimState
+    with pytest.raises(KeyError):
+        SimState({"simulation": {}})
+
+
+def test_sim_state_geometry_cpu_only():
+    from wavefunctionsimpython_amd.sim_state import SimState, SimulationMethod
+    cfg = {"simulation": {"delta_time_h_bar_per_hartree": 0.01, "method": "FDTD"},
+           "volume": {"simulated_volume_dimensions_bohr_radii_3": [60.0] * 3, "number_of_voxels_3": [64] * 3,
+                      "observation_box_bottom_corner_bohr_radii_3": [25.0, -40.0, -25.0],
+                      "observation_box_top_corner_bohr_radii_3": [-25.0, 40.0, 25.0]}}
+    s = SimState(cfg)
+    assert s.get_simulation_method() == SimulationMethod.POWER_SERIES
+    _, _, bv, tv = oracle.observation_box_voxels([60.0] * 3, [64] * 3, [25.0, -40.0, -25.0], [-25.0, 40.0, 25.0])
+    assert np.array_equal(s.get_observation_box_bottom_corner_voxel_3(), bv)
+    assert np.array_equal(s.get_observation_box_top_corner_voxel_3(), tv)
+    assert np.allclose(s.get_delta_x_bohr_radii_3(), 60.0 / 64)
+    s2 = SimState({"simulation": {"delta_time_h_bar_per_hartree": 0.01, "method": "something"}})
+    assert s2.get_simulation_method() == SimulationMethod.FOURIER
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 4, 5, 10, 11, 64, 101])
+def test_indefinite_simpson_integral_matches_oracle(n):
+    a = np.random.default_rng(n).standard_normal(n)
+    assert np.allclose(math_utils.indefinite_simpson_integral(a, 0.37), oracle.indefinite_simpson_integral(a, 0.37),
+                       rtol=1e-13, atol=1e-13)
+
+
+def test_analytic_predictions_match_oracle():
+    assert np.allclose(math_utils.predict_free_space_standard_devation(0.01, 3.0, 1.0, 50),
+                       oracle.predict_free_space_standard_deviation(0.01, 3.0, 1.0, 50))
+    assert np.allclose(math_utils.predict_free_space_expected_location(0.01, 1.0, 2.0, 50),
+                       oracle.predict_free_space_expected_location(0.01, 1.0, 2.0, 50))
+
+
+def test_sigint_sets_quit_flag():
+    from wavefunctionsimpython_amd import core_sim
+    it = IterData({})
+    prev = core_sim.register_signal_handler(it)
+    try:
+        os.kill(os.getpid(), signal.SIGINT)
+        assert it.is_quit
+    finally:
+        signal.signal(signal.SIGINT, prev)
+
+
+def test_toml_decode_error_exits_with_status_1(tmp_path):
+    bad = tmp_path / "bad.toml"
+    bad.write_text("[simulation\nx = ")
+    code = ("import sys; sys.path.insert(0, %r); from wavefunctionsimpython_amd import initialisation; "
+            "initialisation.load_config(%r)" % (ROOT, str(bad)))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True)
+    assert r.returncode == 1 and "Failed to decode TOML" in r.stdout
+
+
+def test_reference_toml_schema_parses(tmp_path):
+    from wavefunctionsimpython_amd import initialisation
+    p = tmp_path / "c.toml"
+    p.write_text("""
+[simulation]
+method = "fourier"
+delta_time_h_bar_per_hartree = 0.2
+total_iteration_count = 541
+double_precision_calculation = true
+[volume]
+number_of_voxels_3 = [512, 256, 256]
+[[potential.walls]]
+potential_hartree = 7.0
+slit_count = 2
+[[measurement.volume_probabilities]]
+name = "First half"
+""")
+    cfg = initialisation.load_config(str(p))
+    assert try_read_param(cfg, "simulation.total_iteration_count", 0) == 541
+    assert cfg["potential"]["walls"][0]["slit_count"] == 2

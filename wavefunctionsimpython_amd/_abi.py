@@ -56,6 +56,10 @@ SIGNATURES = {
                                             _vp, _vp]),
     "wfs_potential_operator": (ctypes.c_int, [ctypes.c_long, _vp, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "wfs_plan_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "wfs_set_potential": (ctypes.c_int, [_vp, _vp, _vp]),
+    "wfs_plan_autotune": (ctypes.c_int, [_vp, _vp]),
+    "wfs_plan_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "wfs_plan_get_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
     "wfs_plan_profile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_long),
                                         ctypes.POINTER(ctypes.c_double)]),
 }
@@ -158,6 +162,8 @@ class Plan:
         self.method = int(method)
         self.order = int(order)
         self._h = ctypes.c_void_p()
+        self._potential_set = False
+        self._V_ref = None
         check(lib().wfs_plan_create(i3(n), d3(dx), self.dt, self.mass, self.precision, self.method,
                                     self.order, ctypes.byref(self._h)), "wfs_plan_create")
 
@@ -165,12 +171,37 @@ class Plan:
     def handle(self) -> ctypes.c_void_p:
         return self._h
 
-    def step(self, psi: torch.Tensor, V: torch.Tensor, nsteps: int = 1, stream=None) -> None:
+    def step(self, psi: torch.Tensor, V: torch.Tensor | None, nsteps: int = 1, stream=None) -> None:
+        """Advance psi by nsteps; V=None reuses the potential given to set_potential()."""
         check_state(psi, self.n, self.precision)
-        check_potential(V, self.n)
+        if V is None and self.method == WFS_METHOD_POWER_SERIES:
+            V = self._V_ref
+        if V is not None:
+            check_potential(V, self.n)
+        elif not self._potential_set:
+            raise ValueError("no potential: pass V or call set_potential() first")
         if nsteps < 0:
             raise ValueError("nsteps must be >= 0")
-        check(lib().wfs_step(self._h, ptr(psi), ptr(V), int(nsteps), stream_handle(stream)), "wfs_step")
+        check(lib().wfs_step(self._h, ptr(psi), ptr(V) if V is not None else None, int(nsteps),
+                             stream_handle(stream)), "wfs_step")
+        if V is not None:
+            self._potential_set = True
+            if self.method == WFS_METHOD_POWER_SERIES:
+                self._V_ref = V
+
+    def set_potential(self, V: torch.Tensor, stream=None) -> None:
+        check_potential(V, self.n)
+        check(lib().wfs_set_potential(self._h, ptr(V), stream_handle(stream)), "wfs_set_potential")
+        self._potential_set = True
+        self._V_ref = V if self.method == WFS_METHOD_POWER_SERIES else None
+
+    def autotune(self, stream=None) -> dict:
+        check(lib().wfs_plan_autotune(self._h, stream_handle(stream)), "wfs_plan_autotune")
+        return {name: lib().wfs_plan_get_variant(self._h, k) for k, name in enumerate(KERNEL_CLASSES)}
+
+    def set_variant(self, kernel_class: str, variant: int) -> None:
+        check(lib().wfs_plan_set_variant(self._h, KERNEL_CLASSES.index(kernel_class), int(variant)),
+              "wfs_plan_set_variant")
 
     def fft3d(self, data: torch.Tensor, direction: int, stream=None) -> None:
         check_state(data, self.n, self.precision, "data")

@@ -112,22 +112,14 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
     __syncthreads();
     transform(std::integral_constant<int, +1>{});
   } else if constexpr (MODE == MODE_IVF) {
-    // z pass at the turning point: inverse, potential phase, forward
+    // z pass at the turning point: inverse, potential phase table, forward.
+    // pv holds cexp_i(-dt V) = (cos(-dt Vr), sin(-dt Vr)) exp(dt Vi) per point
+    // (sources/cuda_kernels/common.cu:98-111, potential_operator.cu:10-15).
     transform(std::integral_constant<int, +1>{});
-    const float2* __restrict__ V = p.V;
-    const R dt = R(p.dt);
+    const C* __restrict__ pv = reinterpret_cast<const C*>(p.pv);
     if (active) {
 #pragma unroll
-      for (int i = 0; i < N1; ++i) {
-        const float2 v = V[base + t + N2 * i];
-        // cexp_i(-dt*V) = (cos(-dt Vr), sin(-dt Vr)) * exp(dt Vi)
-        //   sources/cuda_kernels/common.cu:98-111, potential_operator.cu:10-15
-        R s, c;
-        const R ang = -dt * R(v.x);
-        if constexpr (sizeof(R) == 4) sincosf(ang, &s, &c); else sincos(ang, &s, &c);
-        const R amp = (sizeof(R) == 4) ? R(expf(float(dt * R(v.y)))) : R(exp(double(dt * R(v.y))));
-        a[i] = cmul(a[i], mk<R>(c * amp, s * amp));
-      }
+      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], pv[base + t + N2 * i]);
     }
     __syncthreads();
     transform(std::integral_constant<int, -1>{});
@@ -154,24 +146,31 @@ template <> struct split_of<192> { static constexpr int n2 = 8; };
 template <> struct split_of<256> { static constexpr int n2 = 16; };
 template <> struct split_of<512> { static constexpr int n2 = 16; };
 
-// Columns (COLS) or rows (ROWS) per block: about 256 threads, with the LDS
-// exchange buffer kept within 64 KiB so that at least two blocks fit a CU.
+// Columns (COLS) or rows (ROWS) per block.  The default aims at 256 threads
+// with the LDS exchange buffer within 64 KiB (two or more blocks per CU); the
+// autotuned variants halve or double it (LDS up to 80 KiB, 1024 threads).
+template <typename R, int N, int N2, int LAYOUT>
+constexpr long lds_bytes(int g) {
+  const long e = (LAYOUT == LAYOUT_ROWS) ? long(g) * (N / N2) * (N2 + 1) : long(g) * N;
+  return e * long(sizeof(cx<R>));
+}
+
 template <typename R, int N, int N2, int LAYOUT>
 constexpr int pick_groups() {
   int g = 256 / N2;
   if (g > 64) g = 64;
-  auto lds = [](int gg) {
-    const long e = (LAYOUT == LAYOUT_ROWS) ? long(gg) * (N / N2) * (N2 + 1) : long(gg) * N;
-    return e * long(sizeof(cx<R>));
-  };
-  while (g > 4 && lds(g) > 65536) g /= 2;
+  while (g > 4 && lds_bytes<R, N, N2, LAYOUT>(g) > 65536) g /= 2;
   return g;
 }
 
-template <typename R, int N, int LAYOUT, int MODE>
-static int launch_n(void* data, const PassParams& p, long outer, int /*threads*/, hipStream_t s) {
+template <typename R, int N, int N2, int LAYOUT>
+constexpr bool groups_ok(int g) {
+  return g >= 2 && g <= 128 && N2 * g <= 1024 && lds_bytes<R, N, N2, LAYOUT>(g) <= 81920;
+}
+
+template <typename R, int N, int LAYOUT, int MODE, int G>
+static int launch_g(void* data, const PassParams& p, long outer, hipStream_t s) {
   constexpr int N2 = split_of<N>::n2;
-  constexpr int G = pick_groups<R, N, N2, LAYOUT>();
   const long ncols = (LAYOUT == LAYOUT_ROWS) ? p.rows : p.inner;
   dim3 grid(unsigned((ncols + G - 1) / G), unsigned(LAYOUT == LAYOUT_ROWS ? 1 : outer), 1);
   hipLaunchKernelGGL((fft_pass_kernel<R, N, N2, LAYOUT, MODE, G>), grid, dim3(N2 * G), 0, s,
@@ -179,18 +178,32 @@ static int launch_n(void* data, const PassParams& p, long outer, int /*threads*/
   return int(hipGetLastError());
 }
 
+// variant 0 / 1 / 2 = half / default / double group count (complex64, N >= 64)
+template <typename R, int N, int LAYOUT, int MODE>
+static int launch_n(void* data, const PassParams& p, long outer, int variant, hipStream_t s) {
+  constexpr int N2 = split_of<N>::n2;
+  constexpr int G0 = pick_groups<R, N, N2, LAYOUT>();
+  if constexpr (sizeof(R) == 4 && N >= 64) {
+    constexpr int GS = groups_ok<R, N, N2, LAYOUT>(G0 / 2) ? G0 / 2 : G0;
+    constexpr int GL = groups_ok<R, N, N2, LAYOUT>(G0 * 2) ? G0 * 2 : G0;
+    if (variant == 0) return launch_g<R, N, LAYOUT, MODE, GS>(data, p, outer, s);
+    if (variant == 2) return launch_g<R, N, LAYOUT, MODE, GL>(data, p, outer, s);
+  }
+  return launch_g<R, N, LAYOUT, MODE, G0>(data, p, outer, s);
+}
+
 template <typename R, int LAYOUT, int MODE>
-static int launch_mode(int n, void* data, const PassParams& p, long outer, int threads, hipStream_t s) {
+static int launch_mode(int n, void* data, const PassParams& p, long outer, int variant, hipStream_t s) {
   switch (n) {
-    case 8:   return launch_n<R, 8, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 16:  return launch_n<R, 16, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 32:  return launch_n<R, 32, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 64:  return launch_n<R, 64, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 96:  return launch_n<R, 96, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 128: return launch_n<R, 128, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 192: return launch_n<R, 192, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 256: return launch_n<R, 256, LAYOUT, MODE>(data, p, outer, threads, s);
-    case 512: return launch_n<R, 512, LAYOUT, MODE>(data, p, outer, threads, s);
+    case 8:   return launch_n<R, 8, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 16:  return launch_n<R, 16, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 32:  return launch_n<R, 32, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 64:  return launch_n<R, 64, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 96:  return launch_n<R, 96, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 128: return launch_n<R, 128, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 192: return launch_n<R, 192, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 256: return launch_n<R, 256, LAYOUT, MODE>(data, p, outer, variant, s);
+    case 512: return launch_n<R, 512, LAYOUT, MODE>(data, p, outer, variant, s);
     default:
       set_last_error("unsupported FFT length %d (supported: 8,16,32,64,96,128,192,256,512)", n);
       return WFS_ERR_UNSUPPORTED;
@@ -198,16 +211,16 @@ static int launch_mode(int n, void* data, const PassParams& p, long outer, int t
 }
 
 template <typename R>
-int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams& p, long outer, int threads,
+int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams& p, long outer, int variant,
                       hipStream_t s) {
   if (layout == LAYOUT_ROWS) {
-    if (mode == MODE_FWD) return launch_mode<R, LAYOUT_ROWS, MODE_FWD>(n, data, p, outer, threads, s);
-    if (mode == MODE_INV) return launch_mode<R, LAYOUT_ROWS, MODE_INV>(n, data, p, outer, threads, s);
-    if (mode == MODE_IVF) return launch_mode<R, LAYOUT_ROWS, MODE_IVF>(n, data, p, outer, threads, s);
+    if (mode == MODE_FWD) return launch_mode<R, LAYOUT_ROWS, MODE_FWD>(n, data, p, outer, variant, s);
+    if (mode == MODE_INV) return launch_mode<R, LAYOUT_ROWS, MODE_INV>(n, data, p, outer, variant, s);
+    if (mode == MODE_IVF) return launch_mode<R, LAYOUT_ROWS, MODE_IVF>(n, data, p, outer, variant, s);
   } else {
-    if (mode == MODE_FWD) return launch_mode<R, LAYOUT_COLS, MODE_FWD>(n, data, p, outer, threads, s);
-    if (mode == MODE_INV) return launch_mode<R, LAYOUT_COLS, MODE_INV>(n, data, p, outer, threads, s);
-    if (mode == MODE_FKI) return launch_mode<R, LAYOUT_COLS, MODE_FKI>(n, data, p, outer, threads, s);
+    if (mode == MODE_FWD) return launch_mode<R, LAYOUT_COLS, MODE_FWD>(n, data, p, outer, variant, s);
+    if (mode == MODE_INV) return launch_mode<R, LAYOUT_COLS, MODE_INV>(n, data, p, outer, variant, s);
+    if (mode == MODE_FKI) return launch_mode<R, LAYOUT_COLS, MODE_FKI>(n, data, p, outer, variant, s);
   }
   set_last_error("invalid FFT pass layout/mode %d/%d", layout, mode);
   return WFS_ERR_INVALID;
@@ -225,9 +238,9 @@ bool fft_length_supported(int n) {
 }
 
 int launch_fft_pass(int precision, int n, int layout, int mode, void* data, const PassParams& p, long outer,
-                    int threads, hipStream_t s) {
-  if (precision == WFS_C128) return launch_fft_pass_t<double>(n, layout, mode, data, p, outer, threads, s);
-  return launch_fft_pass_t<float>(n, layout, mode, data, p, outer, threads, s);
+                    int variant, hipStream_t s) {
+  if (precision == WFS_C128) return launch_fft_pass_t<double>(n, layout, mode, data, p, outer, variant, s);
+  return launch_fft_pass_t<float>(n, layout, mode, data, p, outer, variant, s);
 }
 #else
 template int launch_fft_pass_t<double>(int, int, int, void*, const PassParams&, long, int, hipStream_t);

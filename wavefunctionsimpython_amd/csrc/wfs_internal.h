@@ -18,17 +18,18 @@ struct PassParams {
   const void* ky = nullptr;
   const void* kz = nullptr;
   long nz = 1;                  // FKI: column index -> (ky, kz) = (c / nz, c % nz)
-  const float2* V = nullptr;    // IVF: complex64 potential, same layout as the state
-  double dt = 0.0;              // IVF: time step
+  const void* pv = nullptr;     // IVF: potential phase table cexp_i(-dt V), state precision, state layout
   double scale = 1.0;           // FWD/INV: output scale (1 = none)
 };
 
 bool fft_length_supported(int n);
 template <typename R>
-int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams& p, long outer, int threads,
+int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams& p, long outer, int variant,
                       hipStream_t s);
 int launch_fft_pass(int precision, int n, int layout, int mode, void* data, const PassParams& p, long outer,
-                    int threads, hipStream_t s);
+                    int variant, hipStream_t s);
+// pv = cexp_i(-dt V) elementwise in the state precision
+int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s);
 
 // power series: one Horner stage  u_out = psi + (-i dt / n) H u_in
 int launch_horner_stage(int precision, int nx, int ny, int nz, const void* u_in, void* u_out, const void* psi,

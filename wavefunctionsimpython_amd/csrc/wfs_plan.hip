@@ -19,8 +19,9 @@ struct wfs_plan {
   void* scratch[2];       // power-series ping-pong buffers (lazy)
   double* obs_work;       // observable partial records
   long obs_work_len;
-  int threads_rows;       // threads per block of the contiguous-axis pass
-  int threads_cols;       // threads per block of the strided-axis passes
+  void* pv;               // potential phase table cexp_i(-dt V), state precision
+  bool pv_set;
+  int variant[WFS_K_COUNT];   // tile variant per kernel class (wfs_plan_autotune)
   // per-kernel timing (wfs_plan_set_profiling)
   bool profiling;
   std::vector<std::pair<hipEvent_t, hipEvent_t>>* pending;   // [WFS_K_COUNT]
@@ -41,6 +42,12 @@ void set_last_error(const char* fmt, ...) {
 }
 
 namespace {
+
+#define WFS_TRY(expr)              \
+  do {                             \
+    int _rc = (expr);              \
+    if (_rc != 0) return _rc;      \
+  } while (0)
 
 // numpy.fft.fftfreq index -> signed frequency index
 long freq_index(long j, long n) { return (j <= (n - 1) / 2) ? j : j - n; }
@@ -145,7 +152,7 @@ int kernel_class_of(int axis, int mode) {
 }
 
 // one axis pass; axis 0 = x (strided, whole grid), 1 = y (strided per x slab), 2 = z (rows)
-int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* V, const void* const* ktab,
+int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* pv, const void* const* ktab,
              double scale, hipStream_t s) {
   PassParams pp;
   pp.tw = p->tw[axis];
@@ -168,22 +175,28 @@ int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* V, const v
     pp.kx = ktab[0]; pp.ky = ktab[1]; pp.kz = ktab[2];
     pp.nz = p->n[2];
   }
-  if (mode == MODE_IVF) {
-    pp.V = static_cast<const float2*>(V);
-    pp.dt = p->dt;
-  }
-  const int threads = (layout == LAYOUT_ROWS) ? p->threads_rows : p->threads_cols;
-  ProfileScope scope(p, kernel_class_of(axis, mode), s);
-  return launch_fft_pass(p->precision, p->n[axis], layout, mode, data, pp, outer, threads, s);
+  if (mode == MODE_IVF) pp.pv = pv;
+  const int kc = kernel_class_of(axis, mode);
+  ProfileScope scope(p, kc, s);
+  return launch_fft_pass(p->precision, p->n[axis], layout, mode, data, pp, outer, p->variant[kc], s);
 }
 
-#define WFS_TRY(expr)              \
-  do {                             \
-    int _rc = (expr);              \
-    if (_rc != 0) return _rc;      \
-  } while (0)
+size_t grid_points(const wfs_plan* p) { return size_t(p->n[0]) * p->n[1] * p->n[2]; }
+
+int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
+  if (!p->pv) {
+    hipError_t e = hipMalloc(&p->pv, grid_points(p) * p->elem);
+    if (e != hipSuccess) { p->pv = nullptr; set_last_error("hipMalloc for the potential phase table failed"); return WFS_ERR_ALLOC; }
+  }
+  WFS_TRY(launch_potential_phase(p->precision, long(grid_points(p)), static_cast<const float2*>(V), p->dt, p->pv, s));
+  p->pv_set = true;
+  return 0;
+}
 
 int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
+  if (V) WFS_TRY(set_potential(p, V, s));
+  if (!p->pv_set) { set_last_error("wfs_step: no potential (pass V or call wfs_set_potential first)"); return WFS_ERR_INVALID; }
+  const void* pv = p->pv;
   const void* const* kh = p->khalf;
   const void* const* kf = p->kfull;
   WFS_TRY(run_pass(p, 2, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
@@ -191,7 +204,7 @@ int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t
   WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, kh, 1.0, s));
   for (int step = 1; step <= nsteps; ++step) {
     WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s));
-    WFS_TRY(run_pass(p, 2, MODE_IVF, psi, V, nullptr, 1.0, s));
+    WFS_TRY(run_pass(p, 2, MODE_IVF, psi, pv, nullptr, 1.0, s));
     WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
     WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, step == nsteps ? kh : kf, 1.0, s));
   }
@@ -201,6 +214,7 @@ int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t
 }
 
 int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
+  if (!V) { set_last_error("wfs_step: the power series needs V"); return WFS_ERR_INVALID; }
   WFS_TRY(ensure_scratch(p));
   const double inv_2m = 1.0 / (2.0 * p->mass);
   const double inv_dx2 = 1.0 / (p->dx[0] * p->dx[0]);   // x spacing on every axis (power_series_operator.cu:58)
@@ -240,6 +254,7 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
   for (int i = 0; i < 2; ++i)
     if (p->scratch[i]) (void)hipFree(p->scratch[i]);
   if (p->obs_work) (void)hipFree(p->obs_work);
+  if (p->pv) (void)hipFree(p->pv);
   if (p->pending) {
     for (int k = 0; k < WFS_K_COUNT; ++k)
       for (auto& pr : p->pending[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -274,8 +289,7 @@ extern "C" int wfs_plan_create(const int n[3], const double dx[3], double dt, do
   for (int a = 0; a < 3; ++a) { p->n[a] = n[a]; p->dx[a] = dx[a]; }
   p->dt = dt; p->mass = mass; p->precision = precision; p->method = method; p->order = order;
   p->elem = precision == WFS_C128 ? 16 : 8;
-  p->threads_rows = 256;
-  p->threads_cols = 256;
+  for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;
@@ -305,8 +319,14 @@ extern "C" int wfs_plan_create(const int n[3], const double dx[3], double dt, do
   return 0;
 }
 
+extern "C" int wfs_set_potential(wfs_plan* p, const void* V, void* stream) {
+  if (!p || !V) { set_last_error("wfs_set_potential: null argument"); return WFS_ERR_INVALID; }
+  if (p->method != WFS_METHOD_FOURIER) return 0;   // the power series reads V directly
+  return set_potential(p, V, static_cast<hipStream_t>(stream));
+}
+
 extern "C" int wfs_step(wfs_plan* p, void* psi, const void* V, int nsteps, void* stream) {
-  if (!p || !psi || !V) { set_last_error("wfs_step: null argument"); return WFS_ERR_INVALID; }
+  if (!p || !psi) { set_last_error("wfs_step: null argument"); return WFS_ERR_INVALID; }
   if (nsteps < 0) { set_last_error("wfs_step: negative step count"); return WFS_ERR_INVALID; }
   if (nsteps == 0) return 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -369,4 +389,66 @@ extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, 
   if (total_ms) *total_ms = p->total_ms[kc];
   if (reset) { p->launches[kc] = 0; p->total_ms[kc] = 0.0; }
   return 0;
+}
+
+extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
+  if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > 2) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
+  p->variant[kc] = variant;
+  return 0;
+}
+
+extern "C" int wfs_plan_get_variant(wfs_plan* p, int kc) {
+  if (!p || kc < 0 || kc >= WFS_K_COUNT) return -1;
+  return p->variant[kc];
+}
+
+// Time every tile variant of each split-step pass on scratch data and keep
+// the fastest.  Variants only change how many columns a block owns, so the
+// arithmetic per point, and hence the result, is identical for all of them.
+extern "C" int wfs_plan_autotune(wfs_plan* p, void* stream) {
+  if (!p) { set_last_error("wfs_plan_autotune: null plan"); return WFS_ERR_INVALID; }
+  if (p->method != WFS_METHOD_FOURIER) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = grid_points(p) * p->elem;
+  void* buf = nullptr;
+  void* tab = nullptr;
+  WFS_HIP_CHECK(hipMalloc(&buf, bytes));
+  hipError_t e = hipMalloc(&tab, bytes);
+  if (e != hipSuccess) { (void)hipFree(buf); set_last_error("wfs_plan_autotune: scratch allocation failed"); return WFS_ERR_ALLOC; }
+  (void)hipMemsetAsync(buf, 0, bytes, s);
+  (void)hipMemsetAsync(tab, 0, bytes, s);
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  const bool prof = p->profiling;
+  p->profiling = false;
+  struct Kind { int axis, mode; };
+  const Kind kinds[] = {{2, MODE_FWD}, {2, MODE_INV}, {2, MODE_IVF}, {1, MODE_FWD}, {1, MODE_INV}, {0, MODE_FKI}};
+  int rc = 0;
+  for (const Kind& k : kinds) {
+    const int kc = kernel_class_of(k.axis, k.mode);
+    const int keep = p->variant[kc];
+    float best = 1e30f;
+    int best_v = keep;
+    for (int v = 0; v < 3 && rc == 0; ++v) {
+      p->variant[kc] = v;
+      rc = run_pass(p, k.axis, k.mode, buf, tab, p->kfull, 1.0, s);   // warm-up
+      (void)hipEventRecord(a, s);
+      for (int r = 0; r < 3 && rc == 0; ++r) rc = run_pass(p, k.axis, k.mode, buf, tab, p->kfull, 1.0, s);
+      (void)hipEventRecord(b, s);
+      (void)hipEventSynchronize(b);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (rc == 0 && ms < best) { best = ms; best_v = v; }
+    }
+    p->variant[kc] = rc == 0 ? best_v : keep;
+    if (rc) break;
+  }
+  p->profiling = prof;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  (void)hipFree(tab);
+  return rc;
 }

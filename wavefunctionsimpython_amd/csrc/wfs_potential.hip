@@ -421,3 +421,16 @@ extern "C" int wfs_potential_operator(long count, const void* V, double dt, int 
   WFS_HIP_CHECK(hipGetLastError());
   return 0;
 }
+
+namespace wfs {
+int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s) {
+  if (count <= 0) return 0;
+  if (precision == WFS_C128)
+    hipLaunchKernelGGL(potential_table_kernel<double>, dim3(blocks_for(count)), dim3(kGenThreads), 0, s,
+                       static_cast<double2*>(pv), V, count, dt);
+  else
+    hipLaunchKernelGGL(potential_table_kernel<float>, dim3(blocks_for(count)), dim3(kGenThreads), 0, s,
+                       static_cast<float2*>(pv), V, count, dt);
+  return int(hipGetLastError());
+}
+}  // namespace wfs

@@ -119,6 +119,7 @@ class SimState:
         self.__localised_potential_hartree: torch.Tensor | None = None
         self.__localised_potential_to_visualize_hartree: torch.Tensor | None = None
         self.__plan: _abi.Plan | None = None
+        self.__potential_dirty = True   # the plan's phase table must be refreshed from V
 
     # ------------------------------------------------------------------ set-up
     def _shape(self):
@@ -184,6 +185,7 @@ class SimState:
             _abi.WFS_METHOD_FOURIER if self.__simulation_method == SimulationMethod.FOURIER
             else _abi.WFS_METHOD_POWER_SERIES, self.__order_of_approximation)
 
+        self.__potential_dirty = True
         # raw float32 dump of V (sources/sim_state.py:296-300)
         os.makedirs(os.path.join(self.__output_dir, "potential_raw"), exist_ok=True)
         self.__localised_potential_hartree.cpu().numpy().view(np.float32).tofile(
@@ -217,6 +219,7 @@ class SimState:
         if tuple(V.shape) != self._shape():
             raise ValueError("potential shape does not match the grid")
         self.__localised_potential_hartree = V
+        self.__potential_dirty = True
 
     def get_plan(self) -> _abi.Plan:
         return self.__plan
@@ -338,6 +341,7 @@ class SimState:
         self.__harmonic.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
         self.__lattice.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
         self._regenerated = True
+        self.__potential_dirty = True
 
     def potential_is_static(self) -> bool:
         """True when update_potential() can no longer change V."""
@@ -354,7 +358,10 @@ class SimState:
         """Advance psi by ``steps`` time steps in place (sources/sim_state.py:754-762)."""
         if self.__plan is None:
             raise RuntimeError("initialize_state() must be called before evolve_state()")
-        self.__plan.step(self.__wave_tensor, self.__localised_potential_hartree, int(steps))
+        if self.__potential_dirty:
+            self.__plan.set_potential(self.__localised_potential_hartree)
+            self.__potential_dirty = False
+        self.__plan.step(self.__wave_tensor, None, int(steps))
 
     # ------------------------------------------------------------------ text
     def get_sim_state_description_text(self, use_colors=False):
