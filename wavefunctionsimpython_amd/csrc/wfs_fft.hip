@@ -36,21 +36,27 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
   if constexpr (LAYOUT == LAYOUT_ROWS) { t = tid % N2; g = tid / N2; }
   else { g = tid % G; t = tid / G; }
 
-  long base;
-  long stride;
+  // Addresses: a block-uniform base pointer plus a 32-bit per-lane element
+  // offset, so loads and stores can use the scalar-base addressing form (one
+  // VGPR per address instead of two).  Offsets stay below 2^31 elements for
+  // every supported grid (at most 512^3 points).
+  C* __restrict__ blk;
+  unsigned loff;          // lane offset of element i = 0
+  unsigned step;          // offset between consecutive elements of the column
   bool active;
-  long col = 0;  // COLS: column index inside one outer slab
+  long col = 0;           // ROWS: row index; COLS: column index inside one outer slab
   if constexpr (LAYOUT == LAYOUT_ROWS) {
-    const long row = long(blockIdx.x) * G + g;
-    active = row < p.rows;
-    base = row * N;
-    stride = 1;
-    col = row;
+    blk = data + long(blockIdx.x) * G * N;
+    col = long(blockIdx.x) * G + g;
+    active = col < p.rows;
+    loff = unsigned(g * N + t);
+    step = N2;
   } else {
+    blk = data + long(blockIdx.y) * N * p.inner + long(blockIdx.x) * G;
     col = long(blockIdx.x) * G + g;
     active = col < p.inner;
-    base = long(blockIdx.y) * N * p.inner + col;
-    stride = p.inner;
+    loff = unsigned(g) + unsigned(t) * unsigned(p.inner);
+    step = unsigned(N2) * unsigned(p.inner);
   }
 
   auto lidx = [&](int k1, int n2) -> int {
@@ -66,7 +72,7 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
   C a[N1];
   if (active) {
 #pragma unroll
-    for (int i = 0; i < N1; ++i) a[i] = data[base + long(t + N2 * i) * stride];
+    for (int i = 0; i < N1; ++i) a[i] = blk[loff + unsigned(i) * step];
   } else {
 #pragma unroll
     for (int i = 0; i < N1; ++i) a[i] = mk<R>(R(0), R(0));
@@ -116,10 +122,10 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
     // pv holds cexp_i(-dt V) = (cos(-dt Vr), sin(-dt Vr)) exp(dt Vi) per point
     // (sources/cuda_kernels/common.cu:98-111, potential_operator.cu:10-15).
     transform(std::integral_constant<int, +1>{});
-    const C* __restrict__ pv = reinterpret_cast<const C*>(p.pv);
+    const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + long(blockIdx.x) * G * N;
     if (active) {
 #pragma unroll
-      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], pv[base + t + N2 * i]);
+      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], pvb[loff + unsigned(i) * step]);
     }
     __syncthreads();
     transform(std::integral_constant<int, -1>{});
@@ -127,7 +133,7 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
 
   if (active) {
 #pragma unroll
-    for (int i = 0; i < N1; ++i) data[base + long(t + N2 * i) * stride] = a[i];
+    for (int i = 0; i < N1; ++i) blk[loff + unsigned(i) * step] = a[i];
   }
 }
 

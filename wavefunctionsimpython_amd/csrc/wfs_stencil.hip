@@ -13,9 +13,10 @@
 // falls back to the voxel itself at a face (edge replication) and whose sum is
 // divided by dx_x^2 on every axis (power_series_operator.cu:23-58).
 //
-// Each thread owns one (y, z) column of a block tile and marches along x,
-// keeping u(x-1), u(x), u(x+1) in registers; the y/z neighbours come from the
-// same plane, which the block's other lanes load at the same time (L1/L2 hits).
+// Each thread owns one (y, z) column of a block tile and marches along x over
+// a 16-plane slab, keeping u(x-1), u(x), u(x+1) in registers; the y/z
+// neighbours come from the same plane, which the block's other lanes load at
+// the same time (L1/L2 hits).
 #include "wfs_internal.h"
 
 namespace wfs {
@@ -73,8 +74,10 @@ __global__ __launch_bounds__(TZ* TY) void horner_stage_kernel(const cx<R>* __res
 int launch_horner_stage(int precision, int nx, int ny, int nz, const void* u_in, void* u_out, const void* psi,
                         const float2* V, double dt_over_n, double inv_2m, double inv_dx2, int tile_x,
                         hipStream_t s) {
+  // 64 z x 4 y threads per block, each marching 16 x planes: 16x more blocks
+  // than whole-column marching (which left one block per CU, latency bound).
   constexpr int TZ = 64, TY = 4;
-  if (tile_x <= 0) tile_x = nx;
+  if (tile_x <= 0) tile_x = nx < 16 ? nx : 16;
   dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, (nx + tile_x - 1) / tile_x);
   dim3 block(TZ, TY, 1);
   if (precision == WFS_C128) {
