@@ -67,6 +67,17 @@ const char* wfs_last_error(void);
 int wfs_plan_create(const int n[3], const double dx[3], double dt, double mass, int precision, int method,
                     int order, wfs_plan** out);
 
+/*
+ * Plan for `batch` independent grids of identical shape, voxel size, dt and
+ * mass, stored back to back (packet b starts at b * n[0]*n[1]*n[2]).  Every
+ * entry point then works on the whole stack in one launch per pass; each
+ * packet's result is bit-identical to running it alone.  Replaces nothing in
+ * the reference (one SimState per process, sources/sim_state.py:21-60).
+ * wfs_observables on a batched plan writes out[b][box][7].
+ */
+int wfs_plan_create_batched(const int n[3], const double dx[3], double dt, double mass, int precision, int method,
+                            int order, int batch, wfs_plan** out);
+
 /* Release a plan and its device tables. */
 int wfs_plan_destroy(wfs_plan* plan);
 
@@ -192,6 +203,23 @@ int wfs_plan_autotune(wfs_plan* plan, void* stream);
 int wfs_plan_set_variant(wfs_plan* plan, int kernel_class, int variant);
 int wfs_plan_get_variant(wfs_plan* plan, int kernel_class);
 int wfs_plan_profile(wfs_plan* plan, int kernel_class, int reset, long* launches, double* total_ms);
+
+/*
+ * hipGraph capture of library calls (no reference counterpart: the reference
+ * launches every kernel from Python, sources/core_sim.py:42-55).
+ * wfs_graph_begin starts capturing `stream` (not the null stream); every
+ * wfs_* call ordered on it (and on streams joined to it by events) is
+ * recorded instead of executed; wfs_graph_end instantiates the graph;
+ * wfs_graph_launch replays it.  Plans used inside must not allocate during
+ * capture (power-series scratch is allocated by a first uncaptured wfs_step;
+ * wfs_set_potential must have been called once before capture).
+ */
+typedef struct wfs_graph wfs_graph;
+int wfs_graph_begin(void* stream, wfs_graph** out);
+int wfs_graph_end(wfs_graph* graph, void* stream);
+int wfs_graph_launch(wfs_graph* graph, void* stream);
+long wfs_graph_node_count(wfs_graph* graph);
+int wfs_graph_destroy(wfs_graph* graph);
 
 #ifdef __cplusplus
 }

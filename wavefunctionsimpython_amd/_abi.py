@@ -40,7 +40,14 @@ SIGNATURES = {
     "wfs_last_error": (ctypes.c_char_p, []),
     "wfs_plan_create": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "wfs_plan_create_batched": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "wfs_plan_destroy": (ctypes.c_int, [_vp]),
+    "wfs_graph_begin": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "wfs_graph_end": (ctypes.c_int, [_vp, _vp]),
+    "wfs_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "wfs_graph_node_count": (ctypes.c_long, [_vp]),
+    "wfs_graph_destroy": (ctypes.c_int, [_vp]),
     "wfs_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "wfs_fft3d": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "wfs_observables": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(WfsBox), ctypes.c_int, _vp, _vp]),
@@ -149,7 +156,8 @@ def ptr(t: torch.Tensor) -> ctypes.c_void_p:
 class Plan:
     """Owner of one ``wfs_plan`` (twiddle and kinetic tables, scratch buffers)."""
 
-    def __init__(self, n, dx, dt: float, mass: float, precision: int, method: int, order: int = 10):
+    def __init__(self, n, dx, dt: float, mass: float, precision: int, method: int, order: int = 10,
+                 batch: int = 1):
         n = tuple(int(v) for v in n)
         for v in n:
             if v not in SUPPORTED_LENGTHS:
@@ -161,11 +169,16 @@ class Plan:
         self.precision = int(precision)
         self.method = int(method)
         self.order = int(order)
+        self.batch = int(batch)
+        if self.batch < 1:
+            raise ValueError("batch must be >= 1")
+        # tensor shape of the (stacked) state: (n0, n1, n2) or (batch, n0, n1, n2)
+        self.shape = n if self.batch == 1 else (self.batch,) + n
         self._h = ctypes.c_void_p()
         self._potential_set = False
         self._V_ref = None
-        check(lib().wfs_plan_create(i3(n), d3(dx), self.dt, self.mass, self.precision, self.method,
-                                    self.order, ctypes.byref(self._h)), "wfs_plan_create")
+        check(lib().wfs_plan_create_batched(i3(n), d3(dx), self.dt, self.mass, self.precision, self.method,
+                                            self.order, self.batch, ctypes.byref(self._h)), "wfs_plan_create")
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -173,11 +186,11 @@ class Plan:
 
     def step(self, psi: torch.Tensor, V: torch.Tensor | None, nsteps: int = 1, stream=None) -> None:
         """Advance psi by nsteps; V=None reuses the potential given to set_potential()."""
-        check_state(psi, self.n, self.precision)
+        check_state(psi, self.shape, self.precision)
         if V is None and self.method == WFS_METHOD_POWER_SERIES:
             V = self._V_ref
         if V is not None:
-            check_potential(V, self.n)
+            check_potential(V, self.shape)
         elif not self._potential_set:
             raise ValueError("no potential: pass V or call set_potential() first")
         if nsteps < 0:
@@ -190,7 +203,7 @@ class Plan:
                 self._V_ref = V
 
     def set_potential(self, V: torch.Tensor, stream=None) -> None:
-        check_potential(V, self.n)
+        check_potential(V, self.shape)
         check(lib().wfs_set_potential(self._h, ptr(V), stream_handle(stream)), "wfs_set_potential")
         self._potential_set = True
         self._V_ref = V if self.method == WFS_METHOD_POWER_SERIES else None
@@ -204,12 +217,12 @@ class Plan:
               "wfs_plan_set_variant")
 
     def fft3d(self, data: torch.Tensor, direction: int, stream=None) -> None:
-        check_state(data, self.n, self.precision, "data")
+        check_state(data, self.shape, self.precision, "data")
         check(lib().wfs_fft3d(self._h, ptr(data), int(direction), stream_handle(stream)), "wfs_fft3d")
 
     def observables(self, psi: torch.Tensor, boxes, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
-        """Moments of |psi|^2 per box; returns a device float64 tensor (nboxes, 7)."""
-        check_state(psi, self.n, self.precision)
+        """Moments of |psi|^2 per box: device float64 tensor (nboxes, 7), or (batch, nboxes, 7)."""
+        check_state(psi, self.shape, self.precision)
         nb = len(boxes)
         arr = (WfsBox * max(nb, 1))()
         for i, (bottom, count) in enumerate(boxes):
@@ -220,8 +233,10 @@ class Plan:
                 arr[i].bottom[a] = b
                 arr[i].count[a] = c
         if out is None:
-            out = torch.empty((nb, 7), dtype=torch.float64, device=psi.device)
-        elif out.dtype != torch.float64 or out.numel() < 7 * nb or not out.is_contiguous() or out.device != psi.device:
+            shape = (nb, 7) if self.batch == 1 else (self.batch, nb, 7)
+            out = torch.empty(shape, dtype=torch.float64, device=psi.device)
+        elif (out.dtype != torch.float64 or out.numel() < 7 * nb * self.batch or not out.is_contiguous()
+              or out.device != psi.device):
             raise ValueError("out must be a contiguous float64 device tensor with 7 entries per box")
         if nb:
             check(lib().wfs_observables(self._h, ptr(psi), arr, nb, ptr(out), stream_handle(stream)),
@@ -246,6 +261,45 @@ class Plan:
     def close(self) -> None:
         if self._h and self._h.value:
             lib().wfs_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """hipGraph of library calls captured on ``stream`` (``with Graph(stream) as g: ...``)."""
+
+    def __init__(self, stream: torch.cuda.Stream):
+        if stream is None or stream.cuda_stream == 0:
+            raise ValueError("capture needs a non-default stream")
+        self.stream = stream
+        self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        check(lib().wfs_graph_begin(ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(self._h)),
+              "wfs_graph_begin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        rc = lib().wfs_graph_end(self._h, ctypes.c_void_p(self.stream.cuda_stream))
+        if exc_type is None:
+            check(rc, "wfs_graph_end")
+        return False
+
+    def launch(self, stream=None) -> None:
+        s = stream if stream is not None else self.stream
+        check(lib().wfs_graph_launch(self._h, ctypes.c_void_p(s.cuda_stream)), "wfs_graph_launch")
+
+    def node_count(self) -> int:
+        return int(lib().wfs_graph_node_count(self._h))
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            lib().wfs_graph_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

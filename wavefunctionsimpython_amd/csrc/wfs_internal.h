@@ -32,13 +32,14 @@ int launch_fft_pass(int precision, int n, int layout, int mode, void* data, cons
 int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s);
 
 // power series: one Horner stage  u_out = psi + (-i dt / n) H u_in
-int launch_horner_stage(int precision, int nx, int ny, int nz, const void* u_in, void* u_out, const void* psi,
-                        const float2* V, double dt_over_n, double inv_2m, double inv_dx2, int tile_x,
-                        hipStream_t s);
+int launch_horner_stage(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
+                        const void* psi, const float2* V, double dt_over_n, double inv_2m, double inv_dx2,
+                        int tile_x, hipStream_t s);
 
-// observables over one box: writes 7 doubles to out (device)
-int launch_box_moments(int precision, int nx, int ny, int nz, const double dx[3], const void* psi,
-                       const wfs_box& box, double* partials, long max_partials, double* out, hipStream_t s);
+// observables over one box of every packet: writes 7 doubles per packet to out (device)
+int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const double dx[3], const void* psi,
+                       const wfs_box& box, double* partials, long max_partials, double* out, long out_stride,
+                       hipStream_t s);
 long box_moments_partials_needed(const wfs_box& box);
 
 }  // namespace wfs

@@ -12,8 +12,10 @@
 //
 // Each block owns a run of (x, y) rows of the box; a wavefront reduces with
 // cross-lane shuffles, waves combine through LDS, and the block writes one
-// partial record.  A second single-block kernel sums the partials in a fixed
-// order, so results are bit-reproducible run to run.
+// partial record.  A second kernel (one block per packet) sums the partials in
+// a fixed order, so results are bit-reproducible run to run.  A stack of
+// independent grids of the same shape is handled by one launch
+// (blockIdx.y = packet).
 #include "wfs_internal.h"
 
 namespace wfs {
@@ -38,6 +40,9 @@ __global__ __launch_bounds__(kObsThreads) void box_moments_kernel(const cx<R>* _
                                                                   int nz, double dx, double dy, double dz,
                                                                   wfs_box box, double* __restrict__ partials) {
   const long nrows = long(box.count[0]) * box.count[1];
+  // packet blockIdx.y of a stack of independent grids
+  psi += long(blockIdx.y) * nx * ny * nz;
+  partials += long(blockIdx.y) * gridDim.x * kObsQ;
   const long row0 = long(blockIdx.x) * kRowsPerBlock;
   double acc[kObsQ];
 #pragma unroll
@@ -90,7 +95,11 @@ __global__ __launch_bounds__(kObsThreads) void box_moments_kernel(const cx<R>* _
 }
 
 __global__ __launch_bounds__(kObsThreads) void sum_partials_kernel(const double* __restrict__ partials, long nparts,
-                                                                   double scale, double* __restrict__ out) {
+                                                                   double scale, double* __restrict__ out,
+                                                                   long out_stride) {
+  // one block per packet
+  partials += long(blockIdx.x) * nparts * kObsQ;
+  out += long(blockIdx.x) * out_stride;
   double acc[kObsQ];
 #pragma unroll
   for (int q = 0; q < kObsQ; ++q) acc[q] = 0.0;
@@ -120,26 +129,29 @@ long box_moments_partials_needed(const wfs_box& box) {
   return ((nrows + kRowsPerBlock - 1) / kRowsPerBlock) * kObsQ;
 }
 
-int launch_box_moments(int precision, int nx, int ny, int nz, const double dx[3], const void* psi,
-                       const wfs_box& box, double* partials, long max_partials, double* out, hipStream_t s) {
+// out[b * out_stride + q], q < 7, for every packet b < batch
+int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const double dx[3], const void* psi,
+                       const wfs_box& box, double* partials, long max_partials, double* out, long out_stride,
+                       hipStream_t s) {
   const long nrows = long(box.count[0]) * box.count[1];
   const long nblocks = (nrows + kRowsPerBlock - 1) / kRowsPerBlock;
-  if (nblocks * kObsQ > max_partials) {
+  if (nblocks * kObsQ * batch > max_partials) {
     set_last_error("observable workspace too small");
     return WFS_ERR_INVALID;
   }
   // the Simpson weight products carry 1/27 and the volume element dx dy dz
   const double scale = dx[0] * dx[1] * dx[2] / 27.0;
   if (precision == WFS_C128) {
-    hipLaunchKernelGGL(box_moments_kernel<double>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
+    hipLaunchKernelGGL(box_moments_kernel<double>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
                        static_cast<const double2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], box, partials);
   } else {
-    hipLaunchKernelGGL(box_moments_kernel<float>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
+    hipLaunchKernelGGL(box_moments_kernel<float>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
                        static_cast<const float2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], box, partials);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kObsThreads), 0, s, partials, nblocks, scale, out);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(batch)), dim3(kObsThreads), 0, s, partials, nblocks, scale, out,
+                     out_stride);
   return int(hipGetLastError());
 }
 

@@ -9,6 +9,7 @@
 
 struct wfs_plan {
   int n[3];
+  int batch;              // independent grids stacked along x (same shape and parameters)
   double dx[3];
   double dt, mass;
   int precision, method, order;
@@ -103,7 +104,7 @@ void kinetic_table(long n, double dx, double dt, double mass, double f, std::vec
 
 int ensure_scratch(wfs_plan* p) {
   if (p->scratch[0]) return 0;
-  const size_t bytes = size_t(p->n[0]) * p->n[1] * p->n[2] * p->elem;
+  const size_t bytes = size_t(p->batch) * p->n[0] * p->n[1] * p->n[2] * p->elem;
   for (int i = 0; i < 2; ++i) {
     hipError_t e = hipMalloc(&p->scratch[i], bytes);
     if (e != hipSuccess) { set_last_error("hipMalloc(%zu) for power-series scratch failed", bytes); return WFS_ERR_ALLOC; }
@@ -159,16 +160,17 @@ int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* pv, const 
   pp.scale = scale;
   long outer = 1;
   int layout;
+  // stacked packets simply add rows (z pass), x slabs (y pass) or outer slabs (x pass)
   if (axis == 2) {
     layout = LAYOUT_ROWS;
-    pp.rows = long(p->n[0]) * p->n[1];
+    pp.rows = long(p->batch) * p->n[0] * p->n[1];
   } else if (axis == 1) {
     layout = LAYOUT_COLS;
-    outer = p->n[0];
+    outer = long(p->batch) * p->n[0];
     pp.inner = p->n[2];
   } else {
     layout = LAYOUT_COLS;
-    outer = 1;
+    outer = p->batch;
     pp.inner = long(p->n[1]) * p->n[2];
   }
   if (mode == MODE_FKI) {
@@ -181,7 +183,7 @@ int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* pv, const 
   return launch_fft_pass(p->precision, p->n[axis], layout, mode, data, pp, outer, p->variant[kc], s);
 }
 
-size_t grid_points(const wfs_plan* p) { return size_t(p->n[0]) * p->n[1] * p->n[2]; }
+size_t grid_points(const wfs_plan* p) { return size_t(p->batch) * p->n[0] * p->n[1] * p->n[2]; }
 
 int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
   if (!p->pv) {
@@ -218,14 +220,14 @@ int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStr
   WFS_TRY(ensure_scratch(p));
   const double inv_2m = 1.0 / (2.0 * p->mass);
   const double inv_dx2 = 1.0 / (p->dx[0] * p->dx[0]);   // x spacing on every axis (power_series_operator.cu:58)
-  const size_t bytes = size_t(p->n[0]) * p->n[1] * p->n[2] * p->elem;
+  const size_t bytes = grid_points(p) * p->elem;
   for (int step = 0; step < nsteps; ++step) {
     const void* u_in = psi;
     int which = 0;
     for (int nn = p->order; nn >= 1; --nn) {
       void* out = (nn == 1 && u_in != psi) ? psi : p->scratch[which];
       ProfileScope scope(p, WFS_K_HORNER, s);
-      WFS_TRY(launch_horner_stage(p->precision, p->n[0], p->n[1], p->n[2], u_in, out, psi,
+      WFS_TRY(launch_horner_stage(p->precision, p->batch, p->n[0], p->n[1], p->n[2], u_in, out, psi,
                                   static_cast<const float2*>(V), p->dt / double(nn), inv_2m, inv_dx2, 0, s));
       u_in = out;
       which ^= 1;
@@ -270,7 +272,13 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
 
 extern "C" int wfs_plan_create(const int n[3], const double dx[3], double dt, double mass, int precision,
                                int method, int order, wfs_plan** out) {
+  return wfs_plan_create_batched(n, dx, dt, mass, precision, method, order, 1, out);
+}
+
+extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], double dt, double mass, int precision,
+                                       int method, int order, int batch, wfs_plan** out) {
   if (!out || !n || !dx) { set_last_error("wfs_plan_create: null argument"); return WFS_ERR_INVALID; }
+  if (batch < 1) { set_last_error("wfs_plan_create: batch must be >= 1"); return WFS_ERR_INVALID; }
   *out = nullptr;
   for (int a = 0; a < 3; ++a) {
     if (!fft_length_supported(n[a])) {
@@ -287,6 +295,7 @@ extern "C" int wfs_plan_create(const int n[3], const double dx[3], double dt, do
   wfs_plan* p = new wfs_plan();
   std::memset(p, 0, sizeof(*p));
   for (int a = 0; a < 3; ++a) { p->n[a] = n[a]; p->dx[a] = dx[a]; }
+  p->batch = batch;
   p->dt = dt; p->mass = mass; p->precision = precision; p->method = method; p->order = order;
   p->elem = precision == WFS_C128 ? 16 : 8;
   for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
@@ -310,7 +319,7 @@ extern "C" int wfs_plan_create(const int n[3], const double dx[3], double dt, do
   }
   if (rc == 0) {
     wfs_box whole{{0, 0, 0}, {n[0], n[1], n[2]}};
-    p->obs_work_len = box_moments_partials_needed(whole);
+    p->obs_work_len = box_moments_partials_needed(whole) * batch;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->obs_work), size_t(p->obs_work_len) * sizeof(double));
     if (e != hipSuccess) { set_last_error("hipMalloc for observable workspace failed"); rc = WFS_ERR_ALLOC; }
   }
@@ -359,8 +368,8 @@ extern "C" int wfs_observables(wfs_plan* p, const void* psi, const wfs_box* boxe
       }
     }
     ProfileScope scope(p, WFS_K_OBSERVABLES, s);
-    const int rc = launch_box_moments(p->precision, p->n[0], p->n[1], p->n[2], p->dx, psi, bx, p->obs_work,
-                                      p->obs_work_len, out + 7 * b, s);
+    const int rc = launch_box_moments(p->precision, p->batch, p->n[0], p->n[1], p->n[2], p->dx, psi, bx,
+                                      p->obs_work, p->obs_work_len, out + 7 * b, 7L * nboxes, s);
     if (rc) return rc;
   }
   return 0;
@@ -451,4 +460,48 @@ extern "C" int wfs_plan_autotune(wfs_plan* p, void* stream) {
   (void)hipFree(buf);
   (void)hipFree(tab);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// hipGraph capture of a stream-ordered sequence of library calls
+// ---------------------------------------------------------------------------
+struct wfs_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int wfs_graph_begin(void* stream, wfs_graph** out) {
+  if (!out || !stream) { set_last_error("wfs_graph_begin: needs a non-null stream"); return WFS_ERR_INVALID; }
+  *out = nullptr;
+  WFS_HIP_CHECK(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+  *out = new wfs_graph();
+  return 0;
+}
+
+extern "C" int wfs_graph_end(wfs_graph* g, void* stream) {
+  if (!g || !stream) { set_last_error("wfs_graph_end: bad argument"); return WFS_ERR_INVALID; }
+  WFS_HIP_CHECK(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g->graph));
+  WFS_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int wfs_graph_launch(wfs_graph* g, void* stream) {
+  if (!g || !g->exec) { set_last_error("wfs_graph_launch: graph not instantiated"); return WFS_ERR_INVALID; }
+  WFS_HIP_CHECK(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+extern "C" long wfs_graph_node_count(wfs_graph* g) {
+  if (!g || !g->graph) return -1;
+  size_t n = 0;
+  if (hipGraphGetNodes(g->graph, nullptr, &n) != hipSuccess) return -1;
+  return long(n);
+}
+
+extern "C" int wfs_graph_destroy(wfs_graph* g) {
+  if (!g) return 0;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return 0;
 }

@@ -25,12 +25,18 @@ template <typename R, int TZ, int TY>
 __global__ __launch_bounds__(TZ* TY) void horner_stage_kernel(const cx<R>* __restrict__ u_in, cx<R>* u_out,
                                                              const cx<R>* psi, const float2* __restrict__ V,
                                                              int nx, int ny, int nz, R dt_over_n, R inv_2m,
-                                                             R inv_dx2, int tile_x) {
+                                                             R inv_dx2, int tile_x, int ntiles) {
   using C = cx<R>;
   const int z = blockIdx.x * TZ + threadIdx.x;
   const int y = blockIdx.y * TY + threadIdx.y;
-  const int x0 = blockIdx.z * tile_x;
+  const int packet = blockIdx.z / ntiles;          // independent grids stacked along x
+  const int x0 = (blockIdx.z % ntiles) * tile_x;
   if (z >= nz || y >= ny) return;
+  const long poff = long(packet) * nx * ny * nz;
+  u_in += poff;
+  u_out += poff;
+  psi += poff;
+  V += poff;
   int x1 = x0 + tile_x;
   if (x1 > nx) x1 = nx;
   const long sx = long(ny) * nz;
@@ -71,24 +77,25 @@ __global__ __launch_bounds__(TZ* TY) void horner_stage_kernel(const cx<R>* __res
   }
 }
 
-int launch_horner_stage(int precision, int nx, int ny, int nz, const void* u_in, void* u_out, const void* psi,
-                        const float2* V, double dt_over_n, double inv_2m, double inv_dx2, int tile_x,
-                        hipStream_t s) {
+int launch_horner_stage(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
+                        const void* psi, const float2* V, double dt_over_n, double inv_2m, double inv_dx2,
+                        int tile_x, hipStream_t s) {
   // 64 z x 4 y threads per block, each marching 16 x planes: 16x more blocks
   // than whole-column marching (which left one block per CU, latency bound).
   constexpr int TZ = 64, TY = 4;
   if (tile_x <= 0) tile_x = nx < 16 ? nx : 16;
-  dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, (nx + tile_x - 1) / tile_x);
+  const int ntiles = (nx + tile_x - 1) / tile_x;
+  dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
   dim3 block(TZ, TY, 1);
   if (precision == WFS_C128) {
     hipLaunchKernelGGL((horner_stage_kernel<double, TZ, TY>), grid, block, 0, s,
                        static_cast<const double2*>(u_in), static_cast<double2*>(u_out),
-                       static_cast<const double2*>(psi), V, nx, ny, nz, dt_over_n, inv_2m, inv_dx2, tile_x);
+                       static_cast<const double2*>(psi), V, nx, ny, nz, dt_over_n, inv_2m, inv_dx2, tile_x, ntiles);
   } else {
     hipLaunchKernelGGL((horner_stage_kernel<float, TZ, TY>), grid, block, 0, s,
                        static_cast<const float2*>(u_in), static_cast<float2*>(u_out),
                        static_cast<const float2*>(psi), V, nx, ny, nz, float(dt_over_n), float(inv_2m),
-                       float(inv_dx2), tile_x);
+                       float(inv_dx2), tile_x, ntiles);
   }
   return int(hipGetLastError());
 }
