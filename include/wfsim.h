@@ -121,6 +121,18 @@ int wfs_observables(wfs_plan* plan, const void* psi, const wfs_box* boxes, int n
                     void* stream);
 
 /*
+ * Probability current through a plane (sources/measurement.py:190-302,
+ * sources/cuda_kernels/probability_current_density.cu:4-218).  count[2] are
+ * the (odd) pixel counts, size[2] the plane's width and height in Bohr radii,
+ * normal a unit vector.  out (device, 1 double) = Simpson integral of j.n over
+ * the plane (FP64 accumulation); density (device, optional, state precision
+ * real) = j.n per pixel as [count[0]][count[1]].  Pixels outside the grid
+ * contribute zero.
+ */
+int wfs_plane_current(wfs_plan* plan, const void* psi, const double center[3], const double normal[3],
+                      const double size[2], const int count[2], void* density, double* out, void* stream);
+
+/*
  * Gaussian wave packet (sources/cuda_kernels/gaussian_wave_packet.cu:6-43,
  * common.cu:565-605), as launched by GaussianWavePacket.init_wave_packet
  * (sources/wave_packet.py:97-151).  Writes (overwrites) psi.

@@ -53,7 +53,7 @@ __all__ = [
     "potential_wall", "harmonic_potential", "lattice_potential",
     "predict_free_space_standard_deviation",
     "predict_free_space_expected_location", "indefinite_simpson_integral",
-    "relative_l2",
+    "relative_l2", "plane_current", "plane_sample_count",
 ]
 
 _WORKERS = int(os.environ.get("WFS_ORACLE_WORKERS", "1"))
@@ -348,6 +348,75 @@ def expected_location(psi, dx, bottom_voxel, sample_count):
 def standard_deviation(e_r, e_r2):
     """sqrt(E[r^2] - E[r]^2) per axis -- sources/measurement.py:394-400."""
     return np.sqrt(np.asarray(e_r2) - np.asarray(e_r) ** 2)
+
+
+def plane_sample_count(resolution):
+    """resolution + 1, an even count reduced by one -- sources/measurement.py:218-223."""
+    c = np.array(resolution, dtype=np.int64) + 1
+    return np.where(c % 2 == 0, c - 1, c)
+
+
+def plane_current(psi, dx, center, normal, size, count, mass, single_precision=True):
+    """Flux of j = Im(conj(psi) grad psi)/m through a sampled plane, and the j.n map.
+
+    sources/cuda_kernels/probability_current_density.cu: in-plane axes (:41-49,
+    float32), pixel positions (:56-57, float32), voxel = trunc(r/dx + N/2)
+    (:61-62, state precision), 6th-order central difference whose k-step
+    neighbour falls back to the centre voxel off the grid (:68-176),
+    Re[(-1/2m) i (conj(psi) dpsi - psi conj(dpsi))] (:184-192), 2-D Simpson
+    weights / 9 times dW dH (:198-202, common.cu:525-543).  Pixels whose voxel
+    lies outside the grid contribute 0.  Returns (flux, density[cu, cv]).
+    """
+    f = np.float32
+    cu, cv = int(count[0]), int(count[1])
+    n = np.array(normal, dtype=f)
+    pu = np.array([0, 1, 0], dtype=f)
+    if abs(n[0] * pu[0] + n[1] * pu[1] + n[2] * pu[2]) > 0.99:
+        pu[1] = f(0.0)
+        pu[2] = f(1.0)
+    r = np.array([n[1] * pu[2] - n[2] * pu[1], n[2] * pu[0] - n[0] * pu[2], n[0] * pu[1] - n[1] * pu[0]], dtype=f)
+    r = r / np.sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2])
+    u = np.array([r[1] * n[2] - r[2] * n[1], r[2] * n[0] - r[0] * n[2], r[0] * n[1] - r[1] * n[0]], dtype=f)
+    u = u / np.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
+    if single_precision:
+        dW = f(size[0]) / f(cu - 1)
+        dH = f(size[1]) / f(cv - 1)
+        T = np.float32
+    else:
+        dW = float(size[0]) / (cu - 1)
+        dH = float(size[1]) / (cv - 1)
+        T = np.float64
+    si = (np.arange(cu) - cu // 2).astype(f)[:, None]
+    sj = (np.arange(cv) - cv // 2).astype(f)[None, :]
+    c = np.array(center, dtype=f)
+    N = psi.shape
+    vox = []
+    inside = np.ones((cu, cv), dtype=bool)
+    for a in range(3):
+        fr = (c[a] + r[a] * f(dW) * si) + u[a] * f(dH) * sj
+        fv = fr.astype(T) / T(dx[a]) + T(0.5) * T(N[a])
+        inside &= (fv >= 0) & (fv < N[a])
+        vox.append(np.where(inside, fv, 0).astype(np.int64))
+    vox = [np.where(inside, v, 0) for v in vox]
+    p = psi.astype(np.complex128)
+    p0 = p[vox[0], vox[1], vox[2]]
+    dpsi = np.zeros((cu, cv), dtype=np.complex128)
+    for a in range(3):
+        vals = {}
+        for k in (1, 2, 3):
+            for sgn in (-1, 1):
+                idx = [v.copy() for v in vox]
+                moved = idx[a] + sgn * k
+                ok = (moved >= 0) & (moved < N[a])
+                idx[a] = np.where(ok, moved, idx[a])
+                vals[sgn * k] = np.where(ok, p[idx[0], idx[1], idx[2]], p0)
+        g = (vals[3] - 9 * vals[2] + 45 * vals[1] - 45 * vals[-1] + 9 * vals[-2] - vals[-3]) / 60.0 / float(dx[a])
+        dpsi += float(n[a]) * g
+    jn = np.where(inside, (np.conj(p0) * dpsi).imag / float(mass), 0.0)
+    wu = simpson_weights_1d(cu) * 3
+    wv = simpson_weights_1d(cv) * 3
+    flux = float(np.einsum("i,j,ij->", wu, wv, jn) / 9.0 * float(dW) * float(dH))
+    return flux, jn
 
 
 # ---------------------------------------------------------------------------

@@ -214,3 +214,31 @@ def test_expected_location_mutates_top_voxel_like_reference(dev, tmp_path):
     _, _, mutated = oracle.expected_location_box(sim.get_observation_box_bottom_corner_voxel_3(), top_before)
     assert np.array_equal(top_before, tv)
     assert np.array_equal(sim.get_observation_box_top_corner_voxel_3(), mutated)
+
+
+def test_plane_current_measurement_obeys_continuity(dev, tmp_path):
+    """Probability gained by the half box x > 0 equals the time integral of the flux through x = 0."""
+    cfg = double_slit_config(str(tmp_path))
+    cfg["potential"]["walls"] = []
+    cfg["absorbing_boundary_condition"]["enable"] = False
+    cfg["simulation"]["total_iteration_count"] = 120
+    cfg["simulation"]["double_precision_calculation"] = True
+    cfg["wave_packet"]["initial_position_bohr_radii_3"] = [-2.0, 0.0, 0.0]
+    cfg["measurement"]["volume_probabilities"] = [{"name": "right", "bottom_corner_bohr_radii_3": [0.0, -9.9, -9.9],
+                                                   "top_corner_bohr_radii_3": [9.6, 9.6, 9.6]}]
+    cfg["measurement"]["plane_probability_currents"] = [{"name": "x0", "center_bohr_radii_3": [0.0, 0.0, 0.0],
+                                                         "normal_vector_3": [1.0, 0.0, 0.0],
+                                                         "size_bohr_radii_2": [19.0, 19.0],
+                                                         "resolution_2": [60, 60]}]
+    sim, tools, it = initialisation.initialize(cfg, use_cache=False)
+    core_sim.run_iteration(sim, tools, it, observe_every=2)
+    right, _ = tools.get_volume_probabilities()[0].get_probability_evolution_with_name()
+    flux, _ = tools.get_plane_probability_currents()[0].get_probability_current_evolution_with_name()
+    transfer, _ = tools.get_plane_probability_currents()[0].get_integrated_probability_current_evolution_with_name()
+    assert flux.shape == (60,) and flux.max() > 0.1
+    # Simpson-in-space boxes start on the x = 0 plane; compare increments
+    gained = right - right[0]
+    assert np.max(np.abs(gained[::2] - transfer[::2])) < 5e-3
+    tools.finish(sim)
+    npy = os.path.join(str(tmp_path), "output", "measurable_npy", "integrated_probability_current_x0.npy")
+    assert np.allclose(np.load(npy), transfer)

@@ -235,3 +235,25 @@ def test_operator_tables(dev):
     out = torch.empty(shape, dtype=torch.complex128, device=dev)
     _abi.potential_operator(_to_dev(V, dev), 0.05, _abi.WFS_C128, out)
     assert np.max(np.abs(out.cpu().numpy() - oracle.potential_phase(V, 0.05))) < 1e-14
+
+
+@pytest.mark.parametrize("prec,tol", [(_abi.WFS_C128, 1e-11), (_abi.WFS_C64, 2e-5)])
+@pytest.mark.parametrize("normal,center,size,count", [
+    ((1.0, 0.0, 0.0), (0.3, 0.0, 0.0), (10.0, 8.0), (41, 33)),
+    ((0.6, 0.8, 0.0), (-0.5, 0.4, 0.2), (6.0, 6.0), (25, 25)),
+    ((0.0, 1.0, 0.0), (0.0, 0.2, 0.0), (30.0, 30.0), (63, 63)),   # reaches outside the grid
+])
+def test_plane_current_matches_oracle(dev, prec, tol, normal, center, size, count):
+    shape = (64, 48, 56)
+    dx = (0.3, 0.35, 0.3)
+    dtype = np.complex128 if prec == _abi.WFS_C128 else np.complex64
+    psi = oracle.gaussian_wave_packet(shape, dx, (2.0, 2.0, 2.5), (-0.5, 0.3, 0.0), (1.2, 0.8, -0.4), dtype)
+    n = np.array(normal) / np.linalg.norm(normal)
+    p = _plan(shape, dx, 0.01, 1.3, prec)
+    dens = torch.empty(count, dtype=torch.float64 if prec == _abi.WFS_C128 else torch.float32, device=dev)
+    out = p.plane_current(_to_dev(psi, dev), center, n, size, count, density=dens)
+    flux_ref, jn_ref = oracle.plane_current(psi, dx, center, n, size, count, 1.3,
+                                            single_precision=(prec == _abi.WFS_C64))
+    got = float(out.cpu().numpy()[0])
+    assert abs(got - flux_ref) <= tol * max(abs(flux_ref), 1e-3)
+    assert np.allclose(dens.cpu().numpy(), jn_ref, rtol=tol * 10, atol=tol * np.max(np.abs(jn_ref)))

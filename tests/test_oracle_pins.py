@@ -367,3 +367,48 @@ def test_indefinite_simpson_integral_hand_worked_and_exact():
     F = lambda x: x ** 4 / 2 - x ** 2 / 2 + x
     got = oracle.indefinite_simpson_integral(f, 0.1)
     assert np.allclose(got[::2], F(t[::2]), atol=1e-12)      # even entries: pure Simpson from 0
+
+
+# ------------------------------------------------------------------ probability current
+def test_plane_current_plane_wave_flux():
+    N, h, k, A = 64, 0.25, 0.9, 0.3
+    r = oracle.voxel_coordinates(N, h)
+    psi = (A * np.exp(1j * k * r))[:, None, None] * np.ones((N, N, N))
+    flux, jn = oracle.plane_current(psi, [h] * 3, (0.1, 0.0, 0.0), (1, 0, 0), (6.0, 6.0), (31, 31), 1.0,
+                                    single_precision=False)
+    # the 6th-order central difference of exp(ikx) is i k_eff exp(ikx), k_eff -> k + O((kh)^6)
+    k_eff = (np.sin(3 * k * h) - 9 * np.sin(2 * k * h) + 45 * np.sin(k * h)) / (30 * h)
+    assert abs(flux - A * A * k_eff * 36.0) < 1e-12
+    assert np.allclose(jn, A * A * k_eff, rtol=1e-12)
+    assert abs(k_eff - k) < 1e-6
+
+
+def test_plane_current_tilted_normal_and_mass():
+    N, h = 64, 0.25
+    kx, ky, A, m = 0.7, -0.4, 0.5, 1.7
+    r = oracle.voxel_coordinates(N, h)
+    psi = A * np.exp(1j * (kx * r[:, None, None] + ky * r[None, :, None])) * np.ones((1, 1, N))
+    n = np.array([0.6, 0.8, 0.0])
+    flux, jn = oracle.plane_current(psi, [h] * 3, (0.0, 0.0, 0.0), n, (4.0, 4.0), (17, 17), m,
+                                    single_precision=False)
+    assert np.allclose(jn, A * A * (kx * n[0] + ky * n[1]) / m, rtol=1e-5)
+    assert abs(flux - A * A * (kx * n[0] + ky * n[1]) / m * 16.0) < 1e-5
+
+
+def test_plane_current_exact_for_degree_six_polynomials():
+    N, h = 40, 0.1
+    x = oracle.voxel_coordinates(N, h)
+    a = 1.0 + 0.3 * x ** 2 - 0.05 * x ** 6
+    b = 0.2 * x + 0.01 * x ** 3 + 0.002 * x ** 5
+    psi = (a + 1j * b)[:, None, None] * np.ones((N, N, N))
+    da = 0.6 * x - 0.3 * x ** 5
+    db = 0.2 + 0.03 * x ** 2 + 0.01 * x ** 4
+    want = (a * db - b * da)          # Im(conj(psi) psi') for m = 1
+    flux, jn = oracle.plane_current(psi, [h] * 3, (0.3, 0.0, 0.0), (1, 0, 0), (1.0, 1.0), (5, 5), 1.0,
+                                    single_precision=False)
+    ix = int(round(0.3 / h)) + N // 2
+    assert np.allclose(jn, want[ix], rtol=1e-10)
+
+
+def test_plane_sample_count_rule():
+    assert list(oracle.plane_sample_count([255, 512])) == [255, 513]

@@ -51,6 +51,8 @@ SIGNATURES = {
     "wfs_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "wfs_fft3d": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "wfs_observables": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(WfsBox), ctypes.c_int, _vp, _vp]),
+    "wfs_plane_current": (ctypes.c_int, [_vp, _vp, _c_dbl3, _c_dbl3, ctypes.c_double * 2, ctypes.c_int * 2, _vp,
+                                         _vp, _vp]),
     "wfs_init_gaussian": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_int, _c_dbl3, _c_dbl3, _c_dbl3, _vp, _vp]),
     "wfs_add_cap": (ctypes.c_int, [_c_int3, _c_dbl3, _c_dbl3, _c_dbl3, ctypes.c_double, _c_dbl3, _c_dbl3,
                                    _c_dbl3, _c_dbl3, _vp, _vp]),
@@ -132,12 +134,23 @@ def _require_device(t: torch.Tensor, name: str) -> None:
         raise ValueError(f"{name} must be contiguous (C-order)")
 
 
+def _same_grid(tshape, shape) -> bool:
+    """Tensor shape equals the grid; a single packet may carry a leading 1."""
+    tshape = tuple(int(s) for s in tshape)
+    shape = tuple(int(s) for s in shape)
+    if tshape == shape:
+        return True
+    if len(shape) == 3 and tshape == (1,) + shape:
+        return True
+    return len(shape) == 4 and shape[0] == 1 and tshape == shape[1:]
+
+
 def check_state(t: torch.Tensor, shape, precision: int, name: str = "psi") -> None:
     _require_device(t, name)
     want = torch.complex128 if precision == WFS_C128 else torch.complex64
     if t.dtype != want:
         raise TypeError(f"{name} must be {want}, got {t.dtype}")
-    if tuple(t.shape) != tuple(int(s) for s in shape):
+    if not _same_grid(t.shape, shape):
         raise ValueError(f"{name} shape {tuple(t.shape)} != grid {tuple(shape)}")
 
 
@@ -145,7 +158,7 @@ def check_potential(t: torch.Tensor, shape, name: str = "V") -> None:
     _require_device(t, name)
     if t.dtype != torch.complex64:
         raise TypeError(f"{name} must be complex64 (the potential is always complex64), got {t.dtype}")
-    if tuple(t.shape) != tuple(int(s) for s in shape):
+    if not _same_grid(t.shape, shape):
         raise ValueError(f"{name} shape {tuple(t.shape)} != grid {tuple(shape)}")
 
 
@@ -241,6 +254,29 @@ class Plan:
         if nb:
             check(lib().wfs_observables(self._h, ptr(psi), arr, nb, ptr(out), stream_handle(stream)),
                   "wfs_observables")
+        return out
+
+    def plane_current(self, psi: torch.Tensor, center, normal, size, count, out: torch.Tensor | None = None,
+                      density: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Flux of j.n through a sampled plane into out[0] (device float64); optional j.n map."""
+        if self.batch != 1:
+            raise ValueError("plane currents need a single-grid plan")
+        check_state(psi, self.shape, self.precision)
+        cu, cv = int(count[0]), int(count[1])
+        if cu < 2 or cv < 2:
+            raise ValueError("a plane needs at least 2 x 2 samples")
+        if out is None:
+            out = torch.empty(1, dtype=torch.float64, device=psi.device)
+        elif out.dtype != torch.float64 or out.numel() < 1 or out.device != psi.device:
+            raise ValueError("out must be a float64 device tensor")
+        if density is not None:
+            want = torch.float64 if self.precision == WFS_C128 else torch.float32
+            _require_device(density, "density")
+            if density.dtype != want or tuple(density.shape) != (cu, cv):
+                raise ValueError(f"density must be {want} of shape ({cu}, {cv})")
+        check(lib().wfs_plane_current(self._h, ptr(psi), d3(center), d3(normal), (ctypes.c_double * 2)(*size),
+                                      (ctypes.c_int * 2)(cu, cv), ptr(density) if density is not None else None,
+                                      ptr(out), stream_handle(stream)), "wfs_plane_current")
         return out
 
     def set_profiling(self, enable: bool) -> None:

@@ -53,6 +53,7 @@ def run_iteration(sim_state: SimState, measurement_tools: MeasurementTools, iter
     iter_data.is_quit = False
     previous = register_signal_handler(iter_data)
     observe_every = max(1, int(observe_every))
+    measurement_tools.set_observe_every(observe_every)
     try:
         i = iter_data.i
         total = iter_data.total_iteration_count

@@ -42,4 +42,11 @@ int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const d
                        hipStream_t s);
 long box_moments_partials_needed(const wfs_box& box);
 
+// probability current through a plane: out7[0] = flux (FP64), density optional [cu][cv]
+long plane_current_partials_needed(int cu, int cv);
+int launch_plane_current(int precision, int nx, int ny, int nz, const double dx[3], const void* psi,
+                         const double center[3], const double normal[3], const double size[2], const int count[2],
+                         double mass, void* density, double* partials, long max_partials, double* out7,
+                         hipStream_t s);
+
 }  // namespace wfs

@@ -18,6 +18,8 @@
 // (blockIdx.y = packet).
 #include "wfs_internal.h"
 
+#include <cmath>
+
 namespace wfs {
 
 constexpr int kObsThreads = 256;
@@ -152,6 +154,160 @@ int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const d
   if (e != hipSuccess) return int(e);
   hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(batch)), dim3(kObsThreads), 0, s, partials, nblocks, scale, out,
                      out_stride);
+  return int(hipGetLastError());
+}
+
+}  // namespace wfs
+
+// ---------------------------------------------------------------------------
+// Probability current through a tilted plane
+// (sources/cuda_kernels/probability_current_density.cu:33-218).
+//
+// Pixel (i, j) of a count_u x count_v grid on the plane sits at
+//   r = c + right * dW * (i - count_u/2) + up * dH * (j - count_v/2)
+// (float32, :41-58), falls into voxel floor(r/dx + N/2) (:61-62), and gets
+//   j.n = Im(conj(psi) * n.grad psi) / m
+// with the 6th-order central difference of :174-176 whose k-step neighbour
+// falls back to the centre voxel when it leaves the grid (:68-144).  The flux
+// is the 2-D Simpson integral of j.n dW dH (:198-214), accumulated in FP64.
+// The density map is written as [count_u][count_v]; pixels whose voxel lies
+// outside the grid get zero density (the reference reads out of bounds there).
+// ---------------------------------------------------------------------------
+namespace wfs {
+
+struct PlaneParams {
+  float center[3], right[3], up[3];
+  float dW, dH;            // float copies used for the pixel positions (:56-57)
+  double dWd, dHd;         // state-precision widths used for the weights (:51-52, :202)
+  double normal[3];
+  int cu, cv;
+  double inv_mass;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kObsThreads) void plane_current_kernel(const cx<R>* __restrict__ psi, int nx, int ny,
+                                                                    int nz, double dx, double dy, double dz,
+                                                                    PlaneParams p, R* __restrict__ density,
+                                                                    double* __restrict__ partials) {
+#pragma clang fp contract(off)
+  const long npix = long(p.cu) * p.cv;
+  const long pix = long(blockIdx.x) * kObsThreads + threadIdx.x;
+  double contrib = 0.0;
+  if (pix < npix) {
+    const int i = int(pix / p.cv);
+    const int j = int(pix % p.cv);
+    const float si = float(i - p.cu / 2), sj = float(j - p.cv / 2);
+    float fr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fr[a] = (p.center[a] + p.right[a] * p.dW * si) + p.up[a] * p.dH * sj;
+    const int N[3] = {nx, ny, nz};
+    const double d[3] = {dx, dy, dz};
+    int v[3];
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const R fv = R(fr[a]) / R(d[a]) + R(0.5f) * R(N[a]);
+      inside = inside && fv >= R(0) && fv < R(N[a]);
+      v[a] = inside ? int(fv) : 0;
+    }
+    R jn = R(0);
+    if (inside) {
+      const long sx = long(ny) * nz, sy = nz;
+      const long c = long(v[0]) * sx + long(v[1]) * sy + v[2];
+      const long strides[3] = {sx, sy, 1};
+      const cx<R> p0 = psi[c];
+      cx<R> grad[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        cx<R> f[7];   // offsets -3..3
+#pragma unroll
+        for (int k = 1; k <= 3; ++k) {
+          f[3 - k] = (v[a] >= k) ? psi[c - k * strides[a]] : p0;
+          f[3 + k] = (v[a] < N[a] - k) ? psi[c + k * strides[a]] : p0;
+        }
+        cx<R> g;
+        g.x = (f[6].x - R(9) * f[5].x + R(45) * f[4].x - R(45) * f[2].x + R(9) * f[1].x - f[0].x) / R(60) / R(d[a]);
+        g.y = (f[6].y - R(9) * f[5].y + R(45) * f[4].y - R(45) * f[2].y + R(9) * f[1].y - f[0].y) / R(60) / R(d[a]);
+        grad[a] = g;
+      }
+      cx<R> dpsi;
+      dpsi.x = R(p.normal[0]) * grad[0].x + R(p.normal[1]) * grad[1].x + R(p.normal[2]) * grad[2].x;
+      dpsi.y = R(p.normal[0]) * grad[0].y + R(p.normal[1]) * grad[1].y + R(p.normal[2]) * grad[2].y;
+      // (-1/2m) i (conj(psi) dpsi - psi conj(dpsi)) = Im(conj(psi) dpsi) / m
+      jn = R(p.inv_mass) * (p0.x * dpsi.y - p0.y * dpsi.x);
+    }
+    if (density) density[pix] = jn;
+    const double su = (i == 0 || i == p.cu - 1) ? 1.0 : ((i & 1) ? 4.0 : 2.0);
+    const double sv = (j == 0 || j == p.cv - 1) ? 1.0 : ((j & 1) ? 4.0 : 2.0);
+    contrib = double(jn) * su * sv;
+  }
+  __shared__ double red[kObsThreads / 64];
+  const double w = wave_sum(contrib);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kObsThreads / 64; ++k) t += red[k];
+    partials[long(blockIdx.x) * kObsQ] = t;
+#pragma unroll
+    for (int q = 1; q < kObsQ; ++q) partials[long(blockIdx.x) * kObsQ + q] = 0.0;
+  }
+}
+
+long plane_current_partials_needed(int cu, int cv) {
+  return ((long(cu) * cv + kObsThreads - 1) / kObsThreads) * kObsQ;
+}
+
+int launch_plane_current(int precision, int nx, int ny, int nz, const double dx[3], const void* psi,
+                         const double center[3], const double normal[3], const double size[2], const int count[2],
+                         double mass, void* density, double* partials, long max_partials, double* out7,
+                         hipStream_t s) {
+#pragma clang fp contract(off)
+  const long nblocks = (long(count[0]) * count[1] + kObsThreads - 1) / kObsThreads;
+  if (nblocks * kObsQ > max_partials) { set_last_error("plane current workspace too small"); return WFS_ERR_INVALID; }
+  PlaneParams p;
+  // right/up in float32 as the reference (:41-49): up = (0,1,0), or (0,0,1) when |n.up| > 0.99
+  float n[3] = {float(normal[0]), float(normal[1]), float(normal[2])};
+  float pu[3] = {0.0f, 1.0f, 0.0f};
+  if (std::fabs(n[0] * pu[0] + n[1] * pu[1] + n[2] * pu[2]) > 0.99) { pu[1] = 0.0f; pu[2] = 1.0f; }
+  float r[3] = {n[1] * pu[2] - n[2] * pu[1], n[2] * pu[0] - n[0] * pu[2], n[0] * pu[1] - n[1] * pu[0]};
+  float rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  for (int a = 0; a < 3; ++a) r[a] = r[a] / rl;
+  float u[3] = {r[1] * n[2] - r[2] * n[1], r[2] * n[0] - r[0] * n[2], r[0] * n[1] - r[1] * n[0]};
+  float ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (int a = 0; a < 3; ++a) {
+    u[a] = u[a] / ul;
+    p.center[a] = float(center[a]);
+    p.right[a] = r[a];
+    p.up[a] = u[a];
+    p.normal[a] = double(n[a]);
+  }
+  if (precision == WFS_C128) {
+    p.dWd = size[0] / double(count[0] - 1);
+    p.dHd = size[1] / double(count[1] - 1);
+  } else {
+    p.dWd = double(float(size[0]) / float(count[0] - 1));
+    p.dHd = double(float(size[1]) / float(count[1] - 1));
+  }
+  p.dW = float(p.dWd);
+  p.dH = float(p.dHd);
+  p.cu = count[0];
+  p.cv = count[1];
+  p.inv_mass = 1.0 / mass;
+  if (precision == WFS_C128)
+    hipLaunchKernelGGL(plane_current_kernel<double>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
+                       static_cast<const double2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], p,
+                       static_cast<double*>(density), partials);
+  else
+    hipLaunchKernelGGL(plane_current_kernel<float>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
+                       static_cast<const float2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], p,
+                       static_cast<float*>(density), partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return int(e);
+  // Simpson 1/9 and the pixel area dW dH
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kObsThreads), 0, s, partials, nblocks,
+                     p.dWd * p.dHd / 9.0, out7, 7L);
   return int(hipGetLastError());
 }
 

@@ -20,6 +20,9 @@ struct wfs_plan {
   void* scratch[2];       // power-series ping-pong buffers (lazy)
   double* obs_work;       // observable partial records
   long obs_work_len;
+  double* plane_work;     // plane-current partial records (grown on demand)
+  long plane_work_len;
+  double* plane_out;      // 7 doubles
   void* pv;               // potential phase table cexp_i(-dt V), state precision
   bool pv_set;
   int variant[WFS_K_COUNT];   // tile variant per kernel class (wfs_plan_autotune)
@@ -256,6 +259,8 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
   for (int i = 0; i < 2; ++i)
     if (p->scratch[i]) (void)hipFree(p->scratch[i]);
   if (p->obs_work) (void)hipFree(p->obs_work);
+  if (p->plane_work) (void)hipFree(p->plane_work);
+  if (p->plane_out) (void)hipFree(p->plane_out);
   if (p->pv) (void)hipFree(p->pv);
   if (p->pending) {
     for (int k = 0; k < WFS_K_COUNT; ++k)
@@ -503,5 +508,28 @@ extern "C" int wfs_graph_destroy(wfs_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+  return 0;
+}
+
+extern "C" int wfs_plane_current(wfs_plan* p, const void* psi, const double center[3], const double normal[3],
+                                 const double size[2], const int count[2], void* density, double* out,
+                                 void* stream) {
+  if (!p || !psi || !center || !normal || !size || !count || !out || count[0] < 2 || count[1] < 2) {
+    set_last_error("wfs_plane_current: bad argument");
+    return WFS_ERR_INVALID;
+  }
+  if (p->batch != 1) { set_last_error("wfs_plane_current: batched plans are not supported"); return WFS_ERR_UNSUPPORTED; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const long need = plane_current_partials_needed(count[0], count[1]);
+  if (need > p->plane_work_len) {
+    if (p->plane_work) { WFS_HIP_CHECK(hipStreamSynchronize(s)); (void)hipFree(p->plane_work); p->plane_work = nullptr; }
+    WFS_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p->plane_work), size_t(need) * sizeof(double)));
+    p->plane_work_len = need;
+  }
+  if (!p->plane_out) WFS_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p->plane_out), 7 * sizeof(double)));
+  ProfileScope scope(p, WFS_K_OBSERVABLES, s);
+  WFS_TRY(launch_plane_current(p->precision, p->n[0], p->n[1], p->n[2], p->dx, psi, center, normal, size, count,
+                               p->mass, density, p->plane_work, p->plane_work_len, p->plane_out, s));
+  WFS_HIP_CHECK(hipMemcpyAsync(out, p->plane_out, sizeof(double), hipMemcpyDeviceToDevice, s));
   return 0;
 }

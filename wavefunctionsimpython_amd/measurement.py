@@ -182,6 +182,7 @@ class MeasurementTools:
         if try_read_param(config, "measurement.expected_location.enable_image", False):
             self.__expected_location = ExpectedLocation(config, sim_state)
         self.__out = None
+        self.__observe_every = 1
 
     @staticmethod
     def _resolve_naming_conflicts(items, new_item):
@@ -202,6 +203,12 @@ class MeasurementTools:
                 new_item.set_name(new_item.get_name() + f"_{index}")
                 print(_RED + f"Renamed one to \"{new_item.get_name()}\"." + _RESET)
                 break
+
+    def set_observe_every(self, k: int):
+        """Samples are k steps apart (time integrals of the series use k * dt)."""
+        self.__observe_every = max(1, int(k))
+        for pc in self.__plane_probability_currents:
+            pc.set_sample_interval_steps(self.__observe_every)
 
     def get_volume_probabilities(self):
         return self.__volume_probabilities
@@ -282,7 +289,8 @@ class MeasurementTools:
             if v.is_enable_image():
                 evo, name = v.get_probability_evolution_with_name()
                 dump(f"volume_probability_evolution_{name}", evo)
-                dwell = math_utils.indefinite_simpson_integral(evo, sim_state.get_delta_time_h_bar_per_hartree())
+                dwell = math_utils.indefinite_simpson_integral(
+                    evo, sim_state.get_delta_time_h_bar_per_hartree() * self.__observe_every)
                 if dwell.size:
                     print(f"Dwell time in {name}: {dwell[-1]:.5f} ħ/Hartree")
         for pc in self.__plane_probability_currents:
