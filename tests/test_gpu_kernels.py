@@ -244,8 +244,8 @@ def test_operator_tables(dev):
     ((0.0, 1.0, 0.0), (0.0, 0.2, 0.0), (30.0, 30.0), (63, 63)),   # reaches outside the grid
 ])
 def test_plane_current_matches_oracle(dev, prec, tol, normal, center, size, count):
-    shape = (64, 48, 56)
-    dx = (0.3, 0.35, 0.3)
+    shape = (64, 32, 64)
+    dx = (0.3, 0.45, 0.3)
     dtype = np.complex128 if prec == _abi.WFS_C128 else np.complex64
     psi = oracle.gaussian_wave_packet(shape, dx, (2.0, 2.0, 2.5), (-0.5, 0.3, 0.0), (1.2, 0.8, -0.4), dtype)
     n = np.array(normal) / np.linalg.norm(normal)

@@ -412,3 +412,17 @@ def test_plane_current_exact_for_degree_six_polynomials():
 
 def test_plane_sample_count_rule():
     assert list(oracle.plane_sample_count([255, 512])) == [255, 513]
+
+
+def test_plane_current_gaussian_flux_at_matched_pitch():
+    """With the pixel pitch equal to the voxel pitch the flux of a Gaussian packet through x = 0
+    is rho_x(0) * v; a larger pitch makes the truncated-voxel sampling skip the centre voxel
+    (reference behaviour) and the integral falls short."""
+    N, L = 64, 20.0
+    h = L / N
+    psi = oracle.gaussian_wave_packet((N,) * 3, [h] * 3, (1.5,) * 3, (-2.0, 0, 0), (2.0, 0, 0))
+    rho_x0 = np.exp(-4.0 / (2 * 1.5 ** 2)) / (np.sqrt(2 * np.pi) * 1.5)
+    matched, _ = oracle.plane_current(psi, [h] * 3, (0, 0, 0), (1, 0, 0), (19.375, 19.375), (63, 63), 1.0, False)
+    assert abs(matched - 2.0 * rho_x0) < 2e-4
+    coarse, _ = oracle.plane_current(psi, [h] * 3, (0, 0, 0), (1, 0, 0), (19.0, 19.0), (61, 61), 1.0, False)
+    assert coarse < 0.9 * matched
