@@ -217,6 +217,53 @@ int wfs_plan_get_variant(wfs_plan* plan, int kernel_class);
 int wfs_plan_profile(wfs_plan* plan, int kernel_class, int reset, long* launches, double* total_ms);
 
 /*
+ * Strided batched GEMM on the bf16 matrix cores (no reference counterpart;
+ * building block of the Fourier neural operator).
+ *   C[b](m,n) = alpha * sum_k A[b](m,k) B[b](k,n) + beta * C[b](m,n)
+ * dims = {batch, M, N, K}; strides (elements) = {sAb, sAm, sAk, sBb, sBk, sBn,
+ * sCb, sCm, sCn}.  A, B, C are FP32 in memory; A and B are rounded to bf16
+ * (nearest even) on load, products accumulate in FP32.  flags: WFS_GEMM_SPLIT_A
+ * / WFS_GEMM_SPLIT_B stage that operand as bf16 hi + lo parts (about 16
+ * significant bits; Ahi Bhi + Alo Bhi + Ahi Blo).  splits > 1 splits K over
+ * blocks and needs work_elems >= batch*splits*M*N floats; the splits are
+ * added in a fixed order.
+ */
+#define WFS_GEMM_SPLIT_A 1
+#define WFS_GEMM_SPLIT_B 2
+int wfs_gemm_bf16(const float* A, const float* B, float* C, const long dims[4], const long strides[9], float alpha,
+                  float beta, int flags, int splits, float* work, long work_elems, void* stream);
+
+/*
+ * Fourier-neural-operator and parareal elementwise kernels (no reference
+ * counterpart).  Activations are channel-major FP32 planes x[c][p].
+ *  wfs_fno_bias_act       u += bias[c]; v = GELU(u) (act != 0) or v = u (v may be NULL)
+ *  wfs_fno_gelu_backward  du = dv * GELU'(u)
+ *  wfs_fno_axpby          y = alpha x + beta y (y not read when beta == 0)
+ *  wfs_fno_pack           a = [pscale Re psi, pscale Im psi, vscale Re V] planes (V may be NULL: 2 planes)
+ *  wfs_fno_unpack         complex64 psi = oscale (plane 0 + i plane 1)
+ *  wfs_fno_mse            out2 = {sum (pred-target)^2, sum target^2} (FP64);
+ *                         dpred = 2 (pred-target) gscale (optional)
+ *  wfs_fno_adam           Adam step on FP32 master weights
+ *  wfs_parareal_update    U = Gnew + Fold - Gold in FP64 (complex64 inputs, complex128 U);
+ *                         out2 = {|U - Uprev|^2, |U|^2}
+ *  wfs_narrow_c128        complex128 -> complex64
+ * Workspaces (partials) need 8192 doubles.
+ */
+int wfs_fno_bias_act(float* u, const float* bias, float* v, long C, long P, int act, void* stream);
+int wfs_fno_gelu_backward(const float* u, const float* dv, float* du, long n, void* stream);
+int wfs_fno_axpby(float* y, const float* x, float alpha, float beta, long n, void* stream);
+int wfs_fno_pack(const void* psi, int precision, const void* V, float pscale, float vscale, float* a, long P,
+                 void* stream);
+int wfs_fno_unpack(const float* out, float oscale, void* psi, long P, void* stream);
+int wfs_fno_mse(const float* pred, const float* target, float* dpred, double gscale, long n, double* partials,
+                long partial_elems, double* out2, void* stream);
+int wfs_fno_adam(float* w, const float* g, float* m, float* v, long n, float lr, float beta1, float beta2, float eps,
+                 int step, void* stream);
+int wfs_parareal_update(const void* gnew, const void* fold, const void* gold, void* U, long n, double* partials,
+                        long partial_elems, double* out2, void* stream);
+int wfs_narrow_c128(const void* U, void* out, long n, void* stream);
+
+/*
  * hipGraph capture of library calls (no reference counterpart: the reference
  * launches every kernel from Python, sources/core_sim.py:42-55).
  * wfs_graph_begin starts capturing `stream` (not the null stream); every

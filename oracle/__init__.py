@@ -7,3 +7,4 @@ and fails loudly when its HIP library is missing.
 """
 
 from .wfs_oracle import *  # noqa: F401,F403
+from . import fno_oracle  # noqa: F401,E402

@@ -1,0 +1,67 @@
+"""Float64 restatement of the FNO's device kernels, for checking its passes.
+
+``EmulatedFNO`` runs the network's own host code (every strided GEMM call and
+elementwise call of ``FNO3d``) with the device kernels replaced by float64
+torch code of the documented semantics.  With ``bf16_operands`` every GEMM
+operand is rounded to bf16 (nearest even) first, as the matrix-core kernel
+does (as hi + lo parts for the operands the network splits), so the GPU
+network can be compared with it closely.
+"""
+
+import numpy as np
+import torch
+
+from oracle import fno_oracle
+from wavefunctionsimpython_amd.fno import FNO3d
+
+
+def round_bf16(t: torch.Tensor) -> torch.Tensor:
+    """float32 -> bf16 (round to nearest even) -> float64, elementwise."""
+    a = np.ascontiguousarray(t.detach().cpu().numpy().astype(np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return torch.from_numpy(u.astype(np.uint32).view(np.float32).astype(np.float64)).to(t.device)
+
+
+class EmulatedFNO(FNO3d):
+    """FNO3d with the device kernels replaced by their float64 semantics."""
+
+    dtype = torch.float64
+    bf16_operands = False
+
+    def _gemm(self, A, B, C, batch, M, N, K, sA, sB, sC, alpha=1.0, beta=0.0, splits=1, flags=0):
+        for t, st, dims in ((A, sA, (batch, M, K)), (B, sB, (batch, K, N)), (C, sC, (batch, M, N))):
+            span = sum((d - 1) * s for d, s in zip(dims, st))
+            assert t.is_contiguous() and min(st) >= 0 and span < t.numel()
+        Av = torch.as_strided(A, (batch, M, K), sA)
+        Bv = torch.as_strided(B, (batch, K, N), sB)
+        Cv = torch.as_strided(C, (batch, M, N), sC)
+        if self.bf16_operands:
+            if not self.split_activations:
+                flags = 0
+            Ah, Bh = round_bf16(Av), round_bf16(Bv)
+            prod = torch.bmm(Ah, Bh)
+            if flags & 1:   # A as hi + lo
+                prod = prod + torch.bmm(round_bf16(Av.float().double() - Ah), Bh)
+            if flags & 2:
+                prod = prod + torch.bmm(Ah, round_bf16(Bv.float().double() - Bh))
+            r = alpha * prod
+        else:
+            r = alpha * torch.bmm(Av, Bv)
+        if beta != 0.0:
+            r = r + beta * Cv
+        Cv.copy_(r)
+
+    def _bias_act(self, u, bias, v, rows, act):
+        uu = u.view(rows, self.P)
+        uu += bias[:, None]
+        if v is not None:
+            v.view(rows, self.P).copy_(fno_oracle.gelu_tanh(uu) if act else uu)
+
+    def _gelu_backward(self, u, dv, du):
+        ut = u.detach().clone().requires_grad_(True)
+        (g,) = torch.autograd.grad(fno_oracle.gelu_tanh(ut), ut, dv)
+        du.copy_(g)
+
+    def _axpby(self, y, x, alpha, beta):
+        y.copy_(alpha * x + beta * y if beta != 0.0 else alpha * x)

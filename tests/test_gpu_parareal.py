@@ -1,0 +1,118 @@
+"""Parareal driver, FP64 correction and the trained coarse propagator on the GPU."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from wavefunctionsimpython_amd import parareal as pr
+from wavefunctionsimpython_amd.fno import FNO3d
+
+pytestmark = pytest.mark.gpu
+
+N_GRID, DX, STEPS = 32, 0.5, 50
+
+
+def _problem(seed=2):
+    return pr.random_lattice_problem(np.random.default_rng(seed), n=N_GRID, dx=DX, k0=0.5, r0=2.0)
+
+
+def _rel(a, b):
+    return float((a - b).abs().pow(2).sum().sqrt() / b.abs().pow(2).sum().sqrt())
+
+
+def _setup(slices, steps=STEPS, seed=2):
+    prob = _problem(seed)
+    V = prob.potential()
+    psi0 = prob.packet()
+    par = pr.Parareal(prob.shape, (DX,) * 3, prob.dt, 1.0, V, steps, slices)
+    return prob, V, psi0, par
+
+
+def test_parareal_identity_first_k_slices_exact(dev):
+    """After k iterations slices 0..k equal the serial fine states bit for bit."""
+    prob, V, psi0, par = _setup(6)
+    _, serial = par.serial_fine(psi0, keep=True)
+    coarse = pr.SplitStepCoarse(prob.shape, (DX,) * 3, STEPS * prob.dt, 1.0, V, 1)
+    seen = {}
+
+    def record(k, U):
+        seen[k] = U.to(torch.complex64).clone()
+
+    par.run(psi0, coarse, max_iterations=3, tol=0.0)
+    par.run(psi0, coarse, max_iterations=3, tol=0.0, record=record)
+    assert sorted(seen) == [1, 2, 3]
+    for k, U in seen.items():
+        for n in range(k + 1):
+            assert torch.equal(U[n], serial[n]), f"iteration {k} slice {n}"
+        # a slice beyond k is not yet the fine state (the test is not vacuous)
+        assert not torch.equal(U[k + 1], serial[k + 1])
+
+
+def test_parareal_converges_to_fine_and_fp64_oracle(dev):
+    prob, V, psi0, par = _setup(8)
+    ref = par.serial_fine(psi0)
+    coarse = pr.SplitStepCoarse(prob.shape, (DX,) * 3, STEPS * prob.dt, 1.0, V, 2)
+    res = par.run(psi0, coarse, tol=3e-5)
+    assert res.converged and not res.fallback and res.iterations < 8
+    assert _rel(res.state, ref) < 1e-4
+    # FP64 oracle over the whole interval (8 slices x 50 steps)
+    psi_np = psi0.cpu().numpy().astype(np.complex128)
+    v_np = V.cpu().numpy()
+    exact = oracle.split_step(psi_np, v_np, (DX,) * 3, prob.dt, 1.0, nsteps=8 * STEPS)
+    err = oracle.relative_l2(res.state.cpu().numpy(), exact)
+    print(f"parareal iterations {res.iterations}, vs serial {_rel(res.state, ref):.2e}, vs FP64 oracle {err:.2e}")
+    assert err < 1e-4
+
+
+def test_parareal_fallback_gives_serial_fine_exactly(dev):
+    prob, V, psi0, par = _setup(6)
+    ref = par.serial_fine(psi0)
+    res = par.run(psi0, pr.IdentityCoarse(), fallback_threshold=0.5)
+    assert res.fallback and not res.converged and res.iterations == 1
+    assert torch.equal(res.state, ref)
+    # iteration limit: finished serially from the last exact slice
+    res = par.run(psi0, pr.IdentityCoarse(), max_iterations=2, fallback_threshold=1e9)
+    assert res.fallback and res.iterations == 2
+    assert torch.equal(res.state, ref)
+
+
+def test_trained_fno_gains_over_coarse_without_it(dev):
+    """Train the FNO correction of a one-step split-step on solver pairs.  On a held-out
+    problem it must lower its base's slice error and first-iteration parareal error, and
+    parareal with it must reach 1e-4 in fewer iterations than with the identity.  (Its
+    iteration count against the base alone is printed, not asserted: the convergence rate
+    follows the mismatch of the Jacobians, which the correction does not reduce.)"""
+    S, N = 200, 8
+    rng = np.random.default_rng(1)
+    probs = [pr.random_lattice_problem(rng, n=N_GRID, dx=DX, k0=0.5, r0=2.0) for _ in range(16)]
+    pairs = pr.TrajectoryPairs(probs, S, base_steps=1)
+    pscale, vscale = math.sqrt((N_GRID * DX) ** 3), 0.2
+    net = FNO3d(N_GRID, width=16, modes=16, layers=4, in_channels=5, output_scale=0.0, residual_scale=1e-2)
+    losses = pr.train_coarse(net, pairs, 8, pscale, vscale, lr=1e-3, lr_final=1e-4, epochs=24, mix=0.5)
+    k = len(losses) // 10
+    first, last = float(np.mean(losses[:k])), float(np.mean(losses[-k:]))
+
+    prob, V, psi0, par = _setup(N, steps=S)
+    _, serial = par.serial_fine(psi0, keep=True)
+    base = pr.SplitStepCoarse(prob.shape, (DX,) * 3, S * prob.dt, 1.0, V, 1)
+    fno = pr.FNOCoarse(net, V, pscale, vscale, base=base)
+    out = torch.empty_like(psi0)
+    e_base, e_fno = [], []
+    for j in range(N):
+        base(serial[j], out)
+        e_base.append(_rel(out, serial[j + 1]))
+        fno(serial[j], out)
+        e_fno.append(_rel(out, serial[j + 1]))
+    runs = {name: par.run(psi0, c, tol=1e-4, fallback_threshold=1e9)
+            for name, c in (("identity", pr.IdentityCoarse()), ("base", base), ("fno", fno))}
+    print(f"train loss {first:.3f} -> {last:.3f}; slice error base {np.mean(e_base):.2e} fno {np.mean(e_fno):.2e}; "
+          + "; ".join(f"{k}: {r.iterations} it {' '.join(f'{h:.1e}' for h in r.history)}" for k, r in runs.items()))
+    assert last < 0.7 * first
+    assert np.mean(e_fno) < 0.8 * np.mean(e_base)
+    assert runs["fno"].history[0] < runs["base"].history[0]
+    assert runs["fno"].converged and runs["fno"].iterations < runs["identity"].iterations
+    assert runs["fno"].iterations <= runs["base"].iterations + 1
+    assert _rel(runs["fno"].state, serial[-1]) < 1e-4

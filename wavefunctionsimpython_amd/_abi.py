@@ -43,6 +43,19 @@ SIGNATURES = {
     "wfs_plan_create_batched": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "wfs_plan_destroy": (ctypes.c_int, [_vp]),
+    "wfs_gemm_bf16": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long * 4, ctypes.c_long * 9, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_long, _vp]),
+    "wfs_fno_bias_act": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_int, _vp]),
+    "wfs_fno_gelu_backward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, _vp]),
+    "wfs_fno_axpby": (ctypes.c_int, [_vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_long, _vp]),
+    "wfs_fno_pack": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_float, ctypes.c_float, _vp, ctypes.c_long,
+                                    _vp]),
+    "wfs_fno_unpack": (ctypes.c_int, [_vp, ctypes.c_float, _vp, ctypes.c_long, _vp]),
+    "wfs_fno_mse": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_long, _vp, ctypes.c_long, _vp, _vp]),
+    "wfs_fno_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_long, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, ctypes.c_int, _vp]),
+    "wfs_parareal_update": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_long, _vp, ctypes.c_long, _vp, _vp]),
+    "wfs_narrow_c128": (ctypes.c_int, [_vp, _vp, ctypes.c_long, _vp]),
     "wfs_graph_begin": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "wfs_graph_end": (ctypes.c_int, [_vp, _vp]),
     "wfs_graph_launch": (ctypes.c_int, [_vp, _vp]),
@@ -304,6 +317,49 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+GEMM_SPLIT_A, GEMM_SPLIT_B = 1, 2
+
+
+def gemm_bf16(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, batch: int, M: int, N: int, K: int,
+              sA, sB, sC, alpha: float = 1.0, beta: float = 0.0, splits: int = 1,
+              work: torch.Tensor | None = None, stream=None, flags: int = 0) -> None:
+    """C[b](m,n) = alpha sum_k A[b](m,k) B[b](k,n) + beta C[b](m,n) on the bf16 matrix cores.
+
+    sA = (batch, m, k), sB = (batch, k, n), sC = (batch, m, n) element strides of float32
+    device tensors.  Every addressed element is checked to lie inside its tensor.
+    flags: GEMM_SPLIT_A / GEMM_SPLIT_B carry that operand as bf16 hi + lo parts.
+    """
+    for t, name in ((A, "A"), (B, "B"), (C, "C")):
+        _require_device_any(t, name)
+        if t.dtype != torch.float32:
+            raise TypeError(f"{name} must be float32")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous (strides address its flat storage)")
+
+    def span(strides, d0, d1, d2):
+        return (d0 - 1) * strides[0] + (d1 - 1) * strides[1] + (d2 - 1) * strides[2]
+
+    if min(batch, M, N, K) < 1:
+        raise ValueError("GEMM sizes must be >= 1")
+    for t, st, dims, name in ((A, sA, (batch, M, K), "A"), (B, sB, (batch, K, N), "B"), (C, sC, (batch, M, N), "C")):
+        if min(st) < 0 or span(st, *dims) >= t.numel():
+            raise ValueError(f"GEMM operand {name} strides {st} reach outside its {t.numel()} elements")
+    if splits > 1:
+        need = batch * splits * M * N
+        if work is None or work.numel() < need or work.dtype != torch.float32:
+            raise ValueError(f"split-K needs a float32 workspace of {need} elements")
+    dims = (ctypes.c_long * 4)(batch, M, N, K)
+    st = (ctypes.c_long * 9)(*(list(sA) + list(sB) + list(sC)))
+    check(lib().wfs_gemm_bf16(ptr(A), ptr(B), ptr(C), dims, st, float(alpha), float(beta), int(flags), int(splits),
+                              ptr(work) if work is not None else None, int(work.numel()) if work is not None else 0,
+                              stream_handle(stream)), "wfs_gemm_bf16")
+
+
+def _require_device_any(t: torch.Tensor, name: str) -> None:
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise ValueError(f"{name} must be a device tensor")
 
 
 class Graph:

@@ -1,0 +1,223 @@
+// Strided batched GEMM on the bf16 matrix cores of gfx950, FP32 accumulation.
+//
+//   C[b](m, n) = alpha * sum_k A[b](m, k) B[b](k, n) + beta * C[b](m, n)
+//
+// with every operand addressed by explicit element strides, so one kernel
+// serves all contractions of the Fourier neural operator (truncated DFTs along
+// each axis, channel mixing, spectral weights) and their backward passes
+// (which are the same contractions with swapped strides).  A and B are FP32 in
+// memory and rounded to bf16 (nearest even) as they are staged into LDS; the
+// bf16 inputs are what the matrix cores multiply.
+//
+// Split operands (flags bit 0: A, bit 1: B): the operand is staged as two bf16
+// parts, hi = bf16(x) and lo = bf16(x - hi), and the tile product adds
+// Ahi Bhi + Alo Bhi + Ahi Blo (lo lo is dropped), so that operand carries about
+// 16 significant bits.  The Fourier neural operator splits its activations and
+// keeps weights and transform tables in plain bf16: rounding a fixed operand
+// perturbs the map by a fixed amount, while rounding activations adds noise
+// that does not shrink with the change of the input.
+//
+// Block tile (32*WM) x (32*WN) x 32, one wavefront per 32 x 32 sub-tile, using
+// v_mfma_f32_32x32x8_bf16: lane l holds A[l%32][4(l/32)..+3] and
+// B[4(l/32)..+3][l%32]; accumulator item v of lane l is C[8(v/4)+4(l/32)+v%4][l%32].
+//
+// Split-K: gridDim.z = batch * splits; with splits > 1 each block writes its
+// partial tile to a workspace and a second kernel adds the splits in a fixed
+// order (deterministic), then applies alpha/beta.
+#include "wfs_internal.h"
+
+namespace wfs {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+struct GemmArgs {
+  const float* A;
+  const float* B;
+  float* C;
+  float* work;         // split-K partials [batch][splits][M][N]
+  long M, N, K;
+  long sAb, sAm, sAk;
+  long sBb, sBm_unused, sBk, sBn;
+  long sCb, sCm, sCn;
+  float alpha, beta;
+  int batch, splits;
+  int flags;           // bit 0: split A into hi + lo; bit 1: split B
+  long kchunk;         // K range per split (multiple of 32)
+};
+
+constexpr int kBK = 32;
+
+template <int WM, int WN, bool SA, bool SB>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int NT = 64 * WM * WN;
+  // k-contiguous bf16 tiles, padded by 4 elements against bank conflicts
+  __shared__ unsigned short As[BM][kBK + 4];
+  __shared__ unsigned short Bs[BN][kBK + 4];
+  __shared__ unsigned short Al[SA ? BM : 1][kBK + 4];
+  __shared__ unsigned short Bl[SB ? BN : 1][kBK + 4];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int b = blockIdx.z / g.splits;
+  const int split = blockIdx.z % g.splits;
+  const long m0 = long(blockIdx.x) * BM;
+  const long n0 = long(blockIdx.y) * BN;
+  const long kbeg = long(split) * g.kchunk;
+  long kend = kbeg + g.kchunk;
+  if (kend > g.K) kend = g.K;
+
+  const float* __restrict__ A = g.A + long(b) * g.sAb;
+  const float* __restrict__ B = g.B + long(b) * g.sBb;
+  const bool a_k_fast = (g.sAk == 1);
+  const bool b_n_fast = (g.sBn == 1);
+
+  f32x16 acc;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+
+  for (long k0 = kbeg; k0 < kend; k0 += kBK) {
+    // stage A (BM x 32) and B (32 x BN) as bf16
+    for (int e = tid; e < BM * kBK; e += NT) {
+      int mm, kk;
+      if (a_k_fast) { mm = e / kBK; kk = e % kBK; } else { kk = e / BM; mm = e % BM; }
+      const long m = m0 + mm, k = k0 + kk;
+      const float v = (m < g.M && k < kend) ? A[m * g.sAm + k * g.sAk] : 0.0f;
+      const unsigned short h = f32_to_bf16(v);
+      As[mm][kk] = h;
+      if constexpr (SA) Al[mm][kk] = f32_to_bf16(v - __uint_as_float(unsigned(h) << 16));
+    }
+    for (int e = tid; e < BN * kBK; e += NT) {
+      int nn, kk;
+      if (b_n_fast) { kk = e / BN; nn = e % BN; } else { nn = e / kBK; kk = e % kBK; }
+      const long n = n0 + nn, k = k0 + kk;
+      const float v = (n < g.N && k < kend) ? B[k * g.sBk + n * g.sBn] : 0.0f;
+      const unsigned short h = f32_to_bf16(v);
+      Bs[nn][kk] = h;
+      if constexpr (SB) Bl[nn][kk] = f32_to_bf16(v - __uint_as_float(unsigned(h) << 16));
+    }
+    __syncthreads();
+    const int row = wm * 32 + (lane & 31);
+    const int col = wn * 32 + (lane & 31);
+    const int kq = 4 * (lane >> 5);
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 8) {
+      bf16x4 a, bb;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[j] = (short)As[row][kk + kq + j];
+        bb[j] = (short)Bs[col][kk + kq + j];
+      }
+      if constexpr (SA) {
+        bf16x4 al;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) al[j] = (short)Al[row][kk + kq + j];
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bb, acc, 0, 0, 0);
+      }
+      if constexpr (SB) {
+        bf16x4 bl;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bl[j] = (short)Bl[col][kk + kq + j];
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, acc, 0, 0, 0);
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bb, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  const long n = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const long m = m0 + wm * 32 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+    if (m < g.M && n < g.N) {
+      if (g.splits == 1) {
+        float* c = g.C + long(b) * g.sCb + m * g.sCm + n * g.sCn;
+        *c = (g.beta != 0.0f) ? g.alpha * acc[v] + g.beta * (*c) : g.alpha * acc[v];
+      } else {
+        g.work[((long(b) * g.splits + split) * g.M + m) * g.N + n] = acc[v];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void gemm_splitk_reduce_kernel(GemmArgs g) {
+  const long total = long(g.batch) * g.M * g.N;
+  for (long i = long(blockIdx.x) * 256 + threadIdx.x; i < total; i += long(gridDim.x) * 256) {
+    const long b = i / (g.M * g.N);
+    const long r = i % (g.M * g.N);
+    const long m = r / g.N, n = r % g.N;
+    float s = 0.0f;
+    for (int sp = 0; sp < g.splits; ++sp) s += g.work[((b * g.splits + sp) * g.M + m) * g.N + n];
+    float* c = g.C + b * g.sCb + m * g.sCm + n * g.sCn;
+    *c = (g.beta != 0.0f) ? g.alpha * s + g.beta * (*c) : g.alpha * s;
+  }
+}
+
+}  // namespace wfs
+
+using namespace wfs;
+
+template <int WM, int WN>
+static void launch_tile(dim3 grid, hipStream_t s, const GemmArgs& g) {
+  switch (g.flags & 3) {
+    case 0: hipLaunchKernelGGL((gemm_bf16_kernel<WM, WN, false, false>), grid, dim3(64 * WM * WN), 0, s, g); break;
+    case 1: hipLaunchKernelGGL((gemm_bf16_kernel<WM, WN, true, false>), grid, dim3(64 * WM * WN), 0, s, g); break;
+    case 2: hipLaunchKernelGGL((gemm_bf16_kernel<WM, WN, false, true>), grid, dim3(64 * WM * WN), 0, s, g); break;
+    default: hipLaunchKernelGGL((gemm_bf16_kernel<WM, WN, true, true>), grid, dim3(64 * WM * WN), 0, s, g); break;
+  }
+}
+
+// dims = {batch, M, N, K}; strides = {sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn} (elements)
+extern "C" int wfs_gemm_bf16(const float* A, const float* B, float* C, const long dims[4], const long strides[9],
+                             float alpha, float beta, int flags, int splits, float* work, long work_elems,
+                             void* stream) {
+  if (!A || !B || !C || !dims || !strides) { set_last_error("wfs_gemm_bf16: null argument"); return WFS_ERR_INVALID; }
+  GemmArgs g{};
+  g.A = A; g.B = B; g.C = C; g.work = work;
+  g.batch = int(dims[0]); g.M = dims[1]; g.N = dims[2]; g.K = dims[3];
+  g.sAb = strides[0]; g.sAm = strides[1]; g.sAk = strides[2];
+  g.sBb = strides[3]; g.sBk = strides[4]; g.sBn = strides[5];
+  g.sCb = strides[6]; g.sCm = strides[7]; g.sCn = strides[8];
+  g.alpha = alpha; g.beta = beta; g.flags = flags;
+  if (flags & ~3) { set_last_error("wfs_gemm_bf16: unknown flags"); return WFS_ERR_INVALID; }
+  if (g.batch < 1 || g.M < 1 || g.N < 1 || g.K < 1 || g.batch > 65535) { set_last_error("wfs_gemm_bf16: bad sizes"); return WFS_ERR_INVALID; }
+  if (splits < 1) splits = 1;
+  const long kblocks = (g.K + kBK - 1) / kBK;
+  if (splits > kblocks) splits = int(kblocks);
+  g.kchunk = ((kblocks + splits - 1) / splits) * kBK;
+  splits = int((g.K + g.kchunk - 1) / g.kchunk);
+  g.splits = splits;
+  if (splits > 1 && (!work || work_elems < long(g.batch) * splits * g.M * g.N)) {
+    set_last_error("wfs_gemm_bf16: split-K needs a workspace of batch*splits*M*N floats");
+    return WFS_ERR_INVALID;
+  }
+  if (long(g.batch) * splits > 65535) { set_last_error("wfs_gemm_bf16: batch*splits too large"); return WFS_ERR_INVALID; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // tall-skinny outputs use 128 x 32 tiles, the rest 64 x 64
+  if (g.N <= 32) {
+    dim3 grid(unsigned((g.M + 127) / 128), unsigned((g.N + 31) / 32), unsigned(g.batch * splits));
+    launch_tile<4, 1>(grid, s, g);
+  } else {
+    dim3 grid(unsigned((g.M + 63) / 64), unsigned((g.N + 63) / 64), unsigned(g.batch * splits));
+    launch_tile<2, 2>(grid, s, g);
+  }
+  WFS_HIP_CHECK(hipGetLastError());
+  if (splits > 1) {
+    const long total = long(g.batch) * g.M * g.N;
+    long blocks = (total + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gemm_splitk_reduce_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, g);
+    WFS_HIP_CHECK(hipGetLastError());
+  }
+  return 0;
+}
