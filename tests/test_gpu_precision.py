@@ -1,0 +1,68 @@
+"""Long-horizon accuracy of the complex64 propagator against the complex128 one.
+
+The complex128 path is pinned to the CPU oracle at 1e-12 (test_gpu_kernels), so
+it serves as the FP64 reference at sizes and horizons the CPU cannot reach.  A
+float-rounded factor applied to the same mode every step errs the same way every
+step, so the complex64 error grows linearly with the step count.  With the
+in-register DFT twiddles and the kinetic factor carried as hi + lo pairs the
+growth is about 3e-8 per step at 128^3 (it was 1e-7 with plain floats): within
+1e-5 up to about 250 steps and within 1e-4 over 1600 steps.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from wavefunctionsimpython_amd import _abi
+from wavefunctionsimpython_amd import parareal as pr
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = a.to(torch.complex128)
+    return float((a - b).abs().pow(2).sum().sqrt() / b.abs().pow(2).sum().sqrt())
+
+
+def _run(n, dx, dt, V, psi0, steps):
+    p32 = _abi.Plan(n, dx, dt, 1.0, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER)
+    p64 = _abi.Plan(n, dx, dt, 1.0, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+    a = psi0.clone()
+    b = psi0.to(torch.complex128)
+    p32.step(a, V, steps)
+    p64.step(b, V, steps)
+    return a, b
+
+
+@pytest.mark.parametrize("potential", ["lattice", "free"])
+def test_complex64_1600_steps_at_128(dev, potential):
+    prob = pr.random_lattice_problem(np.random.default_rng(2), n=128, dx=0.25, k0=0.5, r0=3.0)
+    V = prob.potential() if potential == "lattice" else torch.zeros(prob.shape, dtype=torch.complex64, device=dev)
+    psi0 = prob.packet()
+    a, b = _run(prob.shape, (0.25,) * 3, prob.dt, V, psi0, 1600)
+    err = _rel(a, b)
+    drift = float(a.to(torch.complex128).abs().pow(2).sum() / b.abs().pow(2).sum()) - 1.0
+    print(f"{potential}: 1600 steps complex64 vs complex128: rel L2 {err:.2e}, norm drift {drift:+.1e}")
+    assert err < 1e-4
+    assert abs(drift) < 2e-4
+
+
+def test_complex64_radix3_grid_1000_steps(dev):
+    n = (96, 96, 96)
+    dx = (0.4,) * 3
+    psi0 = torch.empty(n, dtype=torch.complex64, device=dev)
+    _abi.init_gaussian(n, dx, _abi.WFS_C64, (2.0, 2.0, 2.0), (1.0, -0.5, 0.0), (0.5, 0.0, -0.3), psi0)
+    V = torch.zeros(n, dtype=torch.complex64, device=dev)
+    _abi.add_harmonic(n, dx, 0.2, 1.0, (0.0, 0.0, 0.0), V)
+    a, b = _run(n, dx, 0.005, V, psi0, 1000)
+    err = _rel(a, b)
+    print(f"96^3 harmonic, 1000 steps complex64 vs complex128: rel L2 {err:.2e}")
+    assert err < 1e-4
+
+
+def test_complex64_within_1e5_for_250_steps_at_128(dev):
+    prob = pr.random_lattice_problem(np.random.default_rng(2), n=128, dx=0.25, k0=0.5, r0=3.0)
+    a, b = _run(prob.shape, (0.25,) * 3, prob.dt, prob.potential(), prob.packet(), 250)
+    err = _rel(a, b)
+    print(f"250 steps complex64 vs complex128: rel L2 {err:.2e}")
+    assert err < 1e-5

@@ -42,6 +42,31 @@ __host__ __device__ __forceinline__ T cmulc(T a, T b) {
 template <typename T, typename S>
 __host__ __device__ __forceinline__ T cscale(T a, S s) { a.x *= s; a.y *= s; return a; }
 
+// a * (hi + lo) and a * conj(hi + lo) for a factor held as an unevaluated sum of two
+// floats (hi = float(w), lo = float(w - hi)).  A float-rounded factor applied every
+// step to the same mode errs the same way every step, so its error grows linearly
+// with the step count; with the lo part the factor is exact to about 2^-48 and only
+// the (data-dependent) rounding of the products remains.
+// The small products a*lo are formed first and enter the fused multiply-adds of a*hi,
+// so they are added before the main products are rounded (added afterwards they would
+// mostly vanish below half an ulp of the rounded product).
+__host__ __device__ __forceinline__ float2 cmul2(float2 a, float2 hi, float2 lo) {
+  const float cx = a.x * lo.x - a.y * lo.y;
+  const float cy = a.x * lo.y + a.y * lo.x;
+  float2 r;
+  r.x = __builtin_fmaf(a.x, hi.x, __builtin_fmaf(-a.y, hi.y, cx));
+  r.y = __builtin_fmaf(a.x, hi.y, __builtin_fmaf(a.y, hi.x, cy));
+  return r;
+}
+__host__ __device__ __forceinline__ float2 cmulc2(float2 a, float2 hi, float2 lo) {
+  const float cx = a.x * lo.x + a.y * lo.y;
+  const float cy = a.y * lo.x - a.x * lo.y;
+  float2 r;
+  r.x = __builtin_fmaf(a.x, hi.x, __builtin_fmaf(a.y, hi.y, cx));
+  r.y = __builtin_fmaf(a.y, hi.x, __builtin_fmaf(-a.x, hi.y, cy));
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Compile-time cos/sin of 2*pi*e/M for the in-register DFT twiddles.
 // Exact values are returned for multiples of pi/4; otherwise the argument is
@@ -108,7 +133,15 @@ __device__ __forceinline__ T twiddle_ct(T a) {
     T w;
     w.x = R(cs.c);
     w.y = R(SIGN > 0 ? cs.s : -cs.s);
-    return cmul(a, w);
+    if constexpr (sizeof(R) == 4) {
+      // float state: the twiddle as hi + lo (see cmul2)
+      T lo;
+      lo.x = R(cs.c - double(R(cs.c)));
+      lo.y = R((SIGN > 0 ? cs.s : -cs.s) - double(R(SIGN > 0 ? cs.s : -cs.s)));
+      return cmul2(a, w, lo);
+    } else {
+      return cmul(a, w);
+    }
   }
 }
 
@@ -155,9 +188,19 @@ struct DFT<3, SIGN> {
     T m;  // a0 + c*(a1+a2)
     m.x = a[0].x + c * t.x;
     m.y = a[0].y + c * t.y;
-    // SIGN*i*s*(a1 - a2)
+    // SIGN*i*s*(a1 - a2); for float, s as hi + lo with the low product inside the fma
+    R sdx, sdy;
+    if constexpr (sizeof(R) == 4) {
+      constexpr double sd = 0.866025403784438646763723170752936183;
+      const R slo = R(sd - double(R(sd)));
+      sdx = __builtin_fmaf(s, d.x, slo * d.x);
+      sdy = __builtin_fmaf(s, d.y, slo * d.y);
+    } else {
+      sdx = s * d.x;
+      sdy = s * d.y;
+    }
     T q;
-    if constexpr (SIGN > 0) { q.x = -s * d.y; q.y = s * d.x; } else { q.x = s * d.y; q.y = -s * d.x; }
+    if constexpr (SIGN > 0) { q.x = -sdy; q.y = sdx; } else { q.x = sdy; q.y = -sdx; }
     a[0] = cadd(a[0], t);
     a[1] = cadd(m, q);
     a[2] = csub(m, q);

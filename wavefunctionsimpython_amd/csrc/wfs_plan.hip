@@ -15,8 +15,8 @@ struct wfs_plan {
   int precision, method, order;
   size_t elem;            // bytes per state element
   void* tw[3];            // forward twiddles per axis, state precision
-  void* kfull[3];         // exp(-i k_d^2 dt / 2m) / N_d
-  void* khalf[3];         // exp(-i k_d^2 dt / 4m) / N_d
+  void* kfull[3];         // exp(-i k_d^2 dt / 2m) / N_d, complex128 for every plan
+  void* khalf[3];         // exp(-i k_d^2 dt / 4m) / N_d, complex128
   void* scratch[2];       // power-series ping-pong buffers (lazy)
   double* obs_work;       // observable partial records
   long obs_work_len;
@@ -317,10 +317,10 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
     rc = upload_table(&p->tw[a], re, im, precision);
     if (rc) break;
     kinetic_table(n[a], dx[a], dt, mass, 2.0, re, im);
-    rc = upload_table(&p->kfull[a], re, im, precision);
+    rc = upload_table(&p->kfull[a], re, im, WFS_C128);
     if (rc) break;
     kinetic_table(n[a], dx[a], dt, mass, 4.0, re, im);
-    rc = upload_table(&p->khalf[a], re, im, precision);
+    rc = upload_table(&p->khalf[a], re, im, WFS_C128);
   }
   if (rc == 0) {
     wfs_box whole{{0, 0, 0}, {n[0], n[1], n[2]}};
