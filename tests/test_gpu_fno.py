@@ -27,6 +27,8 @@ GEMM_CASES = {
     "split_A": (1, 100, 70, 45, "mk", "kn", "mn", 1, 1.0, 0.0, 1),
     "split_B_tall": (3, 200, 20, 70, "km", "kn", "mn", 1, 1.0, 0.0, 2),
     "split_both_splitk": (1, 33, 40, 5000, "mk", "nk", "mn", 16, 0.5, 2.0, 3),
+    "vector_loads": (2, 200, 96, 256, "mk", "kn", "mn", 1, 1.0, 0.0, 3),
+    "vector_loads_splitk": (1, 64, 128, 4096, "mk", "kn", "mn", 8, 1.0, 1.0, 0),
 }
 
 
@@ -74,6 +76,25 @@ def test_gemm_bf16_matches_numpy(dev, name):
         assert err_exact < 1e-4
     elif flags:
         assert err_exact < 5e-3
+
+
+def test_gemm_vector_paths_with_partial_quads(dev):
+    """Row strides that are multiples of 4 with K and N that are not: the 16-byte load
+    path must fall back to single elements in the last, partial quad."""
+    rng = np.random.default_rng(7)
+    M, N, K, lda, ldb = 70, 101, 70, 72, 104
+    A = rng.standard_normal(M * lda).astype(np.float32)
+    B = rng.standard_normal(K * ldb).astype(np.float32)
+    C = np.zeros(M * N, np.float32)
+    At, Bt, Ct = (torch.from_numpy(x).to(dev) for x in (A, B, C))
+    _abi.gemm_bf16(At, Bt, Ct, 1, M, N, K, (0, lda, 1), (0, ldb, 1), (0, N, 1))
+    torch.cuda.synchronize()
+    Ab = _np_bf16(A).reshape(M, lda)[:, :K]
+    Bb = _np_bf16(B).reshape(K, ldb)[:, :N]
+    ref = Ab @ Bb
+    scale = np.abs(Ab) @ np.abs(Bb)
+    err = (np.abs(Ct.cpu().numpy().reshape(M, N) - ref) / scale).max()
+    assert err < 4e-6 * np.sqrt(K)
 
 
 def test_gemm_mode_mixing_pattern(dev):

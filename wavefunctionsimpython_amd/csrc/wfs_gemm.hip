@@ -81,6 +81,12 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
   const float* __restrict__ B = g.B + long(b) * g.sBb;
   const bool a_k_fast = (g.sAk == 1);
   const bool b_n_fast = (g.sBn == 1);
+  // 16-byte loads of four consecutive elements along the contiguous axis when every
+  // quad start is 16-byte aligned (k0 is a multiple of 32)
+  const bool a_vec = a_k_fast && (g.sAm % 4 == 0) && (g.sAb % 4 == 0) &&
+                     (reinterpret_cast<unsigned long>(g.A) % 16 == 0);
+  const bool b_vec = b_n_fast && (g.sBk % 4 == 0) && (g.sBb % 4 == 0) &&
+                     (reinterpret_cast<unsigned long>(g.B) % 16 == 0);
 
   f32x16 acc;
 #pragma unroll
@@ -88,23 +94,65 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
 
   for (long k0 = kbeg; k0 < kend; k0 += kBK) {
     // stage A (BM x 32) and B (32 x BN) as bf16
-    for (int e = tid; e < BM * kBK; e += NT) {
-      int mm, kk;
-      if (a_k_fast) { mm = e / kBK; kk = e % kBK; } else { kk = e / BM; mm = e % BM; }
-      const long m = m0 + mm, k = k0 + kk;
-      const float v = (m < g.M && k < kend) ? A[m * g.sAm + k * g.sAk] : 0.0f;
-      const unsigned short h = f32_to_bf16(v);
-      As[mm][kk] = h;
-      if constexpr (SA) Al[mm][kk] = f32_to_bf16(v - __uint_as_float(unsigned(h) << 16));
+    if (a_vec) {
+      for (int e = tid; e < BM * (kBK / 4); e += NT) {
+        const int mm = e / (kBK / 4), kk = (e % (kBK / 4)) * 4;
+        const long m = m0 + mm, k = k0 + kk;
+        float v[4];
+        if (m < g.M && k + 3 < kend) {
+          const float4 q = *reinterpret_cast<const float4*>(A + m * g.sAm + k);
+          v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[j] = (m < g.M && k + j < kend) ? A[m * g.sAm + k + j] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned short h = f32_to_bf16(v[j]);
+          As[mm][kk + j] = h;
+          if constexpr (SA) Al[mm][kk + j] = f32_to_bf16(v[j] - __uint_as_float(unsigned(h) << 16));
+        }
+      }
+    } else {
+      for (int e = tid; e < BM * kBK; e += NT) {
+        int mm, kk;
+        if (a_k_fast) { mm = e / kBK; kk = e % kBK; } else { kk = e / BM; mm = e % BM; }
+        const long m = m0 + mm, k = k0 + kk;
+        const float v = (m < g.M && k < kend) ? A[m * g.sAm + k * g.sAk] : 0.0f;
+        const unsigned short h = f32_to_bf16(v);
+        As[mm][kk] = h;
+        if constexpr (SA) Al[mm][kk] = f32_to_bf16(v - __uint_as_float(unsigned(h) << 16));
+      }
     }
-    for (int e = tid; e < BN * kBK; e += NT) {
-      int nn, kk;
-      if (b_n_fast) { kk = e / BN; nn = e % BN; } else { nn = e / kBK; kk = e % kBK; }
-      const long n = n0 + nn, k = k0 + kk;
-      const float v = (n < g.N && k < kend) ? B[k * g.sBk + n * g.sBn] : 0.0f;
-      const unsigned short h = f32_to_bf16(v);
-      Bs[nn][kk] = h;
-      if constexpr (SB) Bl[nn][kk] = f32_to_bf16(v - __uint_as_float(unsigned(h) << 16));
+    if (b_vec) {
+      for (int e = tid; e < kBK * (BN / 4); e += NT) {
+        const int kk = e / (BN / 4), nn = (e % (BN / 4)) * 4;
+        const long n = n0 + nn, k = k0 + kk;
+        float v[4];
+        if (k < kend && n + 3 < g.N) {
+          const float4 q = *reinterpret_cast<const float4*>(B + k * g.sBk + n);
+          v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[j] = (k < kend && n + j < g.N) ? B[k * g.sBk + n + j] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned short h = f32_to_bf16(v[j]);
+          Bs[nn + j][kk] = h;
+          if constexpr (SB) Bl[nn + j][kk] = f32_to_bf16(v[j] - __uint_as_float(unsigned(h) << 16));
+        }
+      }
+    } else {
+      for (int e = tid; e < BN * kBK; e += NT) {
+        int nn, kk;
+        if (b_n_fast) { kk = e / BN; nn = e % BN; } else { nn = e / kBK; kk = e % kBK; }
+        const long n = n0 + nn, k = k0 + kk;
+        const float v = (n < g.N && k < kend) ? B[k * g.sBk + n * g.sBn] : 0.0f;
+        const unsigned short h = f32_to_bf16(v);
+        Bs[nn][kk] = h;
+        if constexpr (SB) Bl[nn][kk] = f32_to_bf16(v - __uint_as_float(unsigned(h) << 16));
+      }
     }
     __syncthreads();
     const int row = wm * 32 + (lane & 31);
@@ -150,14 +198,21 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
   }
 }
 
+// one wavefront per output element: lane l adds splits l, l + 64, ... in order, then a
+// fixed xor tree across the lanes (deterministic)
 __global__ __launch_bounds__(256) void gemm_splitk_reduce_kernel(GemmArgs g) {
-  const long total = long(g.batch) * g.M * g.N;
-  for (long i = long(blockIdx.x) * 256 + threadIdx.x; i < total; i += long(gridDim.x) * 256) {
-    const long b = i / (g.M * g.N);
-    const long r = i % (g.M * g.N);
+  const long mn = g.M * g.N;
+  const long total = long(g.batch) * mn;
+  const int lane = threadIdx.x & 63;
+  const long e = long(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (e >= total) return;   // uniform per wavefront
+  const long b = e / mn, r = e % mn;
+  const float* __restrict__ w = g.work + long(b) * g.splits * mn + r;
+  float s = 0.0f;
+  for (int sp = lane; sp < g.splits; sp += 64) s += w[long(sp) * mn];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) {
     const long m = r / g.N, n = r % g.N;
-    float s = 0.0f;
-    for (int sp = 0; sp < g.splits; ++sp) s += g.work[((b * g.splits + sp) * g.M + m) * g.N + n];
     float* c = g.C + b * g.sCb + m * g.sCm + n * g.sCn;
     *c = (g.beta != 0.0f) ? g.alpha * s + g.beta * (*c) : g.alpha * s;
   }
@@ -214,8 +269,8 @@ extern "C" int wfs_gemm_bf16(const float* A, const float* B, float* C, const lon
   WFS_HIP_CHECK(hipGetLastError());
   if (splits > 1) {
     const long total = long(g.batch) * g.M * g.N;
-    long blocks = (total + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
+    const long blocks = (total + 3) / 4;
+    if (blocks > 0x7fffffffL) { set_last_error("wfs_gemm_bf16: split-K output too large"); return WFS_ERR_INVALID; }
     hipLaunchKernelGGL(gemm_splitk_reduce_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, g);
     WFS_HIP_CHECK(hipGetLastError());
   }
