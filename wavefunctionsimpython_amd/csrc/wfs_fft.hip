@@ -64,11 +64,10 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
     else return (k1 * N2 + n2) * G + g;
   };
 
-  // complex64 state: the in-register DFT twiddles (wfs_common.h) and the kinetic factor
-  // are applied as hi + lo.  The table twiddles stay plain floats: their low parts would
-  // cut the 1600-step error at 128^3 from about 5e-5 to 9e-6 but cost a quarter of the
-  // time of the two double-transform passes.
-  constexpr bool kComp = sizeof(R) == 4;
+  // Table twiddles and kinetic factors are plain floats for complex64 states; the plan
+  // cycles dithered copies of those tables step by step so that their rounding does not
+  // repeat, and the in-register DFT twiddles (compile-time constants) are applied as
+  // hi + lo pairs (wfs_common.h).
   const C* __restrict__ tw = reinterpret_cast<const C*>(p.tw);
   C w[N1];
 #pragma unroll
@@ -113,23 +112,13 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
   } else if constexpr (MODE == MODE_FKI) {
     // x pass in full k-space: column = (ky, kz)
     transform(std::integral_constant<int, -1>{});
-    // K = kx ky kz formed in FP64 from FP64 per-axis tables
-    const double2* __restrict__ kx = reinterpret_cast<const double2*>(p.kx);
-    const double2* __restrict__ ky = reinterpret_cast<const double2*>(p.ky);
-    const double2* __restrict__ kz = reinterpret_cast<const double2*>(p.kz);
+    const C* __restrict__ kx = reinterpret_cast<const C*>(p.kx);
+    const C* __restrict__ ky = reinterpret_cast<const C*>(p.ky);
+    const C* __restrict__ kz = reinterpret_cast<const C*>(p.kz);
     const long cc = active ? col : 0;
-    const double2 kyz = cmul(ky[cc / p.nz], kz[cc % p.nz]);
+    const C kyz = cmul(ky[cc / p.nz], kz[cc % p.nz]);
 #pragma unroll
-    for (int i = 0; i < N1; ++i) {
-      const double2 K = cmul(kx[t + N2 * i], kyz);
-      if constexpr (kComp) {
-        const C hi = mk<R>(R(K.x), R(K.y));
-        const C lo = mk<R>(R(K.x - double(hi.x)), R(K.y - double(hi.y)));
-        a[i] = cmul2(a[i], hi, lo);
-      } else {
-        a[i] = cmul(a[i], K);
-      }
-    }
+    for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], cmul(kx[t + N2 * i], kyz));
     __syncthreads();
     transform(std::integral_constant<int, +1>{});
   } else if constexpr (MODE == MODE_IVF) {

@@ -14,7 +14,7 @@ struct PassParams {
   const void* tw = nullptr;     // forward twiddles exp(-2 pi i j / N), j < N, in the state precision
   long rows = 0;                // ROWS layout: number of contiguous rows of length N
   long inner = 0;               // COLS layout: stride of the axis = columns per outer slab
-  const void* kx = nullptr;     // FKI: per-axis kinetic factors (normalisation folded in), complex128
+  const void* kx = nullptr;     // FKI: per-axis kinetic factors (normalisation folded in), state precision
   const void* ky = nullptr;
   const void* kz = nullptr;
   long nz = 1;                  // FKI: column index -> (ky, kz) = (c / nz, c % nz)

@@ -14,9 +14,10 @@ struct wfs_plan {
   double dt, mass;
   int precision, method, order;
   size_t elem;            // bytes per state element
-  void* tw[3];            // forward twiddles per axis, state precision
-  void* kfull[3];         // exp(-i k_d^2 dt / 2m) / N_d, complex128 for every plan
-  void* khalf[3];         // exp(-i k_d^2 dt / 4m) / N_d, complex128
+  void* tw[3];            // forward twiddles per axis, state precision; ntab dithered copies
+  int ntab;               // copies of each per-axis table (complex64: kDither, complex128: 1)
+  void* kfull[3];         // exp(-i k_d^2 dt / 2m) / N_d, state precision, ntab copies
+  void* khalf[3];         // exp(-i k_d^2 dt / 4m) / N_d, state precision, ntab copies
   void* scratch[2];       // power-series ping-pong buffers (lazy)
   double* obs_work;       // observable partial records
   long obs_work_len;
@@ -75,6 +76,37 @@ int upload_table(void** dst, const std::vector<double>& re, const std::vector<do
   std::vector<float2> h(n);
   for (size_t j = 0; j < n; ++j) { h[j].x = float(re[j]); h[j].y = float(im[j]); }
   return upload(dst, h.data(), n * sizeof(float2));
+}
+
+// Copies of a per-axis table for complex64 plans, cycled step by step.  A factor
+// rounded to float and applied to the same mode every step errs the same way every
+// step, so its error grows linearly with the step count.  Each part x of every entry
+// is rounded down in some copies and up in the others, in proportion to where x lies
+// between the two floats, so the mean over the copies is within 1/(2 kDither) ulp of x;
+// which copies round up rotates with the entry, so no copy is biased as a whole.
+constexpr int kDither = 8;
+
+float dither_part(double x, long entry, int part, int copy) {
+  float lo = float(x);   // nearest float, then the float at or below x
+  if (double(lo) > x) lo = std::nextafter(lo, -INFINITY);
+  if (double(lo) == x) return lo;
+  const float hi = std::nextafter(lo, INFINITY);
+  const double frac = (x - double(lo)) / (double(hi) - double(lo));
+  const int ups = int(std::lround(frac * kDither));
+  const int slot = int((copy + 3 * entry + 5 * part) % kDither);
+  return slot < ups ? hi : lo;
+}
+
+int upload_dithered(void** dst, const std::vector<double>& re, const std::vector<double>& im, int precision) {
+  if (precision == WFS_C128) return upload_table(dst, re, im, precision);
+  const size_t n = re.size();
+  std::vector<float2> h(n * kDither);
+  for (int c = 0; c < kDither; ++c)
+    for (size_t j = 0; j < n; ++j) {
+      h[c * n + j].x = dither_part(re[j], long(j), 0, c);
+      h[c * n + j].y = dither_part(im[j], long(j), 1, c);
+    }
+  return upload(dst, h.data(), h.size() * sizeof(float2));
 }
 
 // cos/sin(2 pi j / n) with exact values at multiples of pi/2 (long double)
@@ -156,10 +188,16 @@ int kernel_class_of(int axis, int mode) {
 }
 
 // one axis pass; axis 0 = x (strided, whole grid), 1 = y (strided per x slab), 2 = z (rows)
+// sel picks the copy of the per-axis tables (the step index)
+const void* table_copy(const wfs_plan* p, const void* base, int axis, int sel) {
+  const size_t off = size_t(sel % p->ntab) * size_t(p->n[axis]) * p->elem;
+  return static_cast<const char*>(base) + off;
+}
+
 int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* pv, const void* const* ktab,
-             double scale, hipStream_t s) {
+             double scale, hipStream_t s, int sel = 0) {
   PassParams pp;
-  pp.tw = p->tw[axis];
+  pp.tw = table_copy(p, p->tw[axis], axis, sel);
   pp.scale = scale;
   long outer = 1;
   int layout;
@@ -177,7 +215,9 @@ int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* pv, const 
     pp.inner = long(p->n[1]) * p->n[2];
   }
   if (mode == MODE_FKI) {
-    pp.kx = ktab[0]; pp.ky = ktab[1]; pp.kz = ktab[2];
+    pp.kx = table_copy(p, ktab[0], 0, sel);
+    pp.ky = table_copy(p, ktab[1], 1, sel);
+    pp.kz = table_copy(p, ktab[2], 2, sel);
     pp.nz = p->n[2];
   }
   if (mode == MODE_IVF) pp.pv = pv;
@@ -204,17 +244,18 @@ int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t
   const void* pv = p->pv;
   const void* const* kh = p->khalf;
   const void* const* kf = p->kfull;
-  WFS_TRY(run_pass(p, 2, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
-  WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
-  WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, kh, 1.0, s));
+  // every pass of step k uses table copy k (mod ntab); the entry passes use copy 0
+  WFS_TRY(run_pass(p, 2, MODE_FWD, psi, nullptr, nullptr, 1.0, s, 0));
+  WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s, 0));
+  WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, kh, 1.0, s, 0));
   for (int step = 1; step <= nsteps; ++step) {
-    WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s));
-    WFS_TRY(run_pass(p, 2, MODE_IVF, psi, pv, nullptr, 1.0, s));
-    WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s));
-    WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, step == nsteps ? kh : kf, 1.0, s));
+    WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s, step - 1));
+    WFS_TRY(run_pass(p, 2, MODE_IVF, psi, pv, nullptr, 1.0, s, step));
+    WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s, step));
+    WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, step == nsteps ? kh : kf, 1.0, s, step));
   }
-  WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s));
-  WFS_TRY(run_pass(p, 2, MODE_INV, psi, nullptr, nullptr, 1.0, s));
+  WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s, nsteps));
+  WFS_TRY(run_pass(p, 2, MODE_INV, psi, nullptr, nullptr, 1.0, s, nsteps));
   return 0;
 }
 
@@ -303,6 +344,7 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->batch = batch;
   p->dt = dt; p->mass = mass; p->precision = precision; p->method = method; p->order = order;
   p->elem = precision == WFS_C128 ? 16 : 8;
+  p->ntab = precision == WFS_C128 ? 1 : kDither;
   for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
@@ -314,13 +356,13 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
       unit_root(j, n[a], c, s);
       re[j] = c; im[j] = -s;   // forward twiddle exp(-2 pi i j / N)
     }
-    rc = upload_table(&p->tw[a], re, im, precision);
+    rc = upload_dithered(&p->tw[a], re, im, precision);
     if (rc) break;
     kinetic_table(n[a], dx[a], dt, mass, 2.0, re, im);
-    rc = upload_table(&p->kfull[a], re, im, WFS_C128);
+    rc = upload_dithered(&p->kfull[a], re, im, precision);
     if (rc) break;
     kinetic_table(n[a], dx[a], dt, mass, 4.0, re, im);
-    rc = upload_table(&p->khalf[a], re, im, WFS_C128);
+    rc = upload_dithered(&p->khalf[a], re, im, precision);
   }
   if (rc == 0) {
     wfs_box whole{{0, 0, 0}, {n[0], n[1], n[2]}};
