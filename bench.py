@@ -125,11 +125,14 @@ def gpu_bench(steps, warmup):
     torch.cuda.synchronize()
     ps_prof = ps_plan.profile(reset=True)
     hl, hms = ps_prof["horner_stage"]
+    # order 10 runs as 5 fused stage pairs per step; a pair reads u, psi and V once and writes
+    # once: 32 algorithmic bytes per point per launch (16 per stage)
     power_series = {
         "value": npts * ps_steps / ps_elapsed, "unit": "grid-pt-steps/s", "order": 10,
         "ms_per_step": ps_elapsed / ps_steps * 1e3,
-        "horner_stage_avg_us": round(hms / hl * 1e3, 2),
-        "horner_stage_GBs": round(32 * npts / (hms / hl * 1e-3) / 1e9, 1),
+        "launches_per_step": hl / 2, "stages_per_launch": 20 / hl,
+        "horner_launch_avg_us": round(hms / hl * 1e3, 2),
+        "horner_launch_GBs": round(32 * npts / (hms / hl * 1e-3) / 1e9, 1),
     }
     return {
         "elapsed": elapsed, "npts": npts, "norm": norm, "kernels": kernels, "dominant": dominant,

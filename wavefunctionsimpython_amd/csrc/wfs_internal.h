@@ -32,6 +32,9 @@ int launch_fft_pass(int precision, int n, int layout, int mode, void* data, cons
 int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s);
 
 // power series: one Horner stage  u_out = psi + (-i dt / n) H u_in
+int launch_horner_pair(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
+                       const void* psi, const float2* V, double c1, double c2, double inv_2m, double inv_dx2,
+                       int variant, hipStream_t s);
 int launch_horner_stage(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
                         const void* psi, const float2* V, double dt_over_n, double inv_2m, double inv_dx2,
                         int tile_x, hipStream_t s);

@@ -265,15 +265,30 @@ int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStr
   const double inv_2m = 1.0 / (2.0 * p->mass);
   const double inv_dx2 = 1.0 / (p->dx[0] * p->dx[0]);   // x spacing on every axis (power_series_operator.cu:58)
   const size_t bytes = grid_points(p) * p->elem;
+  // variant 1 (default): stages in fused pairs; variant 0: one launch per stage.  Both give
+  // the same bits.  A pair reads psi around its tile, so it never writes psi in place; the
+  // result of the last pair is copied back, a last single stage writes psi directly.
+  const bool pairs = p->variant[WFS_K_HORNER] != 0;
   for (int step = 0; step < nsteps; ++step) {
     const void* u_in = psi;
     int which = 0;
-    for (int nn = p->order; nn >= 1; --nn) {
-      void* out = (nn == 1 && u_in != psi) ? psi : p->scratch[which];
+    int nn = p->order;
+    while (nn >= 1) {
       ProfileScope scope(p, WFS_K_HORNER, s);
-      WFS_TRY(launch_horner_stage(p->precision, p->batch, p->n[0], p->n[1], p->n[2], u_in, out, psi,
-                                  static_cast<const float2*>(V), p->dt / double(nn), inv_2m, inv_dx2, 0, s));
-      u_in = out;
+      if (pairs && nn >= 2) {
+        void* out = p->scratch[which];
+        WFS_TRY(launch_horner_pair(p->precision, p->batch, p->n[0], p->n[1], p->n[2], u_in, out, psi,
+                                   static_cast<const float2*>(V), p->dt / double(nn), p->dt / double(nn - 1),
+                                   inv_2m, inv_dx2, p->variant[WFS_K_HORNER], s));
+        u_in = out;
+        nn -= 2;
+      } else {
+        void* out = (nn == 1 && u_in != psi) ? psi : p->scratch[which];
+        WFS_TRY(launch_horner_stage(p->precision, p->batch, p->n[0], p->n[1], p->n[2], u_in, out, psi,
+                                    static_cast<const float2*>(V), p->dt / double(nn), inv_2m, inv_dx2, 0, s));
+        u_in = out;
+        nn -= 1;
+      }
       which ^= 1;
     }
     if (u_in != psi) WFS_HIP_CHECK(hipMemcpyAsync(psi, u_in, bytes, hipMemcpyDeviceToDevice, s));
@@ -346,6 +361,7 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->elem = precision == WFS_C128 ? 16 : 8;
   p->ntab = precision == WFS_C128 ? 1 : kDither;
   for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
+  p->variant[WFS_K_HORNER] = 2;   // fused stage pairs on 8 x 64 tiles (measured fastest at 256^3)
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;

@@ -21,6 +21,25 @@
 
 namespace wfs {
 
+// One Horner update at a point: psi + (-i c) (-(1/2m) L u + V u), with L u * dx^2 summed in the
+// reference's order.  Shared by the single-stage and the two-stage kernels so that both round
+// identically.
+template <typename R>
+__device__ __forceinline__ cx<R> horner_point(cx<R> cur, cx<R> xm, cx<R> xp, cx<R> ym, cx<R> yp, cx<R> zm,
+                                              cx<R> zp, cx<R> ps, float2 v, R c, R inv_2m, R inv_dx2) {
+  cx<R> lap;
+  lap.x = xm.x + xp.x + ym.x - R(6) * cur.x + yp.x + zm.x + zp.x;
+  lap.y = xm.y + xp.y + ym.y - R(6) * cur.y + yp.y + zm.y + zp.y;
+  cx<R> hu;
+  hu.x = -inv_2m * inv_dx2 * lap.x + (R(v.x) * cur.x - R(v.y) * cur.y);
+  hu.y = -inv_2m * inv_dx2 * lap.y + (R(v.x) * cur.y + R(v.y) * cur.x);
+  // psi + (-i c) H u = psi + c (Hu.y, -Hu.x)
+  cx<R> o;
+  o.x = ps.x + c * hu.y;
+  o.y = ps.y - c * hu.x;
+  return o;
+}
+
 template <typename R, int TZ, int TY>
 __global__ __launch_bounds__(TZ* TY) void horner_stage_kernel(const cx<R>* __restrict__ u_in, cx<R>* u_out,
                                                              const cx<R>* psi, const float2* __restrict__ V,
@@ -57,24 +76,212 @@ __global__ __launch_bounds__(TZ* TY) void horner_stage_kernel(const cx<R>* __res
     const C uzm = u_in[at(x, y, zm)];
     const C uzp = u_in[at(x, y, zp)];
     const long i = at(x, y, z);
-    const float2 v = V[i];
-    const C ps = psi[i];
-    // L u * dx^2 in the reference's summation order
-    C lap;
-    lap.x = prev.x + next.x + uym.x - R(6) * cur.x + uyp.x + uzm.x + uzp.x;
-    lap.y = prev.y + next.y + uym.y - R(6) * cur.y + uyp.y + uzm.y + uzp.y;
-    // H u = -(1/2m) L u + V u
-    C hu;
-    hu.x = -inv_2m * inv_dx2 * lap.x + (R(v.x) * cur.x - R(v.y) * cur.y);
-    hu.y = -inv_2m * inv_dx2 * lap.y + (R(v.x) * cur.y + R(v.y) * cur.x);
-    // u_out = psi + (-i dt/n) H u = psi + (dt/n) * (Hu.y, -Hu.x)
-    C o;
-    o.x = ps.x + dt_over_n * hu.y;
-    o.y = ps.y - dt_over_n * hu.x;
-    u_out[i] = o;
+    u_out[i] = horner_point<R>(cur, prev, next, uym, uyp, uzm, uzp, psi[i], V[i], dt_over_n, inv_2m, inv_dx2);
     prev = cur;
     cur = next;
   }
+}
+
+// Two Horner stages in one pass (temporal blocking): w = psi + (-i c1) H u, then
+// u_out = psi + (-i c2) H w.  A block owns a TY x TZ (y, z) tile and marches along x
+// through a slab; the LDS holds three x planes of u with a two-point (y, z) halo and
+// three planes of w with a one-point halo.  Per output point it moves about 37 B per two
+// stages (u with halo, psi and V, u_out) instead of 2 x 32.  Faces replicate the edge
+// voxel as in the single stage: w outside the grid is a copy of w at the clamped voxel,
+// and out-of-grid u cells hold u at the clamped voxel.  w is rounded to the state
+// precision exactly as the single stage stores it, so the result equals two single stages
+// bit for bit.  u_out must not alias psi or u_in (other blocks read their halos).
+template <typename R, int TZ, int TY, int NT>
+__global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u_in, cx<R>* __restrict__ u_out,
+                                                     const cx<R>* __restrict__ psi, const float2* __restrict__ V,
+                                                     int nx, int ny, int nz, R c1, R c2, R inv_2m, R inv_dx2,
+                                                     int tile_x, int ntiles) {
+  using C = cx<R>;
+  constexpr int UY = TY + 4, UZ = TZ + 4;   // u window (halo 2)
+  constexpr int WY = TY + 2, WZ = TZ + 2;   // w window (halo 1)
+  constexpr int RU = (UY * UZ + NT - 1) / NT;   // u cells per thread
+  constexpr int RW = (WY * WZ + NT - 1) / NT;   // w cells per thread
+  __shared__ C su[3][UY][UZ];
+  __shared__ C sw[3][WY][WZ];
+
+  const int tid = threadIdx.x;
+  const int z0 = blockIdx.x * TZ, y0 = blockIdx.y * TY;
+  const int packet = blockIdx.z / ntiles;
+  const int x0 = (blockIdx.z % ntiles) * tile_x;
+  int x1 = x0 + tile_x;
+  if (x1 > nx) x1 = nx;
+  const long poff = long(packet) * nx * ny * nz;
+  u_in += poff;
+  u_out += poff;
+  psi += poff;
+  V += poff;
+  const long sx = long(ny) * nz, sy = nz;
+  auto clampi = [](int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); };
+  auto slot = [](int x) { return ((x % 3) + 3) % 3; };
+  // does the w window reach outside the grid in y or z (then out-of-grid w cells are copied)
+  const bool face = (y0 == 0) || (z0 == 0) || (y0 + TY >= ny) || (z0 + TZ >= nz);
+
+  // Each thread owns the u cells e = tid + NT r and the w cells e = tid + NT r of the two
+  // windows.  The next planes' operands are fetched into registers one iteration ahead so
+  // their latency overlaps the current plane's arithmetic; psi and V of a w cell are kept
+  // for the output at the same point one plane later.
+  C ureg[RU];
+  C pnext[RW], pcur[RW];
+  float2 vnext[RW], vcur[RW];
+
+  auto fetch_u = [&](int x) {
+    const long xo = long(clampi(x, nx)) * sx;
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      const int e = tid + NT * r;
+      if (e < UY * UZ) {
+        const int gy = clampi(y0 - 2 + e / UZ, ny), gz = clampi(z0 - 2 + e % UZ, nz);
+        ureg[r] = u_in[xo + long(gy) * sy + gz];
+      }
+    }
+  };
+  auto store_u = [&](int x) {
+    C* dst = &su[slot(x)][0][0];
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      const int e = tid + NT * r;
+      if (e < UY * UZ) dst[e] = ureg[r];
+    }
+  };
+  // psi and V of the owned in-grid w cells of plane x (x in the grid)
+  auto fetch_pv = [&](int x) {
+    const long xo = long(x) * sx;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int e = tid + NT * r;
+      const int gy = y0 - 1 + e / WZ, gz = z0 - 1 + e % WZ;
+      if (e < WY * WZ && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+        const long i = xo + long(gy) * sy + gz;
+        pnext[r] = psi[i];
+        vnext[r] = V[i];
+      }
+    }
+  };
+  // w plane x from u(x-1), u(x), u(x+1) and pnext/vnext; out-of-grid cells copied after a barrier
+  auto make_w = [&](int x) {
+    if (x >= 0 && x < nx) {
+      const int ws = slot(x), um = slot(x - 1), uc = slot(x), up = slot(x + 1);
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        const int e = tid + NT * r;
+        const int j = e / WZ, k = e % WZ;
+        const int gy = y0 - 1 + j, gz = z0 - 1 + k;
+        if (e < WY * WZ && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+          const int a = j + 1, b = k + 1;
+          sw[ws][j][k] = horner_point<R>(su[uc][a][b], su[um][a][b], su[up][a][b], su[uc][a - 1][b],
+                                         su[uc][a + 1][b], su[uc][a][b - 1], su[uc][a][b + 1], pnext[r], vnext[r],
+                                         c1, inv_2m, inv_dx2);
+        }
+      }
+      if (face) {
+        __syncthreads();
+        for (int e = tid; e < WY * WZ; e += NT) {
+          const int j = e / WZ, k = e % WZ;
+          const int gy = y0 - 1 + j, gz = z0 - 1 + k;
+          if (gy >= 0 && gy < ny && gz >= 0 && gz < nz) continue;
+          sw[ws][j][k] = sw[ws][clampi(gy, ny) - (y0 - 1)][clampi(gz, nz) - (z0 - 1)];
+        }
+      }
+    } else {
+      const int ws = slot(x), fs = slot(clampi(x, nx));
+      for (int e = tid; e < WY * WZ; e += NT) {
+        const int j = e / WZ, k = e % WZ;
+        sw[ws][j][k] = sw[fs][j][k];
+      }
+    }
+  };
+
+  // prologue: u(x0-2..x0+2) and w(x0-1), w(x0); pcur = psi, V of plane x0
+  fetch_u(x0 - 2);
+  store_u(x0 - 2);
+  fetch_u(x0 - 1);
+  store_u(x0 - 1);
+  fetch_u(x0);
+  store_u(x0);
+  if (x0 - 1 >= 0) fetch_pv(x0 - 1);
+  __syncthreads();
+  if (x0 - 1 >= 0) {
+    make_w(x0 - 1);
+    __syncthreads();
+  }
+  fetch_u(x0 + 1);
+  store_u(x0 + 1);           // replaces u(x0-2)
+  fetch_pv(x0);
+  __syncthreads();
+  make_w(x0);
+  __syncthreads();
+  if (x0 - 1 < 0) {
+    make_w(x0 - 1);          // copy of w(x0)
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < RW; ++r) { pcur[r] = pnext[r]; vcur[r] = vnext[r]; }
+  fetch_u(x0 + 2);
+  store_u(x0 + 2);           // replaces u(x0-1)
+  if (x0 + 1 < nx) fetch_pv(x0 + 1);
+  __syncthreads();
+
+  // state at the top of iteration x: su = u(x..x+2), sw = w(x-1), w(x), pnext = psi/V(x+1),
+  // pcur = psi/V(x)
+  for (int x = x0; x < x1; ++x) {
+    fetch_u(x + 3);                           // into registers, stored after this plane
+    make_w(x + 1);                            // replaces w(x-2)
+    __syncthreads();
+    // output plane x at the owned core cells (w cells with 1 <= j <= TY, 1 <= k <= TZ)
+    const int wm = slot(x - 1), wc = slot(x), wp = slot(x + 1);
+    const long xo = long(x) * sx;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int e = tid + NT * r;
+      const int a = e / WZ, b = e % WZ;
+      const int gy = y0 - 1 + a, gz = z0 - 1 + b;
+      if (e < WY * WZ && a >= 1 && a <= TY && b >= 1 && b <= TZ && gy < ny && gz < nz) {
+        const long i = xo + long(gy) * sy + gz;
+        u_out[i] = horner_point<R>(sw[wc][a][b], sw[wm][a][b], sw[wp][a][b], sw[wc][a - 1][b], sw[wc][a + 1][b],
+                                   sw[wc][a][b - 1], sw[wc][a][b + 1], pcur[r], vcur[r], c2, inv_2m, inv_dx2);
+      }
+    }
+    store_u(x + 3);                           // replaces u(x), last read by w(x + 1)
+#pragma unroll
+    for (int r = 0; r < RW; ++r) { pcur[r] = pnext[r]; vcur[r] = vnext[r]; }
+    if (x + 2 < nx) fetch_pv(x + 2);
+    __syncthreads();
+  }
+}
+
+int launch_horner_pair(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
+                       const void* psi, const float2* V, double c1, double c2, double inv_2m, double inv_dx2,
+                       int variant, hipStream_t s) {
+  const int tile_x = nx < 32 ? nx : 32;
+  const int ntiles = (nx + tile_x - 1) / tile_x;
+  if (precision == WFS_C128) {
+    constexpr int TZ = 32, TY = 8, NT = 256;
+    dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
+    hipLaunchKernelGGL((horner2_kernel<double, TZ, TY, NT>), grid, dim3(NT), 0, s,
+                       static_cast<const double2*>(u_in), static_cast<double2*>(u_out),
+                       static_cast<const double2*>(psi), V, nx, ny, nz, c1, c2, inv_2m, inv_dx2, tile_x, ntiles);
+  } else if (variant == 2) {
+    // smaller (y, z) tile: more halo per point, half the LDS, twice the blocks per CU
+    constexpr int TZ = 64, TY = 8, NT = 256;
+    dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
+    hipLaunchKernelGGL((horner2_kernel<float, TZ, TY, NT>), grid, dim3(NT), 0, s,
+                       static_cast<const float2*>(u_in), static_cast<float2*>(u_out),
+                       static_cast<const float2*>(psi), V, nx, ny, nz, float(c1), float(c2), float(inv_2m),
+                       float(inv_dx2), tile_x, ntiles);
+  } else {
+    constexpr int TZ = 64, TY = 16, NT = 256;
+    dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
+    hipLaunchKernelGGL((horner2_kernel<float, TZ, TY, NT>), grid, dim3(NT), 0, s,
+                       static_cast<const float2*>(u_in), static_cast<float2*>(u_out),
+                       static_cast<const float2*>(psi), V, nx, ny, nz, float(c1), float(c2), float(inv_2m),
+                       float(inv_dx2), tile_x, ntiles);
+  }
+  return int(hipGetLastError());
 }
 
 int launch_horner_stage(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
