@@ -317,15 +317,90 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
     }
 
 
+def large_bench(steps):
+    """512^3 complex64, V = 3 sum_d cos^2(pi r_d / 4) + a cubic CAP of 16 Bohr (box 128 Bohr,
+    observation box +-48), packet sigma 5 at the origin with k0 = (1, 1, 0), dt 0.002.
+    Split-step and power series (order 10) over `steps` steps, each against the complex128
+    split-step run of the same problem (the FP64 reference at this size)."""
+    import torch
+    from wavefunctionsimpython_amd import _abi
+    n, box = 512, 128.0
+    shape, dx = (n,) * 3, (box / n,) * 3
+    dt = 0.002
+    dev = torch.device("cuda", 0)
+    V = torch.zeros(shape, dtype=torch.complex64, device=dev)
+    _abi.add_lattice(shape, dx, 3.0, 4.0, V)
+    _abi.add_cap(shape, dx, (-48.0,) * 3, (48.0,) * 3, 0.0, (-100.0,) * 3, (-100.0,) * 3, (3.0,) * 3, (3.0,) * 3, V)
+    psi0 = torch.empty(shape, dtype=torch.complex64, device=dev)
+    _abi.init_gaussian(shape, dx, _abi.WFS_C64, (5.0,) * 3, (0.0, 0.0, 0.0), (1.0, 1.0, 0.0), psi0)
+    npts = n ** 3
+    out = {}
+    ref = psi0.to(torch.complex128)
+    p64 = _abi.Plan(shape, dx, dt, 1.0, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+    p64.set_potential(V)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p64.step(ref, None, steps)
+    torch.cuda.synchronize()
+    out["fourier_c128"] = {"seconds": time.perf_counter() - t0,
+                           "grid_pt_steps_per_s": npts * steps / (time.perf_counter() - t0)}
+    del p64
+    norm_ref = float(ref.abs().pow(2).sum().sqrt())
+
+    def rel(a):
+        return float((a.to(torch.complex128) - ref).abs().pow(2).sum().sqrt()) / norm_ref
+
+    for name, method in (("fourier", _abi.WFS_METHOD_FOURIER), ("power_series", _abi.WFS_METHOD_POWER_SERIES)):
+        plan = _abi.Plan(shape, dx, dt, 1.0, _abi.WFS_C64, method, order=10)
+        plan.set_potential(V)
+        if method == _abi.WFS_METHOD_FOURIER:
+            plan.autotune()
+        psi = psi0.clone()
+        plan.step(psi, V, 2)
+        psi.copy_(psi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.step(psi, V, steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = {"seconds": el, "grid_pt_steps_per_s": npts * steps / el, "ms_per_step": el / steps * 1e3,
+                     "rel_l2_vs_fourier_c128": rel(psi)}
+        print(f"{name}: {el:.2f} s", file=sys.stderr, flush=True)
+        del plan
+        if method == _abi.WFS_METHOD_POWER_SERIES:
+            # the power series against its own complex128 run (precision, not method, error)
+            p64 = _abi.Plan(shape, dx, dt, 1.0, _abi.WFS_C128, method, order=10)
+            own = psi0.to(torch.complex128)
+            p64.step(own, V, steps)
+            d = float((psi.to(torch.complex128) - own).abs().pow(2).sum().sqrt() / own.abs().pow(2).sum().sqrt())
+            out[name]["rel_l2_vs_power_series_c128"] = d
+            del p64, own
+        del psi
+        torch.cuda.empty_cache()
+    return {
+        "metric": "grid-pt-steps/s at 512^3 (split-step Fourier, complex64)",
+        "value": out["fourier"]["grid_pt_steps_per_s"], "unit": "grid-pt-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 2, "ms_per_step": out["fourier"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "512^3 cos^2 lattice (3 Ha, period 4 Bohr) + CAP 16 Bohr, box 128 Bohr, "
+                               "sigma 5, k0 (1,1,0), dt 0.002", "grid": [n, n, n]},
+        "methods": out,
+        "note": "power_series rel_l2_vs_fourier_c128 is mostly method error (7-point Laplacian and Taylor "
+                "truncation against the spectral propagator); rel_l2_vs_power_series_c128 is its precision error. "
+                "The FNO-parareal leg is not run at 512^3 (one width-32 activation is 17 GiB)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal"], default="splitstep256",
-                    help="splitstep256 (default, the headline metric), batch (48 packets, one hipGraph) or "
-                         "parareal (time-to-T on the 128^3 lattice, FNO training included)")
+    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal", "large"], default="splitstep256",
+                    help="splitstep256 (default, the headline metric), batch (48 packets, one hipGraph), "
+                         "parareal (time-to-T on the 128^3 lattice, FNO training included) or large (512^3, "
+                         "split-step and power series against complex128; --steps, 2000 in the config)")
     ap.add_argument("--train-pairs", type=int, default=2000, help="FNO training pairs (parareal workload)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 5 device copies of a 128 MiB buffer (HBM counter calibration) and exit")
@@ -351,6 +426,9 @@ def main():
         return
     if args.workload == "parareal":
         print(json.dumps(parareal_bench(args.train_pairs)), flush=True)
+        return
+    if args.workload == "large":
+        print(json.dumps(large_bench(args.steps)), flush=True)
         return
     g = gpu_bench(args.steps, args.warmup)
     value = g["npts"] * args.steps / g["elapsed"]
