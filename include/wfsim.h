@@ -229,12 +229,16 @@ int wfs_plan_profile(wfs_plan* plan, int kernel_class, int reset, long* launches
  * / WFS_GEMM_SPLIT_B stage that operand as bf16 hi + lo parts (about 16
  * significant bits; Ahi Bhi + Alo Bhi + Ahi Blo).  splits > 1 splits K over
  * blocks and needs work_elems >= batch*splits*M*N floats; the splits are
- * added in a fixed order.
+ * added in a fixed order.  Epilogue: bias (may be NULL) adds bias[m] after
+ * alpha/beta; act (may be NULL) receives GELU of the result with C's strides;
+ * WFS_GEMM_NO_C (with act) leaves C unwritten.
  */
 #define WFS_GEMM_SPLIT_A 1
 #define WFS_GEMM_SPLIT_B 2
+#define WFS_GEMM_NO_C 4
 int wfs_gemm_bf16(const float* A, const float* B, float* C, const long dims[4], const long strides[9], float alpha,
-                  float beta, int flags, int splits, float* work, long work_elems, void* stream);
+                  float beta, int flags, int splits, float* work, long work_elems, const float* bias, float* act,
+                  void* stream);
 
 /*
  * Fourier-neural-operator and parareal elementwise kernels (no reference

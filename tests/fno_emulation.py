@@ -29,7 +29,8 @@ class EmulatedFNO(FNO3d):
     dtype = torch.float64
     bf16_operands = False
 
-    def _gemm(self, A, B, C, batch, M, N, K, sA, sB, sC, alpha=1.0, beta=0.0, splits=1, flags=0):
+    def _gemm(self, A, B, C, batch, M, N, K, sA, sB, sC, alpha=1.0, beta=0.0, splits=1, flags=0, bias=None,
+              act=None):
         for t, st, dims in ((A, sA, (batch, M, K)), (B, sB, (batch, K, N)), (C, sC, (batch, M, N))):
             span = sum((d - 1) * s for d, s in zip(dims, st))
             assert t.is_contiguous() and min(st) >= 0 and span < t.numel()
@@ -37,20 +38,24 @@ class EmulatedFNO(FNO3d):
         Bv = torch.as_strided(B, (batch, K, N), sB)
         Cv = torch.as_strided(C, (batch, M, N), sC)
         if self.bf16_operands:
-            if not self.split_activations:
-                flags = 0
+            split = flags if self.split_activations else 0
             Ah, Bh = round_bf16(Av), round_bf16(Bv)
             prod = torch.bmm(Ah, Bh)
-            if flags & 1:   # A as hi + lo
+            if split & 1:   # A as hi + lo
                 prod = prod + torch.bmm(round_bf16(Av.float().double() - Ah), Bh)
-            if flags & 2:
+            if split & 2:
                 prod = prod + torch.bmm(Ah, round_bf16(Bv.float().double() - Bh))
             r = alpha * prod
         else:
             r = alpha * torch.bmm(Av, Bv)
         if beta != 0.0:
             r = r + beta * Cv
-        Cv.copy_(r)
+        if bias is not None:
+            r = r + bias[:M].view(1, M, 1)
+        if act is not None:
+            torch.as_strided(act, (batch, M, N), sC).copy_(fno_oracle.gelu_tanh(r))
+        if not flags & 4:
+            Cv.copy_(r)
 
     def _bias_act(self, u, bias, v, rows, act):
         uu = u.view(rows, self.P)

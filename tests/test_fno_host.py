@@ -54,3 +54,13 @@ def test_fno_rejects_bad_modes():
         FNO3d(8, modes=3, device=torch.device("cpu"))
     with pytest.raises(ValueError):
         FNO3d(8, modes=10, device=torch.device("cpu"))
+
+
+def test_fno_inference_forward_equals_training_forward():
+    """Without keep the layer outputs skip storing the pre-activations; the result is the same."""
+    net = EmulatedFNO(8, width=4, modes=4, layers=3, in_channels=3, seed=2, device=torch.device("cpu"))
+    a = torch.from_numpy(np.random.default_rng(1).standard_normal(3 * 8 ** 3))
+    kept = net.forward(a, keep=True).clone()
+    net._saved = None
+    plain = net.forward(a, keep=False)
+    assert torch.equal(kept, plain)

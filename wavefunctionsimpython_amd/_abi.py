@@ -44,7 +44,7 @@ SIGNATURES = {
                                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "wfs_plan_destroy": (ctypes.c_int, [_vp]),
     "wfs_gemm_bf16": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long * 4, ctypes.c_long * 9, ctypes.c_float,
-                                     ctypes.c_float, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_long, _vp]),
+                                     ctypes.c_float, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp]),
     "wfs_fno_bias_act": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_int, _vp]),
     "wfs_fno_gelu_backward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, _vp]),
     "wfs_fno_axpby": (ctypes.c_int, [_vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_long, _vp]),
@@ -319,19 +319,22 @@ class Plan:
             pass
 
 
-GEMM_SPLIT_A, GEMM_SPLIT_B = 1, 2
+GEMM_SPLIT_A, GEMM_SPLIT_B, GEMM_NO_C = 1, 2, 4
 
 
 def gemm_bf16(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, batch: int, M: int, N: int, K: int,
               sA, sB, sC, alpha: float = 1.0, beta: float = 0.0, splits: int = 1,
-              work: torch.Tensor | None = None, stream=None, flags: int = 0) -> None:
-    """C[b](m,n) = alpha sum_k A[b](m,k) B[b](k,n) + beta C[b](m,n) on the bf16 matrix cores.
+              work: torch.Tensor | None = None, stream=None, flags: int = 0,
+              bias: torch.Tensor | None = None, act: torch.Tensor | None = None) -> None:
+    """C[b](m,n) = alpha sum_k A[b](m,k) B[b](k,n) + beta C[b](m,n) (+ bias[m]) on the bf16 matrix cores.
 
     sA = (batch, m, k), sB = (batch, k, n), sC = (batch, m, n) element strides of float32
     device tensors.  Every addressed element is checked to lie inside its tensor.
-    flags: GEMM_SPLIT_A / GEMM_SPLIT_B carry that operand as bf16 hi + lo parts.
+    flags: GEMM_SPLIT_A / GEMM_SPLIT_B carry that operand as bf16 hi + lo parts; GEMM_NO_C
+    (with act) leaves C unwritten.  act receives GELU of the result with C's strides.
     """
-    for t, name in ((A, "A"), (B, "B"), (C, "C")):
+    tensors = [(A, "A"), (B, "B"), (C, "C")] + [(t, nm) for t, nm in ((bias, "bias"), (act, "act")) if t is not None]
+    for t, name in tensors:
         _require_device_any(t, name)
         if t.dtype != torch.float32:
             raise TypeError(f"{name} must be float32")
@@ -343,9 +346,16 @@ def gemm_bf16(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, batch: int, M: 
 
     if min(batch, M, N, K) < 1:
         raise ValueError("GEMM sizes must be >= 1")
-    for t, st, dims, name in ((A, sA, (batch, M, K), "A"), (B, sB, (batch, K, N), "B"), (C, sC, (batch, M, N), "C")):
+    checks = [(A, sA, (batch, M, K), "A"), (B, sB, (batch, K, N), "B"), (C, sC, (batch, M, N), "C")]
+    if act is not None:
+        checks.append((act, sC, (batch, M, N), "act"))
+    for t, st, dims, name in checks:
         if min(st) < 0 or span(st, *dims) >= t.numel():
             raise ValueError(f"GEMM operand {name} strides {st} reach outside its {t.numel()} elements")
+    if bias is not None and bias.numel() < M:
+        raise ValueError(f"bias has {bias.numel()} elements, the GEMM has {M} rows")
+    if (flags & GEMM_NO_C) and act is None:
+        raise ValueError("GEMM_NO_C needs an activation output")
     if splits > 1:
         need = batch * splits * M * N
         if work is None or work.numel() < need or work.dtype != torch.float32:
@@ -354,6 +364,7 @@ def gemm_bf16(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, batch: int, M: 
     st = (ctypes.c_long * 9)(*(list(sA) + list(sB) + list(sC)))
     check(lib().wfs_gemm_bf16(ptr(A), ptr(B), ptr(C), dims, st, float(alpha), float(beta), int(flags), int(splits),
                               ptr(work) if work is not None else None, int(work.numel()) if work is not None else 0,
+                              ptr(bias) if bias is not None else None, ptr(act) if act is not None else None,
                               stream_handle(stream)), "wfs_gemm_bf16")
 
 

@@ -251,6 +251,20 @@ struct DFT {
   }
 };
 
+// GELU, tanh form: 0.5 u (1 + tanh(sqrt(2/pi) (u + 0.044715 u^3))), and its derivative
+__device__ __forceinline__ float gelu_f(float u) {
+  const float k = 0.7978845608028654f;
+  const float t = tanhf(k * (u + 0.044715f * u * u * u));
+  return 0.5f * u * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad(float u) {
+  const float k = 0.7978845608028654f;
+  const float a = k * (u + 0.044715f * u * u * u);
+  const float t = tanhf(a);
+  const float da = k * (1.0f + 3.0f * 0.044715f * u * u);
+  return 0.5f * (1.0f + t) + 0.5f * u * (1.0f - t * t) * da;
+}
+
 // error reporting shared by all translation units
 void set_last_error(const char* fmt, ...);
 

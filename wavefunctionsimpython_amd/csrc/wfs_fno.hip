@@ -16,20 +16,6 @@ unsigned blocks_of(long n) {
   return unsigned(b);
 }
 
-// GELU, tanh form: 0.5 u (1 + tanh(sqrt(2/pi) (u + 0.044715 u^3)))
-__device__ __forceinline__ float gelu_f(float u) {
-  const float k = 0.7978845608028654f;
-  const float t = tanhf(k * (u + 0.044715f * u * u * u));
-  return 0.5f * u * (1.0f + t);
-}
-__device__ __forceinline__ float gelu_grad(float u) {
-  const float k = 0.7978845608028654f;
-  const float a = k * (u + 0.044715f * u * u * u);
-  const float t = tanhf(a);
-  const float da = k * (1.0f + 3.0f * 0.044715f * u * u);
-  return 0.5f * (1.0f + t) + 0.5f * u * (1.0f - t * t) * da;
-}
-
 // u[c][p] += bias[c]; v = gelu(u) (or v = u)
 __global__ __launch_bounds__(kT) void bias_act_kernel(float* __restrict__ u, const float* __restrict__ bias,
                                                       float* __restrict__ v, long C, long P, int act) {

@@ -49,11 +49,22 @@ struct GemmArgs {
   long sCb, sCm, sCn;
   float alpha, beta;
   int batch, splits;
-  int flags;           // bit 0: split A into hi + lo; bit 1: split B
+  int flags;           // bit 0: split A into hi + lo; bit 1: split B; bit 2: do not store C
+  const float* bias;   // per-row bias added after alpha/beta (may be null)
+  float* act;          // GELU of the result, same strides as C (may be null)
   long kchunk;         // K range per split (multiple of 32)
 };
 
 constexpr int kBK = 32;
+
+// C = alpha acc + beta C (+ bias[m]); act = GELU(C); C not stored with flag bit 2
+__device__ __forceinline__ void epilogue(const GemmArgs& g, long b, long m, long n, float accv) {
+  const long off = b * g.sCb + m * g.sCm + n * g.sCn;
+  float val = (g.beta != 0.0f) ? g.alpha * accv + g.beta * g.C[off] : g.alpha * accv;
+  if (g.bias) val = val + g.bias[m];
+  if (!(g.flags & 4)) g.C[off] = val;
+  if (g.act) g.act[off] = gelu_f(val);
+}
 
 template <int WM, int WN, bool SA, bool SB>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
@@ -189,8 +200,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
     const long m = m0 + wm * 32 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
     if (m < g.M && n < g.N) {
       if (g.splits == 1) {
-        float* c = g.C + long(b) * g.sCb + m * g.sCm + n * g.sCn;
-        *c = (g.beta != 0.0f) ? g.alpha * acc[v] + g.beta * (*c) : g.alpha * acc[v];
+        epilogue(g, b, m, n, acc[v]);
       } else {
         g.work[((long(b) * g.splits + split) * g.M + m) * g.N + n] = acc[v];
       }
@@ -211,11 +221,7 @@ __global__ __launch_bounds__(256) void gemm_splitk_reduce_kernel(GemmArgs g) {
   float s = 0.0f;
   for (int sp = lane; sp < g.splits; sp += 64) s += w[long(sp) * mn];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) {
-    const long m = r / g.N, n = r % g.N;
-    float* c = g.C + b * g.sCb + m * g.sCm + n * g.sCn;
-    *c = (g.beta != 0.0f) ? g.alpha * s + g.beta * (*c) : g.alpha * s;
-  }
+  if (lane == 0) epilogue(g, b, r / g.N, r % g.N, s);
 }
 
 }  // namespace wfs
@@ -235,7 +241,7 @@ static void launch_tile(dim3 grid, hipStream_t s, const GemmArgs& g) {
 // dims = {batch, M, N, K}; strides = {sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn} (elements)
 extern "C" int wfs_gemm_bf16(const float* A, const float* B, float* C, const long dims[4], const long strides[9],
                              float alpha, float beta, int flags, int splits, float* work, long work_elems,
-                             void* stream) {
+                             const float* bias, float* act, void* stream) {
   if (!A || !B || !C || !dims || !strides) { set_last_error("wfs_gemm_bf16: null argument"); return WFS_ERR_INVALID; }
   GemmArgs g{};
   g.A = A; g.B = B; g.C = C; g.work = work;
@@ -243,8 +249,9 @@ extern "C" int wfs_gemm_bf16(const float* A, const float* B, float* C, const lon
   g.sAb = strides[0]; g.sAm = strides[1]; g.sAk = strides[2];
   g.sBb = strides[3]; g.sBk = strides[4]; g.sBn = strides[5];
   g.sCb = strides[6]; g.sCm = strides[7]; g.sCn = strides[8];
-  g.alpha = alpha; g.beta = beta; g.flags = flags;
-  if (flags & ~3) { set_last_error("wfs_gemm_bf16: unknown flags"); return WFS_ERR_INVALID; }
+  g.alpha = alpha; g.beta = beta; g.flags = flags; g.bias = bias; g.act = act;
+  if (flags & ~7) { set_last_error("wfs_gemm_bf16: unknown flags"); return WFS_ERR_INVALID; }
+  if ((flags & 4) && !act) { set_last_error("wfs_gemm_bf16: WFS_GEMM_NO_C needs an activation output"); return WFS_ERR_INVALID; }
   if (g.batch < 1 || g.M < 1 || g.N < 1 || g.K < 1 || g.batch > 65535) { set_last_error("wfs_gemm_bf16: bad sizes"); return WFS_ERR_INVALID; }
   if (splits < 1) splits = 1;
   const long kblocks = (g.K + kBK - 1) / kBK;

@@ -30,7 +30,7 @@ import torch
 from . import _abi
 
 # operands carried as bf16 hi + lo parts: activations and gradients (weights and tables stay bf16)
-SA, SB = _abi.GEMM_SPLIT_A, _abi.GEMM_SPLIT_B
+SA, SB, NO_C = _abi.GEMM_SPLIT_A, _abi.GEMM_SPLIT_B, _abi.GEMM_NO_C
 SAB = SA | SB
 
 
@@ -133,10 +133,15 @@ class FNO3d:
             }
         return self._ws
 
-    def _gemm(self, A, B, C, batch, M, N, K, sA, sB, sC, alpha=1.0, beta=0.0, splits=1, flags=0):
+    def _gemm(self, A, B, C, batch, M, N, K, sA, sB, sC, alpha=1.0, beta=0.0, splits=1, flags=0, bias=None,
+              act=None):
+        """C = alpha A B + beta C (+ bias[m]); act = GELU(C) when given; flags SA/SB split the
+        activation operands (dropped when split_activations is off), NO_C skips storing C."""
         ws = self._workspace()
+        if not self.split_activations:
+            flags &= ~SAB
         _abi.gemm_bf16(A, B, C, batch, M, N, K, sA, sB, sC, alpha, beta, splits,
-                       ws["work"] if splits > 1 else None, flags=flags if self.split_activations else 0)
+                       ws["work"] if splits > 1 else None, flags=flags, bias=bias, act=act)
 
     # ------------------------------------------------------------------ spectral convolution
     def _spectral(self, v: torch.Tensor, l: int, S: torch.Tensor) -> None:
@@ -225,22 +230,24 @@ class FNO3d:
         saved_v: List[torch.Tensor] = []
         saved_u: List[torch.Tensor] = []
         v = torch.empty(C * P, dtype=self.dtype, device=self.device)
-        self._gemm(pr["P"], a, v, 1, C, P, self.cin, (0, self.cin, 1), (0, P, 1), (0, P, 1), flags=SB)
-        self._bias_act(v, pr["pb"], None, C, False)
+        self._gemm(pr["P"], a, v, 1, C, P, self.cin, (0, self.cin, 1), (0, P, 1), (0, P, 1), flags=SB,
+                   bias=pr["pb"])
         for l in range(self.L):
             u = torch.empty(C * P, dtype=self.dtype, device=self.device)
             self._spectral(v, l, u)
-            self._gemm(pr[f"W{l}"], v, u, 1, C, P, C, (0, C, 1), (0, P, 1), (0, P, 1), beta=1.0, flags=SB)
             last = l == self.L - 1
             vn = None if last else torch.empty(C * P, dtype=self.dtype, device=self.device)
-            self._bias_act(u, pr[f"b{l}"], vn, C, not last)
+            # u = W v + S + b and v' = GELU(u) in the GEMM's epilogue; u is stored only when
+            # the backward pass needs it (or when it is the network's last activation)
+            self._gemm(pr[f"W{l}"], v, u, 1, C, P, C, (0, C, 1), (0, P, 1), (0, P, 1), beta=1.0,
+                       flags=SB | (NO_C if (not keep and not last) else 0), bias=pr[f"b{l}"], act=vn)
             if keep:
                 saved_v.append(v)
                 saved_u.append(u)
             v = u if last else vn
         out = ws["out"]
-        self._gemm(pr["Q"], v, out, 1, self.cout, P, C, (0, C, 1), (0, P, 1), (0, P, 1), flags=SB)
-        self._bias_act(out, pr["qb"], None, self.cout, False)
+        self._gemm(pr["Q"], v, out, 1, self.cout, P, C, (0, C, 1), (0, P, 1), (0, P, 1), flags=SB,
+                   bias=pr["qb"])
         if self.residual:
             # out = a[0:2] + rscale * out elementwise (the GEMM's inputs would round a to bf16)
             self._axpby(out, a[: 2 * P], 1.0, self.rscale)
