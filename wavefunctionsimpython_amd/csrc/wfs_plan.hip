@@ -464,7 +464,8 @@ extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, 
 }
 
 extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
-  if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > 2) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
+  const int vmax = kc == WFS_K_HORNER ? 3 : 2;
+  if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > vmax) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
   p->variant[kc] = variant;
   return 0;
 }
