@@ -29,6 +29,9 @@ GEMM_CASES = {
     "split_both_splitk": (1, 33, 40, 5000, "mk", "nk", "mn", 16, 0.5, 2.0, 3),
     "vector_loads": (2, 200, 96, 256, "mk", "kn", "mn", 1, 1.0, 0.0, 3),
     "vector_loads_splitk": (1, 64, 128, 4096, "mk", "kn", "mn", 8, 1.0, 1.0, 0),
+    "short_wide": (2, 20, 300, 40, "mk", "kn", "mn", 1, 1.0, 0.5, 2),
+    "short_wide_vector": (1, 32, 1024, 32, "mk", "kn", "mn", 1, 1.0, 1.0, 2),
+    "small_tile": (3, 24, 12, 256, "mk", "kn", "mn", 1, 1.0, 0.0, 2),
 }
 
 

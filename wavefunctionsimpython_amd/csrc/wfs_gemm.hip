@@ -82,8 +82,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
   const int wm = wave % WM, wn = wave / WM;
   const int b = blockIdx.z / g.splits;
   const int split = blockIdx.z % g.splits;
-  const long m0 = long(blockIdx.x) * BM;
-  const long n0 = long(blockIdx.y) * BN;
+  const long n0 = long(blockIdx.x) * BN;   // x: column tiles (the long dimension of the network's GEMMs)
+  const long m0 = long(blockIdx.y) * BM;
   const long kbeg = long(split) * g.kchunk;
   long kend = kbeg + g.kchunk;
   if (kend > g.K) kend = g.K;
@@ -265,14 +265,22 @@ extern "C" int wfs_gemm_bf16(const float* A, const float* B, float* C, const lon
   }
   if (long(g.batch) * splits > 65535) { set_last_error("wfs_gemm_bf16: batch*splits too large"); return WFS_ERR_INVALID; }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  // tall-skinny outputs use 128 x 32 tiles, the rest 64 x 64
-  if (g.N <= 32) {
-    dim3 grid(unsigned((g.M + 127) / 128), unsigned((g.N + 31) / 32), unsigned(g.batch * splits));
-    launch_tile<4, 1>(grid, s, g);
-  } else {
-    dim3 grid(unsigned((g.M + 63) / 64), unsigned((g.N + 63) / 64), unsigned(g.batch * splits));
-    launch_tile<2, 2>(grid, s, g);
-  }
+  // tile shape by output shape: 32-row outputs use one wave row (32 x 128 tiles when wide, one
+  // 32 x 32 wave when small and the k range per block is short), tall-skinny outputs and long-k
+  // small outputs 128 x 32 (four waves share the staging), the rest 64 x 64
+  int wm, wn;
+  if (g.M <= 32 && g.N > 32) { wm = 1; wn = 4; }
+  else if (g.M <= 32 && g.kchunk <= 256) { wm = 1; wn = 1; }
+  else if (g.N <= 32) { wm = 4; wn = 1; }
+  else { wm = 2; wn = 2; }
+  const long gy = (g.M + 32 * wm - 1) / (32 * wm);
+  const long gx = (g.N + 32 * wn - 1) / (32 * wn);
+  if (gy > 65535 || gx > 0x7fffffffL) { set_last_error("wfs_gemm_bf16: output too large for the launch grid"); return WFS_ERR_INVALID; }
+  dim3 grid(unsigned(gx), unsigned(gy), unsigned(g.batch * splits));
+  if (wm == 1 && wn == 1) launch_tile<1, 1>(grid, s, g);
+  else if (wm == 1) launch_tile<1, 4>(grid, s, g);
+  else if (wm == 4) launch_tile<4, 1>(grid, s, g);
+  else launch_tile<2, 2>(grid, s, g);
   WFS_HIP_CHECK(hipGetLastError());
   if (splits > 1) {
     const long total = long(g.batch) * g.M * g.N;
