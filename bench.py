@@ -296,6 +296,20 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
                      "converged": res.converged, "fallback": res.fallback,
                      "rel_l2_vs_fp64": rel(res.state), "history": res.history,
                      "phases_s": {k: v for k, v in res.timings.items() if k != "total_s"}}
+    # the network alone: inference forward time, matrix-core and HBM rates
+    a = torch.zeros(net.cin * net.P, dtype=torch.float32, device=V.device)
+    for _ in range(3):
+        net.forward(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        net.forward(a)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / 10
+    cost = net.forward_cost()
+    fno_perf = {"forward_ms": t_fwd * 1e3, "mfma_tflops": cost["flops"] / t_fwd / 1e12,
+                "mfma_peak_tflops_bf16": 2500.0, "hbm_GBs": cost["hbm_bytes"] / t_fwd / 1e9,
+                "hbm_bytes_per_forward": cost["hbm_bytes"], "flops_per_forward": cost["flops"]}
     k = max(1, len(losses) // 10)
     return {
         "metric": "time-to-T at 1e-4 relative L2, FNO-parareal vs plain (128^3 lattice, 32 slices x 50 steps)",
@@ -311,6 +325,7 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
                   "grid_pt_steps_per_s": n ** 3 * S * NS / t_plain},
         "speedup_fno_parareal_vs_plain": t_plain / out["parareal_fno"]["time_to_T_s"],
         **out,
+        "fno_forward": fno_perf,
         "training": {"pairs": len(losses), "time_s": t_train, "loss_first": float(np.mean(losses[:k])),
                      "loss_last": float(np.mean(losses[-k:])),
                      "loss_note": "relative to the one-step split-step's own error (1 = no gain)"},

@@ -292,6 +292,27 @@ class FNO3d:
         self._gemm(dv, self.ones, gr["pb"], 1, C, 1, P, (0, P, 1), (0, 0, 0), (0, 1, 0), beta=1.0, splits=sp, flags=SA)
         self._saved = None
 
+    def forward_cost(self) -> Dict[str, float]:
+        """Matrix-core flops (split operands counted twice, as issued) and minimal HBM bytes of
+        one inference forward pass (activations FP32, read or written once per GEMM)."""
+        C, n, m, P, L = self.C, self.n, self.m, self.P, self.L
+        tm = 2 * m
+        per_layer = {
+            "z": (C * n * n, tm, n, 2), "y": (C * n * tm, m, 2 * n, 2), "x": (C * tm, m * m, 2 * n, 2),
+            "modes": (m ** 3 * C, 1, C, 4 * 2), "ix": (C * 2 * n, m * m, tm, 2), "iy": (C * n * 2 * n, m, tm, 2),
+            "iz": (C * n * n, n, tm, 2), "w": (C, P, C, 2),
+        }
+        flops = 0.0
+        for M, N, K, products in per_layer.values():   # products: MFMA products per GEMM (split = 2)
+            flops += 2.0 * M * N * K * products
+        flops *= L
+        flops += 2.0 * 2 * (C * P * self.cin + self.cout * P * C)
+        act = 4.0 * C * P
+        # z-stage reads v; inverse z writes S; W-GEMM reads v and S and writes GELU(u) (u too at
+        # the last layer); the small truncated spectra are negligible
+        bytes_ = L * 5 * act + 4.0 * P * (self.cin + C) + 4.0 * P * (C + self.cout)
+        return {"flops": flops, "hbm_bytes": bytes_}
+
     def zero_grad(self) -> None:
         for g in self.grads.values():
             g.zero_()
