@@ -267,11 +267,12 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
     test = pr.random_lattice_problem(np.random.default_rng(seed_test), n=n, dx=dx, k0=0.5, r0=3.0)
     V = test.potential()
     psi0 = test.packet()
-    par = pr.Parareal(test.shape, (dx,) * 3, test.dt, 1.0, V, S, NS, autotune=True)
+    par = pr.Parareal(test.shape, (dx,) * 3, test.dt, 1.0, V, S, NS, autotune=True, axes=test.axes())
     # FP64 reference: the complex128 GPU propagator
     p64 = _abi.Plan(test.shape, (dx,) * 3, test.dt, 1.0, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+    p64.set_potential_axes(*test.axes())
     psi64 = psi0.to(torch.complex128)
-    p64.step(psi64, V, S * NS)
+    p64.step(psi64, None, S * NS)
 
     def rel(a):
         a = a.to(torch.complex128)
@@ -286,7 +287,7 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     t_plain = min(times)
-    base = pr.SplitStepCoarse(test.shape, (dx,) * 3, S * test.dt, 1.0, V, 1)
+    base = pr.SplitStepCoarse(test.shape, (dx,) * 3, S * test.dt, 1.0, V, 1, axes=test.axes())
     fno = pr.FNOCoarse(net, V, pscale, vscale, base=base)
     out = {}
     for name, c in (("parareal_splitstep", base), ("parareal_fno", fno)):

@@ -103,6 +103,17 @@ int wfs_step(wfs_plan* plan, void* psi, const void* V, int nsteps, void* stream)
 int wfs_set_potential(wfs_plan* plan, const void* V, void* stream);
 
 /*
+ * Separable real potential V(x, y, z) = fx(x) + fy(y) + fz(z) (no reference
+ * counterpart; harmonic wells and cos^2 lattices are of this form).  fx, fy,
+ * fz are host arrays of batch * n[d] doubles (Hartree, packet-major).  The plan
+ * keeps the per-axis factors exp(-i dt f_d) instead of an N^3 phase table, so
+ * the potential pass reads 16 instead of 24 B per point.  Split-step plans
+ * only; replaced by the next wfs_set_potential or wfs_step with V != NULL.
+ * Synchronous (host upload).
+ */
+int wfs_set_potential_axes(wfs_plan* plan, const double* fx, const double* fy, const double* fz);
+
+/*
  * Unitary 3-D DFT in place: direction -1 = forward (cupy.fft.fftn(norm="ortho"),
  * sources/sim_state.py:646,677), +1 = inverse (ifftn, :665,696).
  */

@@ -15,6 +15,7 @@ import ctypes
 import os
 import threading
 
+import numpy as np
 import torch
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libwfsim.so")
@@ -79,6 +80,7 @@ SIGNATURES = {
     "wfs_potential_operator": (ctypes.c_int, [ctypes.c_long, _vp, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "wfs_plan_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "wfs_set_potential": (ctypes.c_int, [_vp, _vp, _vp]),
+    "wfs_set_potential_axes": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "wfs_plan_autotune": (ctypes.c_int, [_vp, _vp]),
     "wfs_plan_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "wfs_plan_get_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -233,6 +235,21 @@ class Plan:
         check(lib().wfs_set_potential(self._h, ptr(V), stream_handle(stream)), "wfs_set_potential")
         self._potential_set = True
         self._V_ref = V if self.method == WFS_METHOD_POWER_SERIES else None
+
+    def set_potential_axes(self, fx, fy, fz) -> None:
+        """Separable real potential V = fx(x) + fy(y) + fz(z): host arrays of shape (n_d,) or
+        (batch, n_d), float64 Hartree.  Synchronises with the device (host upload)."""
+        arrs = []
+        for d, f in enumerate((fx, fy, fz)):
+            a = np.ascontiguousarray(np.asarray(f, dtype=np.float64)).reshape(-1)
+            if a.size != self.batch * self.n[d]:
+                raise ValueError(f"axis {d}: expected {self.batch * self.n[d]} values, got {a.size}")
+            arrs.append(a)
+        torch.cuda.synchronize()
+        check(lib().wfs_set_potential_axes(self._h, *(a.ctypes.data_as(ctypes.c_void_p) for a in arrs)),
+              "wfs_set_potential_axes")
+        self._potential_set = True
+        self._V_ref = None
 
     def autotune(self, stream=None) -> dict:
         check(lib().wfs_plan_autotune(self._h, stream_handle(stream)), "wfs_plan_autotune")
@@ -441,6 +458,22 @@ def add_harmonic(n, dx, omega, mass, center, V: torch.Tensor, stream=None) -> No
     check_potential(V, n)
     check(lib().wfs_add_harmonic(i3(n), d3(dx), float(omega), float(mass), d3(center), ptr(V),
                                  stream_handle(stream)), "wfs_add_harmonic")
+
+
+def separable_axes(n, dx, omega: float = 0.0, mass: float = 1.0, center=(0.0, 0.0, 0.0), depth: float = 0.0,
+                   period: float = 1.0):
+    """Per-axis parts f_d of V = sum_d f_d(r_d) for the harmonic well 1/2 m omega^2 |r - c|^2 plus
+    the lattice depth * sum_d cos^2(pi r_d / period), r_d(i) = dx_d (i - n_d / 2) — the same
+    terms, in FP64, that add_harmonic and add_lattice add point by point."""
+    out = []
+    k = 0.5 * mass * omega * omega
+    for d in range(3):
+        r = float(dx[d]) * (np.arange(int(n[d]), dtype=np.float64) - int(n[d]) // 2)
+        f = k * (r - float(center[d])) ** 2
+        if depth:
+            f = f + depth * np.cos(np.pi * r / period) ** 2
+        out.append(f)
+    return tuple(out)
 
 
 def add_lattice(n, dx, depth, period, V: torch.Tensor, stream=None) -> None:

@@ -126,10 +126,26 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
     // pv holds cexp_i(-dt V) = (cos(-dt Vr), sin(-dt Vr)) exp(dt Vi) per point
     // (sources/cuda_kernels/common.cu:98-111, potential_operator.cu:10-15).
     transform(std::integral_constant<int, +1>{});
-    const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + long(blockIdx.x) * G * N;
-    if (active) {
+    if (p.spx) {
+      // separable potential: exp(-i dt (fx + fy + fz)) = px(x) py(y) pz(z), no N^3 table read
+      const C* __restrict__ px = reinterpret_cast<const C*>(p.spx);
+      const C* __restrict__ py = reinterpret_cast<const C*>(p.spy);
+      const C* __restrict__ pz = reinterpret_cast<const C*>(p.spz);
+      const long row = active ? col : 0;
+      const long plane = p.snx * p.sny;
+      const long pk = row / plane, r = row % plane;
+      const C pxy = cmul(px[pk * p.snx + r / p.sny], py[pk * p.sny + r % p.sny]);
+      const C* __restrict__ pzb = pz + pk * N;
+      if (active) {
 #pragma unroll
-      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], pvb[loff + unsigned(i) * step]);
+        for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], cmul(pzb[t + N2 * i], pxy));
+      }
+    } else {
+      const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + long(blockIdx.x) * G * N;
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], pvb[loff + unsigned(i) * step]);
+      }
     }
     __syncthreads();
     transform(std::integral_constant<int, -1>{});

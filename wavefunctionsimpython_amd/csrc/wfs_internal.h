@@ -19,6 +19,12 @@ struct PassParams {
   const void* kz = nullptr;
   long nz = 1;                  // FKI: column index -> (ky, kz) = (c / nz, c % nz)
   const void* pv = nullptr;     // IVF: potential phase table cexp_i(-dt V), state precision, state layout
+  // IVF with a separable real potential V = fx(x) + fy(y) + fz(z): per-axis phase factors
+  // exp(-i dt f_d), [packet][n_d], state precision (used instead of pv when spx is set)
+  const void* spx = nullptr;
+  const void* spy = nullptr;
+  const void* spz = nullptr;
+  long snx = 0, sny = 0;        // grid lengths along x and y (row -> packet, x, y)
   double scale = 1.0;           // FWD/INV: output scale (1 = none)
 };
 

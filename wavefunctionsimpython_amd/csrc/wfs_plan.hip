@@ -26,6 +26,8 @@ struct wfs_plan {
   double* plane_out;      // 7 doubles
   void* pv;               // potential phase table cexp_i(-dt V), state precision
   bool pv_set;
+  void* sep[3];           // separable potential: per-axis exp(-i dt f_d), [ntab][batch][n_d]
+  bool sep_set;           // the step uses sep[] instead of pv
   int variant[WFS_K_COUNT];   // tile variant per kernel class (wfs_plan_autotune)
   // per-kernel timing (wfs_plan_set_profiling)
   bool profiling;
@@ -220,7 +222,19 @@ int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* pv, const 
     pp.kz = table_copy(p, ktab[2], 2, sel);
     pp.nz = p->n[2];
   }
-  if (mode == MODE_IVF) pp.pv = pv;
+  if (mode == MODE_IVF) {
+    if (p->sep_set) {
+      const size_t per = size_t(p->batch) * p->elem;
+      const int c = sel % p->ntab;
+      pp.spx = static_cast<const char*>(p->sep[0]) + size_t(c) * per * p->n[0];
+      pp.spy = static_cast<const char*>(p->sep[1]) + size_t(c) * per * p->n[1];
+      pp.spz = static_cast<const char*>(p->sep[2]) + size_t(c) * per * p->n[2];
+      pp.snx = p->n[0];
+      pp.sny = p->n[1];
+    } else {
+      pp.pv = pv;
+    }
+  }
   const int kc = kernel_class_of(axis, mode);
   ProfileScope scope(p, kc, s);
   return launch_fft_pass(p->precision, p->n[axis], layout, mode, data, pp, outer, p->variant[kc], s);
@@ -234,6 +248,29 @@ int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
     if (e != hipSuccess) { p->pv = nullptr; set_last_error("hipMalloc for the potential phase table failed"); return WFS_ERR_ALLOC; }
   }
   WFS_TRY(launch_potential_phase(p->precision, long(grid_points(p)), static_cast<const float2*>(V), p->dt, p->pv, s));
+  p->pv_set = true;
+  p->sep_set = false;
+  return 0;
+}
+
+// per-axis phase tables exp(-i dt f_d) of a separable real potential (host, long double phase
+// reduced mod 2 pi; dithered copies for complex64 plans)
+int set_potential_axes(wfs_plan* p, const double* const f[3]) {
+  const long double two_pi = 2.0L * 3.14159265358979323846264338327950288L;
+  for (int a = 0; a < 3; ++a) {
+    const long len = long(p->batch) * p->n[a];
+    std::vector<double> re(len), im(len);
+    for (long j = 0; j < len; ++j) {
+      if (!std::isfinite(f[a][j])) { set_last_error("wfs_set_potential_axes: non-finite potential"); return WFS_ERR_INVALID; }
+      long double th = -(long double)p->dt * (long double)f[a][j];
+      th = fmodl(th, two_pi);
+      re[j] = double(cosl(th));
+      im[j] = double(sinl(th));
+    }
+    if (p->sep[a]) { (void)hipFree(p->sep[a]); p->sep[a] = nullptr; }
+    WFS_TRY(upload_dithered(&p->sep[a], re, im, p->precision));
+  }
+  p->sep_set = true;
   p->pv_set = true;
   return 0;
 }
@@ -311,6 +348,7 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
     if (p->tw[a]) (void)hipFree(p->tw[a]);
     if (p->kfull[a]) (void)hipFree(p->kfull[a]);
     if (p->khalf[a]) (void)hipFree(p->khalf[a]);
+    if (p->sep[a]) (void)hipFree(p->sep[a]);
   }
   for (int i = 0; i < 2; ++i)
     if (p->scratch[i]) (void)hipFree(p->scratch[i]);
@@ -395,6 +433,13 @@ extern "C" int wfs_set_potential(wfs_plan* p, const void* V, void* stream) {
   if (!p || !V) { set_last_error("wfs_set_potential: null argument"); return WFS_ERR_INVALID; }
   if (p->method != WFS_METHOD_FOURIER) return 0;   // the power series reads V directly
   return set_potential(p, V, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int wfs_set_potential_axes(wfs_plan* p, const double* fx, const double* fy, const double* fz) {
+  if (!p || !fx || !fy || !fz) { set_last_error("wfs_set_potential_axes: null argument"); return WFS_ERR_INVALID; }
+  if (p->method != WFS_METHOD_FOURIER) { set_last_error("wfs_set_potential_axes: split-step plans only"); return WFS_ERR_UNSUPPORTED; }
+  const double* const f[3] = {fx, fy, fz};
+  return set_potential_axes(p, f);
 }
 
 extern "C" int wfs_step(wfs_plan* p, void* psi, const void* V, int nsteps, void* stream) {

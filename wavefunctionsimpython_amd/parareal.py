@@ -71,6 +71,11 @@ class LatticeProblem:
             _abi.add_lattice(self.shape, (self.dx,) * 3, self.depth, self.period, V)
         return V
 
+    def axes(self):
+        """Per-axis parts (fx, fy, fz) of the potential, which is separable (FP64 host arrays)."""
+        return _abi.separable_axes(self.shape, (self.dx,) * 3, self.omega, self.mass, (0.0, 0.0, 0.0), self.depth,
+                                   self.period)
+
     def packet(self, device=None) -> torch.Tensor:
         psi = torch.empty(self.shape, dtype=C64, device=device or _cuda())
         _abi.init_gaussian(self.shape, (self.dx,) * 3, _abi.WFS_C64, (self.sigma,) * 3, self.r0, self.k0, psi)
@@ -104,11 +109,15 @@ class SplitStepCoarse:
 
     name = "splitstep"
 
-    def __init__(self, n, dx, slice_time: float, mass: float, V: torch.Tensor, steps: int, batch: int = 1):
+    def __init__(self, n, dx, slice_time: float, mass: float, V: torch.Tensor, steps: int, batch: int = 1,
+                 axes=None):
         self.steps = int(steps)
         self.plan = _abi.Plan(n, dx, slice_time / self.steps, mass, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER,
                               batch=batch)
-        self.plan.set_potential(V)
+        if axes is not None:
+            self.plan.set_potential_axes(*axes)
+        else:
+            self.plan.set_potential(V)
 
     def __call__(self, src: torch.Tensor, dst: torch.Tensor) -> None:
         if dst.data_ptr() != src.data_ptr():
@@ -185,7 +194,7 @@ class Parareal:
     """N slices of ``steps_per_slice`` fine steps on one grid and potential."""
 
     def __init__(self, n, dx, dt: float, mass: float, V: torch.Tensor, steps_per_slice: int, slices: int,
-                 autotune: bool = False):
+                 autotune: bool = False, axes=None):
         self.n = tuple(int(v) for v in n)
         self.dx = tuple(float(v) for v in dx)
         self.dt, self.mass = float(dt), float(mass)
@@ -197,9 +206,14 @@ class Parareal:
         self.V = V
         self.fine = _abi.Plan(self.n, self.dx, self.dt, self.mass, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER,
                               batch=self.N)
-        self.fine.set_potential(V.expand((self.N,) + self.n).contiguous())
         self.serial = _abi.Plan(self.n, self.dx, self.dt, self.mass, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER)
-        self.serial.set_potential(V)
+        if axes is not None:
+            # separable potential: per-axis phase factors, one copy per slice
+            self.fine.set_potential_axes(*(np.tile(np.asarray(a, dtype=np.float64), (self.N, 1)) for a in axes))
+            self.serial.set_potential_axes(*axes)
+        else:
+            self.fine.set_potential(V.expand((self.N,) + self.n).contiguous())
+            self.serial.set_potential(V)
         if autotune:
             self.fine.autotune()
             self.serial.autotune()
@@ -338,14 +352,17 @@ class TrajectoryPairs:
         self.psi = torch.stack([p.packet(dev) for p in problems]).contiguous()
         self.prev = torch.empty_like(self.psi)
         dx = (p0.dx,) * 3
+        axes = [p.axes() for p in problems]
+        stacked = tuple(np.stack([a[d] for a in axes]) for d in range(3))
         self.fine = _abi.Plan(self.n, dx, p0.dt, p0.mass, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER, batch=self.B)
-        self.fine.set_potential(self.V)
+        self.fine.set_potential_axes(*stacked)
         if autotune:
             self.fine.autotune()
         self.base = None
         self.base_out = None
         if base_steps > 0:
-            self.base = SplitStepCoarse(self.n, dx, self.steps * p0.dt, p0.mass, self.V, base_steps, batch=self.B)
+            self.base = SplitStepCoarse(self.n, dx, self.steps * p0.dt, p0.mass, self.V, base_steps, batch=self.B,
+                                        axes=stacked)
             self.base_out = torch.empty_like(self.psi)
 
     def advance(self) -> None:
