@@ -57,12 +57,6 @@ class EmulatedFNO(FNO3d):
         if not flags & 4:
             Cv.copy_(r)
 
-    def _bias_act(self, u, bias, v, rows, act):
-        uu = u.view(rows, self.P)
-        uu += bias[:, None]
-        if v is not None:
-            v.view(rows, self.P).copy_(fno_oracle.gelu_tanh(uu) if act else uu)
-
     def _gelu_backward(self, u, dv, du):
         ut = u.detach().clone().requires_grad_(True)
         (g,) = torch.autograd.grad(fno_oracle.gelu_tanh(ut), ut, dv)

@@ -197,3 +197,53 @@ def test_fno_forward_and_gradients(dev, n, width, modes, layers):
         print(f"  grad {k:4s}: vs FP64 autograd {eo:.2e}, vs emulation {ee:.2e}")
         assert eo < 3e-2, k
         assert ee < 2e-4, k
+
+
+def test_elementwise_kernels_match_torch(dev):
+    """bias + GELU, GELU', axpby, relative MSE and Adam against float32/float64 torch."""
+    lib, s = _abi.lib(), _abi.stream_handle()
+    g = torch.Generator(device=dev).manual_seed(3)
+    C, P = 5, 1000
+    u = torch.randn(C * P, device=dev, generator=g)
+    bias = torch.randn(C, device=dev, generator=g)
+    v = torch.empty_like(u)
+    ref_u = (u.view(C, P) + bias[:, None]).reshape(-1)
+    _abi.check(lib.wfs_fno_bias_act(_abi.ptr(u), _abi.ptr(bias), _abi.ptr(v), C, P, 1, s), "bias_act")
+    torch.cuda.synchronize()
+    assert torch.allclose(u, ref_u, atol=0, rtol=0)
+    assert torch.allclose(v, torch.nn.functional.gelu(ref_u, approximate="tanh"), atol=1e-6, rtol=1e-5)
+    dv = torch.randn_like(u)
+    du = torch.empty_like(u)
+    _abi.check(lib.wfs_fno_gelu_backward(_abi.ptr(ref_u), _abi.ptr(dv), _abi.ptr(du), u.numel(), s), "gelu_bw")
+    x = ref_u.double().requires_grad_(True)
+    (gref,) = torch.autograd.grad(torch.nn.functional.gelu(x, approximate="tanh"), x, dv.double())
+    torch.cuda.synchronize()
+    assert torch.allclose(du.double(), gref, atol=1e-5, rtol=1e-5)
+    y = torch.randn_like(u)
+    y0 = y.clone()
+    _abi.check(lib.wfs_fno_axpby(_abi.ptr(y), _abi.ptr(dv), 0.5, -2.0, u.numel(), s), "axpby")
+    torch.cuda.synchronize()
+    assert torch.allclose(y, 0.5 * dv - 2.0 * y0, atol=1e-6)
+    pred, targ = torch.randn_like(u), torch.randn_like(u)
+    dpred = torch.empty_like(u)
+    work = torch.empty(8192, dtype=torch.float64, device=dev)
+    out2 = torch.empty(2, dtype=torch.float64, device=dev)
+    _abi.check(lib.wfs_fno_mse(_abi.ptr(pred), _abi.ptr(targ), _abi.ptr(dpred), 0.25, u.numel(), _abi.ptr(work),
+                               8192, _abi.ptr(out2), s), "mse")
+    torch.cuda.synchronize()
+    d = pred.double() - targ.double()
+    assert abs(float(out2[0]) - float((d * d).sum())) < 1e-9 * float((d * d).sum())
+    assert abs(float(out2[1]) - float((targ.double() ** 2).sum())) < 1e-9 * float((targ.double() ** 2).sum())
+    assert torch.allclose(dpred.double(), 2 * d * 0.25, atol=1e-6)
+    w = torch.randn(100, device=dev, generator=g)
+    grad = torch.randn_like(w)
+    m, vv = torch.zeros_like(w), torch.zeros_like(w)
+    opt_w = w.detach().clone().requires_grad_(True)
+    opt = torch.optim.Adam([opt_w], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    for step in (1, 2):
+        _abi.check(lib.wfs_fno_adam(_abi.ptr(w), _abi.ptr(grad), _abi.ptr(m), _abi.ptr(vv), w.numel(), 1e-2, 0.9,
+                                    0.999, 1e-8, step, s), "adam")
+        opt_w.grad = grad.clone()
+        opt.step()
+    torch.cuda.synchronize()
+    assert torch.allclose(w, opt_w.detach(), atol=1e-6, rtol=1e-5)

@@ -204,12 +204,6 @@ class FNO3d:
 
     # ------------------------------------------------------------------ forward / backward
     # ------------------------------------------------------------------ elementwise kernels
-    def _bias_act(self, u: torch.Tensor, bias: torch.Tensor, v: torch.Tensor | None, rows: int, act: bool) -> None:
-        """u[c] += bias[c]; v = GELU(u) when act else v = u (v may be None)."""
-        _abi.check(_abi.lib().wfs_fno_bias_act(_abi.ptr(u), _abi.ptr(bias), _abi.ptr(v) if v is not None else None,
-                                               rows, self.P, 1 if act else 0, _abi.stream_handle()),
-                   "wfs_fno_bias_act")
-
     def _gelu_backward(self, u: torch.Tensor, dv: torch.Tensor, du: torch.Tensor) -> None:
         _abi.check(_abi.lib().wfs_fno_gelu_backward(_abi.ptr(u), _abi.ptr(dv), _abi.ptr(du), u.numel(),
                                                     _abi.stream_handle()), "wfs_fno_gelu_backward")
