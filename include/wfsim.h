@@ -213,7 +213,8 @@ int wfs_potential_operator(long count, const void* V, double dt, int precision, 
 #define WFS_K_X_INV 7
 #define WFS_K_HORNER 8
 #define WFS_K_OBSERVABLES 9
-#define WFS_K_COUNT 10
+#define WFS_K_YZ_PLANE 10
+#define WFS_K_COUNT 11
 int wfs_plan_set_profiling(wfs_plan* plan, int enable);
 
 /*
@@ -224,6 +225,11 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * WFS_K_HORNER: variant 0 runs one launch per power-series stage, 1, 2
  * (default) and 3 run the stages in fused pairs on 16 x 64, 8 x 64 and
  * 8 x 128 (y, z) tiles; the results are bit-identical.
+ * WFS_K_YZ_PLANE: variant 1 (default where supported: complex64 planes of
+ * ny = nz <= 128, complex128 planes of ny = nz <= 64) runs the y inverse, z
+ * inverse / potential / z forward and y forward passes of each step as one
+ * kernel on whole x planes held in LDS; variant 0 runs the three passes.  The
+ * results are identical; the autotuner keeps the faster.
  */
 int wfs_plan_autotune(wfs_plan* plan, void* stream);
 int wfs_plan_set_variant(wfs_plan* plan, int kernel_class, int variant);

@@ -90,7 +90,7 @@ SIGNATURES = {
 
 # kernel classes of wfs_plan_profile
 KERNEL_CLASSES = ("z_fwd", "z_inv", "z_inv_v_fwd", "y_fwd", "y_inv", "x_fwd_k_inv", "x_fwd", "x_inv",
-                  "horner_stage", "observables")
+                  "horner_stage", "observables", "yz_plane")
 
 
 class WfsError(RuntimeError):
@@ -253,6 +253,10 @@ class Plan:
 
     def autotune(self, stream=None) -> dict:
         check(lib().wfs_plan_autotune(self._h, stream_handle(stream)), "wfs_plan_autotune")
+        return {name: lib().wfs_plan_get_variant(self._h, k) for k, name in enumerate(KERNEL_CLASSES)}
+
+    def variants(self) -> dict:
+        """{kernel class: variant} in use."""
         return {name: lib().wfs_plan_get_variant(self._h, k) for k, name in enumerate(KERNEL_CLASSES)}
 
     def set_variant(self, kernel_class: str, variant: int) -> None:

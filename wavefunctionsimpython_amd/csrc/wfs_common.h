@@ -24,19 +24,25 @@ template <typename T>
 __host__ __device__ __forceinline__ T cadd(T a, T b) { a.x += b.x; a.y += b.y; return a; }
 template <typename T>
 __host__ __device__ __forceinline__ T csub(T a, T b) { a.x -= b.x; a.y -= b.y; return a; }
+// Complex products with the fused multiply-adds written out: which of the two
+// products a contracting compiler would fuse depends on the surrounding code, so a
+// kernel that repeats another's arithmetic could round differently.
+__host__ __device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __host__ __device__ __forceinline__ T cmul(T a, T b) {
   T r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
+  r.x = fma_r(a.x, b.x, -(a.y * b.y));
+  r.y = fma_r(a.x, b.y, a.y * b.x);
   return r;
 }
 // a * conj(b)
 template <typename T>
 __host__ __device__ __forceinline__ T cmulc(T a, T b) {
   T r;
-  r.x = a.x * b.x + a.y * b.y;
-  r.y = a.y * b.x - a.x * b.y;
+  r.x = fma_r(a.x, b.x, a.y * b.y);
+  r.y = fma_r(a.y, b.x, -(a.x * b.y));
   return r;
 }
 template <typename T, typename S>
@@ -51,16 +57,16 @@ __host__ __device__ __forceinline__ T cscale(T a, S s) { a.x *= s; a.y *= s; ret
 // so they are added before the main products are rounded (added afterwards they would
 // mostly vanish below half an ulp of the rounded product).
 __host__ __device__ __forceinline__ float2 cmul2(float2 a, float2 hi, float2 lo) {
-  const float cx = a.x * lo.x - a.y * lo.y;
-  const float cy = a.x * lo.y + a.y * lo.x;
+  const float cx = __builtin_fmaf(a.x, lo.x, -(a.y * lo.y));
+  const float cy = __builtin_fmaf(a.x, lo.y, a.y * lo.x);
   float2 r;
   r.x = __builtin_fmaf(a.x, hi.x, __builtin_fmaf(-a.y, hi.y, cx));
   r.y = __builtin_fmaf(a.x, hi.y, __builtin_fmaf(a.y, hi.x, cy));
   return r;
 }
 __host__ __device__ __forceinline__ float2 cmulc2(float2 a, float2 hi, float2 lo) {
-  const float cx = a.x * lo.x + a.y * lo.y;
-  const float cy = a.y * lo.x - a.x * lo.y;
+  const float cx = __builtin_fmaf(a.x, lo.x, a.y * lo.y);
+  const float cy = __builtin_fmaf(a.y, lo.x, -(a.x * lo.y));
   float2 r;
   r.x = __builtin_fmaf(a.x, hi.x, __builtin_fmaf(a.y, hi.y, cx));
   r.y = __builtin_fmaf(a.y, hi.x, __builtin_fmaf(-a.x, hi.y, cy));
@@ -250,6 +256,37 @@ struct DFT {
     });
   }
 };
+
+// ---------------------------------------------------------------------------
+// One length-N transform of a column shared by N2 threads, N = N1 * N2, N1 % N2 == 0:
+//   X[k1 + N1 k2] = sum_n2 W_N2^{n2 k2} W_N^{n2 k1} sum_n1 x[N2 n1 + n2] W_N1^{n1 k1}
+// Thread t (0 <= t < N2) holds a[i] = x[t + N2 i] on entry and a[i] = X[t + N2 i] on
+// exit; w[k1] = W_N^{t k1} (forward twiddles; the inverse uses their conjugates).
+// lidx(k1, n2) names the LDS slot of element (k1, n2) of this column.  The routine
+// synchronises the block once between its LDS writes and reads; the caller
+// synchronises before the same slots are written again.
+// ---------------------------------------------------------------------------
+template <int N, int N2, int SIGN, typename C, typename LIDX>
+__device__ __forceinline__ void column_dft(C (&a)[N / N2], const C (&w)[N / N2], C* lds, int t, LIDX lidx) {
+  constexpr int N1 = N / N2;
+  constexpr int M = N1 / N2;            // length-N2 DFTs per thread in the second step
+  static_assert(N1 * N2 == N && N1 % N2 == 0, "bad split");
+  DFT<N1, SIGN>::run(a);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) a[k1] = (SIGN < 0) ? cmul(a[k1], w[k1]) : cmulc(a[k1], w[k1]);
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) lds[lidx(k1, t)] = a[k1];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    C b[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) b[n2] = lds[lidx(t + N2 * m, n2)];
+    DFT<N2, SIGN>::run(b);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) a[m + M * k2] = b[k2];
+  }
+}
 
 // GELU, tanh form: 0.5 u (1 + tanh(sqrt(2/pi) (u + 0.044715 u^3))), and its derivative
 __device__ __forceinline__ float gelu_f(float u) {

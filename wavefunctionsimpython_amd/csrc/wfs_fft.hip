@@ -24,7 +24,6 @@ template <typename R, int N, int N2, int LAYOUT, int MODE, int G>
 __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ data, const PassParams p) {
   constexpr int N1 = N / N2;
   static_assert(N1 * N2 == N && N1 % N2 == 0, "bad split");
-  constexpr int M = N1 / N2;            // length-N2 DFTs per thread in step 2
   using C = cx<R>;
   // LDS exchange buffer: element (k1, n2) of group g
   constexpr int ROW_STRIDE = N1 * (N2 + 1);   // padded: conflict-free step-2 reads
@@ -84,22 +83,7 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
 
   // one length-N transform of the column held in a[] (SIGN -1 forward, +1 inverse)
   auto transform = [&](auto sign_c) {
-    constexpr int SIGN = decltype(sign_c)::value;
-    DFT<N1, SIGN>::run(a);
-#pragma unroll
-    for (int k1 = 1; k1 < N1; ++k1) a[k1] = (SIGN < 0) ? cmul(a[k1], w[k1]) : cmulc(a[k1], w[k1]);
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) lds[lidx(k1, t)] = a[k1];
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      C b[N2];
-#pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) b[n2] = lds[lidx(t + N2 * m, n2)];
-      DFT<N2, SIGN>::run(b);
-#pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) a[m + M * k2] = b[k2];
-    }
+    column_dft<N, N2, decltype(sign_c)::value>(a, w, lds, t, lidx);
   };
 
   if constexpr (MODE == MODE_FWD || MODE == MODE_INV) {
@@ -171,6 +155,154 @@ template <> struct split_of<128> { static constexpr int n2 = 8; };
 template <> struct split_of<192> { static constexpr int n2 = 8; };
 template <> struct split_of<256> { static constexpr int n2 = 16; };
 template <> struct split_of<512> { static constexpr int n2 = 16; };
+
+// ---------------------------------------------------------------------------
+// Fused y-z pass on whole x planes (ny = nz = N).  One block holds one plane,
+// N*N points, in registers (N1 per thread) and moves it through LDS:
+//   y inverse -> transpose -> z inverse -> potential phase -> z forward
+//   -> transpose -> y forward
+// Each transform is column_dft with the split, table twiddles and operation
+// order of the axis passes y_inv, z_inv_v_fwd and y_fwd, so the result is the
+// same; the plane crosses HBM once (16 B per point for complex64) instead of
+// three times (48 B per point).  The LDS holds the largest of the column
+// exchange (N*N), the padded row exchange (N * N1 * (N2+1)) and the padded
+// transposition (N * (N + N2)); for N = 128 complex64 that is 144 KiB, one
+// block of 1024 threads per CU.
+// ---------------------------------------------------------------------------
+constexpr int max3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+
+template <typename R, int N>
+constexpr long yz_plane_lds_bytes() {
+  constexpr int N2 = split_of<N>::n2, N1 = N / N2;
+  return long(max3(N * N, N * N1 * (N2 + 1), N * (N + N2)) + N) * long(sizeof(cx<R>));
+}
+
+// An opaque copy of v: the compiler recomputes what derives from it instead of
+// keeping values from an earlier stage alive (register pressure).
+__device__ __forceinline__ int opaque_v(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__(N * split_of<N>::n2) void yz_plane_kernel(cx<R>* __restrict__ data,
+                                                                        const YzPlaneParams p) {
+  constexpr int N2 = split_of<N>::n2;
+  constexpr int N1 = N / N2;
+  constexpr int TS = N + N2;                 // padded row stride of the transposition
+  constexpr int RS = N1 * (N2 + 1);          // padded row stride of the row exchange
+  // y-column thread index uniform over a wavefront: y twiddles by scalar loads
+  constexpr bool Y_UNIFORM = (N % 64) == 0;
+  using C = cx<R>;
+  using CC = const __attribute__((address_space(4))) C;
+  __shared__ C lds[max3(N * N, N * RS, N * TS)];
+  __shared__ C twz[N];                       // z twiddles
+
+  const int tid = threadIdx.x;
+  const long plane = blockIdx.x;
+  C* __restrict__ blk = data + plane * long(N) * N;
+  if (tid < N) twz[tid] = reinterpret_cast<const C*>(p.tw_z)[tid];
+
+  C a[N1];
+  C w[N1];
+  // y transform on columns: thread ty of column gz holds y = ty + N2 i
+  auto y_stage = [&](const void* twp, auto sign_c) {
+    constexpr int SIGN = decltype(sign_c)::value;
+    const int gz = opaque_v(tid) % N;
+    if constexpr (Y_UNIFORM) {
+      const int ty = __builtin_amdgcn_readfirstlane(tid / N);
+      CC* tw = (CC*)(twp);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = mk<R>(tw[(ty * k1) % N].x, tw[(ty * k1) % N].y);
+      column_dft<N, N2, SIGN>(a, w, lds, ty, [&](int k1, int n2) { return (k1 * N2 + n2) * N + gz; });
+    } else {
+      const int ty = opaque_v(tid) / N;
+      const C* __restrict__ tw = reinterpret_cast<const C*>(twp);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(ty * k1) % N];
+      column_dft<N, N2, SIGN>(a, w, lds, ty, [&](int k1, int n2) { return (k1 * N2 + n2) * N + gz; });
+    }
+  };
+  // transposition through LDS between the column and row layouts
+  auto cols_to_rows = [&]() {
+    const int t = opaque_v(tid);
+    const int gz = t % N, ty = t / N, tz = t % N2, gy = t / N2;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) lds[(ty + N2 * i) * TS + gz] = a[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[i] = lds[gy * TS + tz + N2 * i];
+  };
+  auto rows_to_cols = [&]() {
+    const int t = opaque_v(tid);
+    const int gz = t % N, ty = t / N, tz = t % N2, gy = t / N2;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) lds[gy * TS + tz + N2 * i] = a[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[i] = lds[(ty + N2 * i) * TS + gz];
+  };
+
+  {
+    const int gz = tid % N, ty = tid / N;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[i] = blk[unsigned(gz + (ty + N2 * i) * N)];
+  }
+
+  // y inverse
+  y_stage(p.tw_yi, std::integral_constant<int, +1>{});
+  __syncthreads();
+  cols_to_rows();
+  __syncthreads();
+
+  // z inverse, potential phase, z forward on rows: thread tz of row gy holds z = tz + N2 i
+  {
+    const int tz = tid % N2, gy = tid / N2;
+    auto lidx_r = [&](int k1, int n2) -> int { return gy * RS + k1 * (N2 + 1) + n2; };
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) w[k1] = twz[(tz * k1) % N];
+    column_dft<N, N2, +1>(a, w, lds, tz, lidx_r);
+    if (p.spx) {
+      const long b = plane / p.nx;
+      const C* __restrict__ px = reinterpret_cast<const C*>(p.spx);
+      const C* __restrict__ py = reinterpret_cast<const C*>(p.spy) + b * N;
+      const C* __restrict__ pzb = reinterpret_cast<const C*>(p.spz) + b * N;
+      const C pxy = cmul(px[plane], py[gy]);
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], cmul(pzb[tz + N2 * i], pxy));
+    } else {
+      const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + plane * long(N) * N;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], pvb[unsigned(gy * N + tz + N2 * i)]);
+    }
+    __syncthreads();
+    column_dft<N, N2, -1>(a, w, lds, tz, lidx_r);
+  }
+  __syncthreads();
+  rows_to_cols();
+  __syncthreads();
+
+  // y forward
+  y_stage(p.tw_yf, std::integral_constant<int, -1>{});
+  {
+    const int t = opaque_v(tid);
+    const int gz = t % N, ty = t / N;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) blk[unsigned(gz + (ty + N2 * i) * N)] = a[i];
+  }
+}
+
+template <typename R, int N>
+static int launch_yz_plane_n(void* data, const YzPlaneParams& p, hipStream_t s) {
+  static_assert(yz_plane_lds_bytes<R, N>() <= 160 * 1024, "plane does not fit in LDS");
+  if (p.planes <= 0 || p.planes > 0x7fffffffL) {
+    set_last_error("yz plane pass: bad plane count %ld", p.planes);
+    return WFS_ERR_INVALID;
+  }
+  hipLaunchKernelGGL((yz_plane_kernel<R, N>), dim3(unsigned(p.planes)), dim3(N * split_of<N>::n2), 0, s,
+                     reinterpret_cast<cx<R>*>(data), p);
+  return int(hipGetLastError());
+}
 
 // Columns (COLS) or rows (ROWS) per block.  The default aims at 256 threads
 // with the LDS exchange buffer within 64 KiB (two or more blocks per CU); the
@@ -256,6 +388,30 @@ int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams&
 #if WFS_FFT_PREC == 0
 template int launch_fft_pass_t<float>(int, int, int, void*, const PassParams&, long, int, hipStream_t);
 
+int launch_yz_plane_c128(int n, void* data, const YzPlaneParams& p, hipStream_t s);
+
+// planes that fit in LDS: complex64 up to 128 x 128, complex128 up to 64 x 64
+bool yz_plane_supported(int precision, int ny, int nz) {
+  if (ny != nz) return false;
+  if (precision == WFS_C128) return ny == 16 || ny == 32 || ny == 64;
+  return ny == 16 || ny == 32 || ny == 64 || ny == 96 || ny == 128;
+}
+
+int launch_yz_plane(int precision, int n, void* data, const YzPlaneParams& p, hipStream_t s) {
+  if (!yz_plane_supported(precision, n, n)) {
+    set_last_error("yz plane pass: unsupported plane %d x %d", n, n);
+    return WFS_ERR_UNSUPPORTED;
+  }
+  if (precision == WFS_C128) return launch_yz_plane_c128(n, data, p, s);
+  switch (n) {
+    case 16:  return launch_yz_plane_n<float, 16>(data, p, s);
+    case 32:  return launch_yz_plane_n<float, 32>(data, p, s);
+    case 64:  return launch_yz_plane_n<float, 64>(data, p, s);
+    case 96:  return launch_yz_plane_n<float, 96>(data, p, s);
+    default:  return launch_yz_plane_n<float, 128>(data, p, s);
+  }
+}
+
 bool fft_length_supported(int n) {
   switch (n) {
     case 8: case 16: case 32: case 64: case 96: case 128: case 192: case 256: case 512: return true;
@@ -270,6 +426,14 @@ int launch_fft_pass(int precision, int n, int layout, int mode, void* data, cons
 }
 #else
 template int launch_fft_pass_t<double>(int, int, int, void*, const PassParams&, long, int, hipStream_t);
+
+int launch_yz_plane_c128(int n, void* data, const YzPlaneParams& p, hipStream_t s) {
+  switch (n) {
+    case 16:  return launch_yz_plane_n<double, 16>(data, p, s);
+    case 32:  return launch_yz_plane_n<double, 32>(data, p, s);
+    default:  return launch_yz_plane_n<double, 64>(data, p, s);
+  }
+}
 #endif
 
 }  // namespace wfs

@@ -34,6 +34,22 @@ int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams&
                       hipStream_t s);
 int launch_fft_pass(int precision, int n, int layout, int mode, void* data, const PassParams& p, long outer,
                     int variant, hipStream_t s);
+// Fused y-z pass of the split step on whole x planes (ny == nz, plane held in LDS):
+// y inverse, z inverse, potential phase, z forward, y forward, in place.
+struct YzPlaneParams {
+  const void* tw_yi = nullptr;  // y twiddles of the inverse y transform (state precision)
+  const void* tw_yf = nullptr;  // y twiddles of the forward y transform
+  const void* tw_z = nullptr;   // z twiddles (both z transforms)
+  const void* pv = nullptr;     // potential phase table, state layout (null when spx is set)
+  const void* spx = nullptr;    // separable potential: per-axis phase factors [packet][n_d]
+  const void* spy = nullptr;
+  const void* spz = nullptr;
+  long nx = 1;                  // x planes per packet
+  long planes = 0;              // packets * nx
+};
+bool yz_plane_supported(int precision, int ny, int nz);
+int launch_yz_plane(int precision, int n, void* data, const YzPlaneParams& p, hipStream_t s);
+
 // pv = cexp_i(-dt V) elementwise in the state precision
 int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s);
 

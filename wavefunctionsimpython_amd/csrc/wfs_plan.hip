@@ -275,6 +275,36 @@ int set_potential_axes(wfs_plan* p, const double* const f[3]) {
   return 0;
 }
 
+// y inverse (table copy sel - 1), z inverse / potential / z forward and y forward (copy sel)
+// of one step as one launch on whole x planes
+int run_yz_plane(wfs_plan* p, void* data, const void* pv, hipStream_t s, int sel) {
+  YzPlaneParams pp;
+  pp.tw_yi = table_copy(p, p->tw[1], 1, sel - 1);
+  pp.tw_yf = table_copy(p, p->tw[1], 1, sel);
+  pp.tw_z = table_copy(p, p->tw[2], 2, sel);
+  if (p->sep_set) {
+    const size_t per = size_t(p->batch) * p->elem;
+    const int c = sel % p->ntab;
+    pp.spx = static_cast<const char*>(p->sep[0]) + size_t(c) * per * p->n[0];
+    pp.spy = static_cast<const char*>(p->sep[1]) + size_t(c) * per * p->n[1];
+    pp.spz = static_cast<const char*>(p->sep[2]) + size_t(c) * per * p->n[2];
+  } else {
+    pp.pv = pv;
+  }
+  pp.nx = p->n[0];
+  pp.planes = long(p->batch) * p->n[0];
+  ProfileScope scope(p, WFS_K_YZ_PLANE, s);
+  return launch_yz_plane(p->precision, p->n[1], data, pp, s);
+}
+
+// the passes between two x passes: fused plane kernel or three axis passes
+int run_yz(wfs_plan* p, void* data, const void* pv, hipStream_t s, int sel) {
+  if (p->variant[WFS_K_YZ_PLANE] == 1) return run_yz_plane(p, data, pv, s, sel);
+  WFS_TRY(run_pass(p, 1, MODE_INV, data, nullptr, nullptr, 1.0, s, sel - 1));
+  WFS_TRY(run_pass(p, 2, MODE_IVF, data, pv, nullptr, 1.0, s, sel));
+  return run_pass(p, 1, MODE_FWD, data, nullptr, nullptr, 1.0, s, sel);
+}
+
 int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
   if (V) WFS_TRY(set_potential(p, V, s));
   if (!p->pv_set) { set_last_error("wfs_step: no potential (pass V or call wfs_set_potential first)"); return WFS_ERR_INVALID; }
@@ -286,9 +316,7 @@ int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t
   WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s, 0));
   WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, kh, 1.0, s, 0));
   for (int step = 1; step <= nsteps; ++step) {
-    WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s, step - 1));
-    WFS_TRY(run_pass(p, 2, MODE_IVF, psi, pv, nullptr, 1.0, s, step));
-    WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s, step));
+    WFS_TRY(run_yz(p, psi, pv, s, step));
     WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, step == nsteps ? kh : kf, 1.0, s, step));
   }
   WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s, nsteps));
@@ -400,6 +428,7 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->ntab = precision == WFS_C128 ? 1 : kDither;
   for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
   p->variant[WFS_K_HORNER] = 2;   // fused stage pairs on 8 x 64 tiles (measured fastest at 256^3)
+  p->variant[WFS_K_YZ_PLANE] = yz_plane_supported(precision, n[1], n[2]) ? 1 : 0;
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;
@@ -509,8 +538,12 @@ extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, 
 }
 
 extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
-  const int vmax = kc == WFS_K_HORNER ? 3 : 2;
+  const int vmax = kc == WFS_K_HORNER ? 3 : kc == WFS_K_YZ_PLANE ? 1 : 2;
   if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > vmax) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
+  if (kc == WFS_K_YZ_PLANE && variant == 1 && !yz_plane_supported(p->precision, p->n[1], p->n[2])) {
+    set_last_error("wfs_plan_set_variant: the fused plane pass needs ny == nz <= %d", p->precision == WFS_C128 ? 64 : 128);
+    return WFS_ERR_UNSUPPORTED;
+  }
   p->variant[kc] = variant;
   return 0;
 }
@@ -561,6 +594,24 @@ extern "C" int wfs_plan_autotune(wfs_plan* p, void* stream) {
     }
     p->variant[kc] = rc == 0 ? best_v : keep;
     if (rc) break;
+  }
+  // fused plane pass against the three (tuned) axis passes it replaces
+  if (rc == 0 && yz_plane_supported(p->precision, p->n[1], p->n[2])) {
+    const int keep = p->variant[WFS_K_YZ_PLANE];
+    float best = 1e30f;
+    int best_v = keep;
+    for (int v = 0; v < 2 && rc == 0; ++v) {
+      p->variant[WFS_K_YZ_PLANE] = v;
+      rc = run_yz(p, buf, tab, s, 1);
+      (void)hipEventRecord(a, s);
+      for (int r = 0; r < 3 && rc == 0; ++r) rc = run_yz(p, buf, tab, s, 1);
+      (void)hipEventRecord(b, s);
+      (void)hipEventSynchronize(b);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (rc == 0 && ms < best) { best = ms; best_v = v; }
+    }
+    p->variant[WFS_K_YZ_PLANE] = rc == 0 ? best_v : keep;
   }
   p->profiling = prof;
   (void)hipEventDestroy(a);
