@@ -80,12 +80,12 @@ class PacketBatch:
                     _abi.add_lattice(n, dx, s.lattice_depth, s.lattice_period, V[j])
                 self._where[i] = (len(self.classes), j)
             plan = _abi.Plan(n, dx, dt, mass, precision, _abi.WFS_METHOD_FOURIER, batch=B)
-            if autotune:
-                plan.autotune()
             # harmonic + lattice is separable: per-axis phase factors instead of an N^3 table
             axes = [_abi.separable_axes(n, dx, self.specs[i].omega, self.specs[i].mass, (0.0, 0.0, 0.0),
                                         self.specs[i].lattice_depth, self.specs[i].lattice_period) for i in members]
             plan.set_potential_axes(*(np.stack([a[d] for a in axes]) for d in range(3)))
+            if autotune:
+                plan.autotune()         # after the potential: tunes the separable-potential path
             self.classes.append(_Class(key, members, plan, psi, V, torch.cuda.Stream(device=self.device)))
         torch.cuda.synchronize(self.device)
 

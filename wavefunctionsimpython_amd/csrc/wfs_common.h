@@ -266,14 +266,21 @@ struct DFT {
 // synchronises the block once between its LDS writes and reads; the caller
 // synchronises before the same slots are written again.
 // ---------------------------------------------------------------------------
-template <int N, int N2, int SIGN, typename C, typename LIDX>
-__device__ __forceinline__ void column_dft(C (&a)[N / N2], const C (&w)[N / N2], C* lds, int t, LIDX lidx) {
+// First step of column_dft: the length-N1 DFT in registers and the table twiddles.
+template <int N, int N2, int SIGN, typename C>
+__device__ __forceinline__ void column_dft_first(C (&a)[N / N2], const C (&w)[N / N2]) {
   constexpr int N1 = N / N2;
-  constexpr int M = N1 / N2;            // length-N2 DFTs per thread in the second step
   static_assert(N1 * N2 == N && N1 % N2 == 0, "bad split");
   DFT<N1, SIGN>::run(a);
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) a[k1] = (SIGN < 0) ? cmul(a[k1], w[k1]) : cmulc(a[k1], w[k1]);
+}
+
+// Second step of column_dft: LDS exchange and the length-N2 DFTs.
+template <int N, int N2, int SIGN, typename C, typename LIDX>
+__device__ __forceinline__ void column_dft_second(C (&a)[N / N2], C* lds, int t, LIDX lidx) {
+  constexpr int N1 = N / N2;
+  constexpr int M = N1 / N2;            // length-N2 DFTs per thread in the second step
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) lds[lidx(k1, t)] = a[k1];
   __syncthreads();
@@ -286,6 +293,12 @@ __device__ __forceinline__ void column_dft(C (&a)[N / N2], const C (&w)[N / N2],
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) a[m + M * k2] = b[k2];
   }
+}
+
+template <int N, int N2, int SIGN, typename C, typename LIDX>
+__device__ __forceinline__ void column_dft(C (&a)[N / N2], const C (&w)[N / N2], C* lds, int t, LIDX lidx) {
+  column_dft_first<N, N2, SIGN>(a, w);
+  column_dft_second<N, N2, SIGN>(a, lds, t, lidx);
 }
 
 // GELU, tanh form: 0.5 u (1 + tanh(sqrt(2/pi) (u + 0.044715 u^3))), and its derivative

@@ -158,23 +158,26 @@ template <> struct split_of<512> { static constexpr int n2 = 16; };
 
 // ---------------------------------------------------------------------------
 // Fused y-z pass on whole x planes (ny = nz = N).  One block holds one plane,
-// N*N points, in registers (N1 per thread) and moves it through LDS:
+// N*N points, in registers and moves it through LDS:
 //   y inverse -> transpose -> z inverse -> potential phase -> z forward
 //   -> transpose -> y forward
 // Each transform is column_dft with the split, table twiddles and operation
 // order of the axis passes y_inv, z_inv_v_fwd and y_fwd, so the result is the
 // same; the plane crosses HBM once (16 B per point for complex64) instead of
-// three times (48 B per point).  The LDS holds the largest of the column
-// exchange (N*N), the padded row exchange (N * N1 * (N2+1)) and the padded
-// transposition (N * (N + N2)); for N = 128 complex64 that is 144 KiB, one
-// block of 1024 threads per CU.
+// three times (48 B per point).
+// Thread t owns lines c0 + h N/H (h < H; columns z for the y transforms, rows y
+// for the z transforms) as thread u = t / (N/H) of each, N1 points per line:
+// the same mapping for both axes, so with N/H a multiple of 64 the table
+// twiddles of both transforms are wavefront-uniform (scalar loads).  The
+// exchanges and transpositions go through LDS in H rounds of N/H lines: the
+// LDS holds N/H * (N + 1) points (the transposition, padded; the exchanges
+// take N/H * N).  For N = 128 complex64 and H = 2: 66 KiB and 512 threads, two
+// blocks per CU, so one block's loads and stores overlap the other's
+// arithmetic.
 // ---------------------------------------------------------------------------
-constexpr int max3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
-
-template <typename R, int N>
+template <typename R, int N, int H>
 constexpr long yz_plane_lds_bytes() {
-  constexpr int N2 = split_of<N>::n2, N1 = N / N2;
-  return long(max3(N * N, N * N1 * (N2 + 1), N * (N + N2)) + N) * long(sizeof(cx<R>));
+  return long(N / H) * (N + 1) * long(sizeof(cx<R>));
 }
 
 // An opaque copy of v: the compiler recomputes what derives from it instead of
@@ -184,122 +187,174 @@ __device__ __forceinline__ int opaque_v(int v) {
   return v;
 }
 
-template <typename R, int N>
-__global__ __launch_bounds__(N * split_of<N>::n2) void yz_plane_kernel(cx<R>* __restrict__ data,
-                                                                        const YzPlaneParams p) {
+template <typename R, int N, int H>
+__global__ __launch_bounds__(N * split_of<N>::n2 / H)
+__attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_plane_kernel(cx<R>* __restrict__ data,
+                                                                           const YzPlaneParams p) {
   constexpr int N2 = split_of<N>::n2;
   constexpr int N1 = N / N2;
-  constexpr int TS = N + N2;                 // padded row stride of the transposition
-  constexpr int RS = N1 * (N2 + 1);          // padded row stride of the row exchange
-  // y-column thread index uniform over a wavefront: y twiddles by scalar loads
-  constexpr bool Y_UNIFORM = (N % 64) == 0;
+  constexpr int NH = N / H;                  // lines per round
+  constexpr int N1H = N1 / H;                // points of a line per round of the transposition
+  constexpr int TS = N + 1;                  // padded row stride of the transposition
+  static_assert(N1 % H == 0 && N2 >= H, "bad plane split");
+  constexpr bool UNIFORM = (NH % 64) == 0;   // u is the same across a wavefront
   using C = cx<R>;
   using CC = const __attribute__((address_space(4))) C;
-  __shared__ C lds[max3(N * N, N * RS, N * TS)];
-  __shared__ C twz[N];                       // z twiddles
+  __shared__ C lds[NH * TS];
 
   const int tid = threadIdx.x;
   const long plane = blockIdx.x;
   C* __restrict__ blk = data + plane * long(N) * N;
-  if (tid < N) twz[tid] = reinterpret_cast<const C*>(p.tw_z)[tid];
 
-  C a[N1];
+  C a[H][N1];
   C w[N1];
-  // y transform on columns: thread ty of column gz holds y = ty + N2 i
-  auto y_stage = [&](const void* twp, auto sign_c) {
-    constexpr int SIGN = decltype(sign_c)::value;
-    const int gz = opaque_v(tid) % N;
-    if constexpr (Y_UNIFORM) {
-      const int ty = __builtin_amdgcn_readfirstlane(tid / N);
+  auto thread_u = [&]() -> int {
+    if constexpr (UNIFORM) return __builtin_amdgcn_readfirstlane(tid / NH);
+    else return opaque_v(tid) / NH;
+  };
+  auto load_w = [&](const void* twp, int u) {
+    if constexpr (UNIFORM) {
       CC* tw = (CC*)(twp);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = mk<R>(tw[(ty * k1) % N].x, tw[(ty * k1) % N].y);
-      column_dft<N, N2, SIGN>(a, w, lds, ty, [&](int k1, int n2) { return (k1 * N2 + n2) * N + gz; });
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = mk<R>(tw[(u * k1) % N].x, tw[(u * k1) % N].y);
     } else {
-      const int ty = opaque_v(tid) / N;
       const C* __restrict__ tw = reinterpret_cast<const C*>(twp);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(ty * k1) % N];
-      column_dft<N, N2, SIGN>(a, w, lds, ty, [&](int k1, int n2) { return (k1 * N2 + n2) * N + gz; });
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(u * k1) % N];
     }
   };
-  // transposition through LDS between the column and row layouts
+  // the exchange of column_dft for the lines of one round
+  auto exchange = [&](C (&line)[N1], int u, auto sign_c) {
+    const int c0 = opaque_v(tid) % NH;
+    column_dft_second<N, N2, decltype(sign_c)::value>(line, lds, u,
+                                                      [&](int k1, int n2) { return (k1 * N2 + n2) * NH + c0; });
+  };
+  // one transform of every line, round by round (one round's temporaries live at a time)
+  auto transform = [&](const void* twp, auto sign_c) {
+    const int u = thread_u();
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      if (h) __syncthreads();
+      load_w(twp, u);
+      column_dft_first<N, N2, decltype(sign_c)::value>(a[h], w);
+      exchange(a[h], u, sign_c);
+    }
+  };
+  // transpositions between the column (y) and row (z) lines, N/H rows a round
   auto cols_to_rows = [&]() {
     const int t = opaque_v(tid);
-    const int gz = t % N, ty = t / N, tz = t % N2, gy = t / N2;
+    const int c0 = t % NH, u = t / NH;
+    C b[H][N1];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) lds[(ty + N2 * i) * TS + gz] = a[i];
-    __syncthreads();
+    for (int r = 0; r < H; ++r) {
+      if (r) __syncthreads();
 #pragma unroll
-    for (int i = 0; i < N1; ++i) a[i] = lds[gy * TS + tz + N2 * i];
+      for (int h = 0; h < H; ++h)
+#pragma unroll
+        for (int i = 0; i < N1H; ++i) lds[(u + N2 * i) * TS + c0 + h * NH] = a[h][r * N1H + i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < N1; ++i) b[r][i] = lds[c0 * TS + u + N2 * i];
+    }
+#pragma unroll
+    for (int r = 0; r < H; ++r)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a[r][i] = b[r][i];
   };
   auto rows_to_cols = [&]() {
     const int t = opaque_v(tid);
-    const int gz = t % N, ty = t / N, tz = t % N2, gy = t / N2;
+    const int c0 = t % NH, u = t / NH;
+    C b[H][N1];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) lds[gy * TS + tz + N2 * i] = a[i];
-    __syncthreads();
+    for (int r = 0; r < H; ++r) {
+      if (r) __syncthreads();
 #pragma unroll
-    for (int i = 0; i < N1; ++i) a[i] = lds[(ty + N2 * i) * TS + gz];
+      for (int i = 0; i < N1; ++i) lds[c0 * TS + u + N2 * i] = a[r][i];
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < H; ++h)
+#pragma unroll
+        for (int i = 0; i < N1H; ++i) b[h][r * N1H + i] = lds[(u + N2 * i) * TS + c0 + h * NH];
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a[h][i] = b[h][i];
   };
 
   {
-    const int gz = tid % N, ty = tid / N;
+    const int c0 = tid % NH, u = tid / NH;
 #pragma unroll
-    for (int i = 0; i < N1; ++i) a[i] = blk[unsigned(gz + (ty + N2 * i) * N)];
+    for (int h = 0; h < H; ++h)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a[h][i] = blk[unsigned(c0 + h * NH + (u + N2 * i) * N)];
   }
 
-  // y inverse
-  y_stage(p.tw_yi, std::integral_constant<int, +1>{});
+  // y inverse on columns
+  transform(p.tw_yi, std::integral_constant<int, +1>{});
   __syncthreads();
   cols_to_rows();
   __syncthreads();
 
-  // z inverse, potential phase, z forward on rows: thread tz of row gy holds z = tz + N2 i
+  // z inverse, potential phase and z forward on rows y = c0 + r N/H (z = u + N2 i).  Each
+  // row round is finished (exchange, potential, first forward step) before the next one's
+  // exchange, so that only one round's potential factors are held.
   {
-    const int tz = tid % N2, gy = tid / N2;
-    auto lidx_r = [&](int k1, int n2) -> int { return gy * RS + k1 * (N2 + 1) + n2; };
+    const int u = thread_u();
 #pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) w[k1] = twz[(tz * k1) % N];
-    column_dft<N, N2, +1>(a, w, lds, tz, lidx_r);
-    if (p.spx) {
-      const long b = plane / p.nx;
-      const C* __restrict__ px = reinterpret_cast<const C*>(p.spx);
-      const C* __restrict__ py = reinterpret_cast<const C*>(p.spy) + b * N;
-      const C* __restrict__ pzb = reinterpret_cast<const C*>(p.spz) + b * N;
-      const C pxy = cmul(px[plane], py[gy]);
+    for (int r = 0; r < H; ++r) {
+      if (r) __syncthreads();
+      load_w(p.tw_z, u);
+      column_dft_first<N, N2, +1>(a[r], w);
+      exchange(a[r], u, std::integral_constant<int, +1>{});
+      const int c0 = opaque_v(tid) % NH;
+      if (p.spx) {
+        const long b = plane / p.nx;
+        const C* __restrict__ px = reinterpret_cast<const C*>(p.spx);
+        const C* __restrict__ py = reinterpret_cast<const C*>(p.spy) + b * N;
+        const C* __restrict__ pzb = reinterpret_cast<const C*>(p.spz) + b * N;
+        const C pxy = cmul(px[plane], py[c0 + r * NH]);
+        const int ur = opaque_v(u);     // separate loads per round (not held across rounds)
 #pragma unroll
-      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], cmul(pzb[tz + N2 * i], pxy));
-    } else {
-      const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + plane * long(N) * N;
+        for (int i = 0; i < N1; ++i) a[r][i] = cmul(a[r][i], cmul(pzb[ur + N2 * i], pxy));
+      } else {
+        const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + plane * long(N) * N;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) a[i] = cmul(a[i], pvb[unsigned(gy * N + tz + N2 * i)]);
+        for (int i = 0; i < N1; ++i) a[r][i] = cmul(a[r][i], pvb[unsigned((c0 + r * NH) * N + u + N2 * i)]);
+      }
+      load_w(p.tw_z, u);
+      column_dft_first<N, N2, -1>(a[r], w);
     }
-    __syncthreads();
-    column_dft<N, N2, -1>(a, w, lds, tz, lidx_r);
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      __syncthreads();
+      exchange(a[r], u, std::integral_constant<int, -1>{});
+    }
   }
   __syncthreads();
   rows_to_cols();
   __syncthreads();
 
-  // y forward
-  y_stage(p.tw_yf, std::integral_constant<int, -1>{});
+  // y forward on columns
+  transform(p.tw_yf, std::integral_constant<int, -1>{});
   {
     const int t = opaque_v(tid);
-    const int gz = t % N, ty = t / N;
+    const int c0 = t % NH, u = t / NH;
 #pragma unroll
-    for (int i = 0; i < N1; ++i) blk[unsigned(gz + (ty + N2 * i) * N)] = a[i];
+    for (int h = 0; h < H; ++h)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) blk[unsigned(c0 + h * NH + (u + N2 * i) * N)] = a[h][i];
   }
 }
 
-template <typename R, int N>
+template <typename R, int N, int H>
 static int launch_yz_plane_n(void* data, const YzPlaneParams& p, hipStream_t s) {
-  static_assert(yz_plane_lds_bytes<R, N>() <= 160 * 1024, "plane does not fit in LDS");
+  static_assert(yz_plane_lds_bytes<R, N, H>() <= 160 * 1024, "plane does not fit in LDS");
   if (p.planes <= 0 || p.planes > 0x7fffffffL) {
     set_last_error("yz plane pass: bad plane count %ld", p.planes);
     return WFS_ERR_INVALID;
   }
-  hipLaunchKernelGGL((yz_plane_kernel<R, N>), dim3(unsigned(p.planes)), dim3(N * split_of<N>::n2), 0, s,
+  hipLaunchKernelGGL((yz_plane_kernel<R, N, H>), dim3(unsigned(p.planes)), dim3(N * split_of<N>::n2 / H), 0, s,
                      reinterpret_cast<cx<R>*>(data), p);
   return int(hipGetLastError());
 }
@@ -388,27 +443,30 @@ int launch_fft_pass_t(int n, int layout, int mode, void* data, const PassParams&
 #if WFS_FFT_PREC == 0
 template int launch_fft_pass_t<float>(int, int, int, void*, const PassParams&, long, int, hipStream_t);
 
-int launch_yz_plane_c128(int n, void* data, const YzPlaneParams& p, hipStream_t s);
+int launch_yz_plane_c128(int n, int halves, void* data, const YzPlaneParams& p, hipStream_t s);
 
-// planes that fit in LDS: complex64 up to 128 x 128, complex128 up to 64 x 64
-bool yz_plane_supported(int precision, int ny, int nz) {
-  if (ny != nz) return false;
-  if (precision == WFS_C128) return ny == 16 || ny == 32 || ny == 64;
-  return ny == 16 || ny == 32 || ny == 64 || ny == 96 || ny == 128;
+// planes that fit in LDS: complex64 up to 128 x 128, complex128 up to 64 x 64; two rounds
+// (halves = 2) for complex64 64 and 128 and complex128 64
+bool yz_plane_supported(int precision, int ny, int nz, int halves) {
+  if (ny != nz || halves < 1 || halves > 3) return false;
+  if (precision == WFS_C128) return (ny == 16 || ny == 32 || ny == 64) && (halves == 1 || ny == 64);
+  if (ny == 192) return halves == 3;
+  return (ny == 16 || ny == 32 || ny == 64 || ny == 96 || ny == 128) && (halves == 1 || ny == 64 || ny == 128);
 }
 
-int launch_yz_plane(int precision, int n, void* data, const YzPlaneParams& p, hipStream_t s) {
-  if (!yz_plane_supported(precision, n, n)) {
-    set_last_error("yz plane pass: unsupported plane %d x %d", n, n);
+int launch_yz_plane(int precision, int n, int halves, void* data, const YzPlaneParams& p, hipStream_t s) {
+  if (!yz_plane_supported(precision, n, n, halves)) {
+    set_last_error("yz plane pass: unsupported plane %d x %d (%d rounds)", n, n, halves);
     return WFS_ERR_UNSUPPORTED;
   }
-  if (precision == WFS_C128) return launch_yz_plane_c128(n, data, p, s);
+  if (precision == WFS_C128) return launch_yz_plane_c128(n, halves, data, p, s);
   switch (n) {
-    case 16:  return launch_yz_plane_n<float, 16>(data, p, s);
-    case 32:  return launch_yz_plane_n<float, 32>(data, p, s);
-    case 64:  return launch_yz_plane_n<float, 64>(data, p, s);
-    case 96:  return launch_yz_plane_n<float, 96>(data, p, s);
-    default:  return launch_yz_plane_n<float, 128>(data, p, s);
+    case 16:  return launch_yz_plane_n<float, 16, 1>(data, p, s);
+    case 32:  return launch_yz_plane_n<float, 32, 1>(data, p, s);
+    case 64:  return halves == 2 ? launch_yz_plane_n<float, 64, 2>(data, p, s) : launch_yz_plane_n<float, 64, 1>(data, p, s);
+    case 96:  return launch_yz_plane_n<float, 96, 1>(data, p, s);
+    case 192: return launch_yz_plane_n<float, 192, 3>(data, p, s);
+    default:  return halves == 2 ? launch_yz_plane_n<float, 128, 2>(data, p, s) : launch_yz_plane_n<float, 128, 1>(data, p, s);
   }
 }
 
@@ -427,11 +485,11 @@ int launch_fft_pass(int precision, int n, int layout, int mode, void* data, cons
 #else
 template int launch_fft_pass_t<double>(int, int, int, void*, const PassParams&, long, int, hipStream_t);
 
-int launch_yz_plane_c128(int n, void* data, const YzPlaneParams& p, hipStream_t s) {
+int launch_yz_plane_c128(int n, int halves, void* data, const YzPlaneParams& p, hipStream_t s) {
   switch (n) {
-    case 16:  return launch_yz_plane_n<double, 16>(data, p, s);
-    case 32:  return launch_yz_plane_n<double, 32>(data, p, s);
-    default:  return launch_yz_plane_n<double, 64>(data, p, s);
+    case 16:  return launch_yz_plane_n<double, 16, 1>(data, p, s);
+    case 32:  return launch_yz_plane_n<double, 32, 1>(data, p, s);
+    default:  return halves == 2 ? launch_yz_plane_n<double, 64, 2>(data, p, s) : launch_yz_plane_n<double, 64, 1>(data, p, s);
   }
 }
 #endif

@@ -47,8 +47,9 @@ struct YzPlaneParams {
   long nx = 1;                  // x planes per packet
   long planes = 0;              // packets * nx
 };
-bool yz_plane_supported(int precision, int ny, int nz);
-int launch_yz_plane(int precision, int n, void* data, const YzPlaneParams& p, hipStream_t s);
+// halves: lines per thread (1, 2 or 3), the plane passing through LDS in that many rounds
+bool yz_plane_supported(int precision, int ny, int nz, int halves);
+int launch_yz_plane(int precision, int n, int halves, void* data, const YzPlaneParams& p, hipStream_t s);
 
 // pv = cexp_i(-dt V) elementwise in the state precision
 int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s);

@@ -294,12 +294,12 @@ int run_yz_plane(wfs_plan* p, void* data, const void* pv, hipStream_t s, int sel
   pp.nx = p->n[0];
   pp.planes = long(p->batch) * p->n[0];
   ProfileScope scope(p, WFS_K_YZ_PLANE, s);
-  return launch_yz_plane(p->precision, p->n[1], data, pp, s);
+  return launch_yz_plane(p->precision, p->n[1], p->variant[WFS_K_YZ_PLANE], data, pp, s);
 }
 
 // the passes between two x passes: fused plane kernel or three axis passes
 int run_yz(wfs_plan* p, void* data, const void* pv, hipStream_t s, int sel) {
-  if (p->variant[WFS_K_YZ_PLANE] == 1) return run_yz_plane(p, data, pv, s, sel);
+  if (p->variant[WFS_K_YZ_PLANE] != 0) return run_yz_plane(p, data, pv, s, sel);
   WFS_TRY(run_pass(p, 1, MODE_INV, data, nullptr, nullptr, 1.0, s, sel - 1));
   WFS_TRY(run_pass(p, 2, MODE_IVF, data, pv, nullptr, 1.0, s, sel));
   return run_pass(p, 1, MODE_FWD, data, nullptr, nullptr, 1.0, s, sel);
@@ -428,7 +428,14 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->ntab = precision == WFS_C128 ? 1 : kDither;
   for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
   p->variant[WFS_K_HORNER] = 2;   // fused stage pairs on 8 x 64 tiles (measured fastest at 256^3)
-  p->variant[WFS_K_YZ_PLANE] = yz_plane_supported(precision, n[1], n[2]) ? 1 : 0;
+  // fused plane pass where the plane fits in LDS; at 128^2 two rounds (two blocks per CU) once
+  // there are enough planes to fill the CUs twice over (measured: 1024 planes 13% faster, 128
+  // planes 11% slower)
+  // 192^2 planes (three rounds, one block of 8 wavefronts per CU) are left to the autotuner:
+  // faster than the axis passes with a separable potential, slower with a potential table
+  p->variant[WFS_K_YZ_PLANE] =
+      (yz_plane_supported(precision, n[1], n[2], 2) && n[1] == 128 && long(batch) * n[0] >= 512) ? 2
+      : yz_plane_supported(precision, n[1], n[2], 1) ? 1 : 0;
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;
@@ -538,10 +545,10 @@ extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, 
 }
 
 extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
-  const int vmax = kc == WFS_K_HORNER ? 3 : kc == WFS_K_YZ_PLANE ? 1 : 2;
+  const int vmax = (kc == WFS_K_HORNER || kc == WFS_K_YZ_PLANE) ? 3 : 2;
   if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > vmax) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
-  if (kc == WFS_K_YZ_PLANE && variant == 1 && !yz_plane_supported(p->precision, p->n[1], p->n[2])) {
-    set_last_error("wfs_plan_set_variant: the fused plane pass needs ny == nz <= %d", p->precision == WFS_C128 ? 64 : 128);
+  if (kc == WFS_K_YZ_PLANE && variant > 0 && !yz_plane_supported(p->precision, p->n[1], p->n[2], variant)) {
+    set_last_error("wfs_plan_set_variant: fused plane pass variant %d unsupported for %d x %d planes", variant, p->n[1], p->n[2]);
     return WFS_ERR_UNSUPPORTED;
   }
   p->variant[kc] = variant;
@@ -596,11 +603,14 @@ extern "C" int wfs_plan_autotune(wfs_plan* p, void* stream) {
     if (rc) break;
   }
   // fused plane pass against the three (tuned) axis passes it replaces
-  if (rc == 0 && yz_plane_supported(p->precision, p->n[1], p->n[2])) {
+  bool any_plane = false;
+  for (int v = 1; v < 4; ++v) any_plane = any_plane || yz_plane_supported(p->precision, p->n[1], p->n[2], v);
+  if (rc == 0 && any_plane) {
     const int keep = p->variant[WFS_K_YZ_PLANE];
     float best = 1e30f;
     int best_v = keep;
-    for (int v = 0; v < 2 && rc == 0; ++v) {
+    for (int v = 0; v < 4 && rc == 0; ++v) {
+      if (v > 0 && !yz_plane_supported(p->precision, p->n[1], p->n[2], v)) continue;
       p->variant[WFS_K_YZ_PLANE] = v;
       rc = run_yz(p, buf, tab, s, 1);
       (void)hipEventRecord(a, s);
