@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 C64, C128 = _abi.WFS_C64, _abi.WFS_C128
 DX = (0.3, 0.25, 0.35)
-CASES = [(C64, 16), (C64, 32), (C64, 64), (C64, 96), (C64, 128), (C64, 192), (C128, 16), (C128, 32), (C128, 64)]
+CASES = [(C64, 16), (C64, 32), (C64, 64), (C64, 96), (C64, 128), (C128, 16), (C128, 32), (C128, 64)]
 
 
 def _cdtype(precision):
@@ -51,8 +51,6 @@ def _run(shape, precision, variant, psi0, V=None, axes=None, batch=1, steps=5):
 
 def _variants(precision, n):
     """plane variants: lines per thread, the plane passing through LDS in that many rounds"""
-    if precision == C64 and n == 192:
-        return (3,)
     return (1, 2) if (precision == C64 and n in (64, 128)) or (precision == C128 and n == 64) else (1,)
 
 
@@ -68,7 +66,7 @@ def test_plane_pass_equals_axis_passes_table_potential(dev, precision, n):
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("precision,n", [(C64, 64), (C64, 96), (C64, 128), (C64, 192), (C128, 32), (C128, 64)])
+@pytest.mark.parametrize("precision,n", [(C64, 64), (C64, 96), (C64, 128), (C128, 32), (C128, 64)])
 def test_plane_pass_equals_axis_passes_separable_batched(dev, precision, n):
     batch = 3
     shape = (8, n, n)
@@ -104,8 +102,7 @@ def test_plane_pass_defaults_and_limits(dev):
     assert variant((128, 128, 128), C64, batch=4) == 2  # 512 planes: two blocks per CU
     assert variant((32, 96, 96), C64) == 1
     assert variant((32, 128, 128), C128) == 0         # 256 KiB plane: three passes
-    assert variant((32, 192, 192), C64) == 0            # three rounds: only by choice or autotuning
-    assert variant((32, 192, 192), C128) == 0
+    assert variant((32, 192, 192), C64) == 0            # 288 KiB plane: three passes
     assert variant((32, 256, 256), C64) == 0
     assert variant((32, 64, 128), C64) == 0           # ny != nz
     p = _abi.Plan((16, 64, 32), (0.3,) * 3, 0.01, 1.0, C64, _abi.WFS_METHOD_FOURIER)
@@ -124,7 +121,7 @@ def test_autotune_picks_a_plane_variant_with_identical_result(dev):
     p.set_potential(V)
     chosen = p.autotune()
     print(f"autotuned yz_plane variant at 64^3: {chosen['yz_plane']}")
-    assert chosen["yz_plane"] in (0, 1, 2, 3)
+    assert chosen["yz_plane"] in (0, 1, 2)
     psi = psi0.clone()
     p.step(psi, None, 5)
     ref = _run(shape, C64, 0, psi0, V)

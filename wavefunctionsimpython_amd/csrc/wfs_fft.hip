@@ -316,11 +316,17 @@ __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_plane_kernel(cx<R>*
         const C pxy = cmul(px[plane], py[c0 + r * NH]);
         const int ur = opaque_v(u);     // separate loads per round (not held across rounds)
 #pragma unroll
-        for (int i = 0; i < N1; ++i) a[r][i] = cmul(a[r][i], cmul(pzb[ur + N2 * i], pxy));
+        for (int i = 0; i < N1; ++i) {
+          a[r][i] = cmul(a[r][i], cmul(pzb[ur + N2 * i], pxy));
+          if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");   // at most 4 factors in flight
+        }
       } else {
         const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + plane * long(N) * N;
 #pragma unroll
-        for (int i = 0; i < N1; ++i) a[r][i] = cmul(a[r][i], pvb[unsigned((c0 + r * NH) * N + u + N2 * i)]);
+        for (int i = 0; i < N1; ++i) {
+          a[r][i] = cmul(a[r][i], pvb[unsigned((c0 + r * NH) * N + u + N2 * i)]);
+          if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");
+        }
       }
       load_w(p.tw_z, u);
       column_dft_first<N, N2, -1>(a[r], w);
@@ -448,9 +454,8 @@ int launch_yz_plane_c128(int n, int halves, void* data, const YzPlaneParams& p, 
 // planes that fit in LDS: complex64 up to 128 x 128, complex128 up to 64 x 64; two rounds
 // (halves = 2) for complex64 64 and 128 and complex128 64
 bool yz_plane_supported(int precision, int ny, int nz, int halves) {
-  if (ny != nz || halves < 1 || halves > 3) return false;
+  if (ny != nz || halves < 1 || halves > 2) return false;
   if (precision == WFS_C128) return (ny == 16 || ny == 32 || ny == 64) && (halves == 1 || ny == 64);
-  if (ny == 192) return halves == 3;
   return (ny == 16 || ny == 32 || ny == 64 || ny == 96 || ny == 128) && (halves == 1 || ny == 64 || ny == 128);
 }
 
@@ -465,7 +470,6 @@ int launch_yz_plane(int precision, int n, int halves, void* data, const YzPlaneP
     case 32:  return launch_yz_plane_n<float, 32, 1>(data, p, s);
     case 64:  return halves == 2 ? launch_yz_plane_n<float, 64, 2>(data, p, s) : launch_yz_plane_n<float, 64, 1>(data, p, s);
     case 96:  return launch_yz_plane_n<float, 96, 1>(data, p, s);
-    case 192: return launch_yz_plane_n<float, 192, 3>(data, p, s);
     default:  return halves == 2 ? launch_yz_plane_n<float, 128, 2>(data, p, s) : launch_yz_plane_n<float, 128, 1>(data, p, s);
   }
 }

@@ -47,7 +47,7 @@ struct YzPlaneParams {
   long nx = 1;                  // x planes per packet
   long planes = 0;              // packets * nx
 };
-// halves: lines per thread (1, 2 or 3), the plane passing through LDS in that many rounds
+// halves: lines per thread (1 or 2), the plane passing through LDS in that many rounds
 bool yz_plane_supported(int precision, int ny, int nz, int halves);
 int launch_yz_plane(int precision, int n, int halves, void* data, const YzPlaneParams& p, hipStream_t s);
 

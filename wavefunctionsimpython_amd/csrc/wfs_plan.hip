@@ -431,8 +431,6 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   // fused plane pass where the plane fits in LDS; at 128^2 two rounds (two blocks per CU) once
   // there are enough planes to fill the CUs twice over (measured: 1024 planes 13% faster, 128
   // planes 11% slower)
-  // 192^2 planes (three rounds, one block of 8 wavefronts per CU) are left to the autotuner:
-  // faster than the axis passes with a separable potential, slower with a potential table
   p->variant[WFS_K_YZ_PLANE] =
       (yz_plane_supported(precision, n[1], n[2], 2) && n[1] == 128 && long(batch) * n[0] >= 512) ? 2
       : yz_plane_supported(precision, n[1], n[2], 1) ? 1 : 0;
@@ -545,7 +543,7 @@ extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, 
 }
 
 extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
-  const int vmax = (kc == WFS_K_HORNER || kc == WFS_K_YZ_PLANE) ? 3 : 2;
+  const int vmax = kc == WFS_K_HORNER ? 3 : 2;
   if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > vmax) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
   if (kc == WFS_K_YZ_PLANE && variant > 0 && !yz_plane_supported(p->precision, p->n[1], p->n[2], variant)) {
     set_last_error("wfs_plan_set_variant: fused plane pass variant %d unsupported for %d x %d planes", variant, p->n[1], p->n[2]);
@@ -604,12 +602,12 @@ extern "C" int wfs_plan_autotune(wfs_plan* p, void* stream) {
   }
   // fused plane pass against the three (tuned) axis passes it replaces
   bool any_plane = false;
-  for (int v = 1; v < 4; ++v) any_plane = any_plane || yz_plane_supported(p->precision, p->n[1], p->n[2], v);
+  for (int v = 1; v < 3; ++v) any_plane = any_plane || yz_plane_supported(p->precision, p->n[1], p->n[2], v);
   if (rc == 0 && any_plane) {
     const int keep = p->variant[WFS_K_YZ_PLANE];
     float best = 1e30f;
     int best_v = keep;
-    for (int v = 0; v < 4 && rc == 0; ++v) {
+    for (int v = 0; v < 3 && rc == 0; ++v) {
       if (v > 0 && !yz_plane_supported(p->precision, p->n[1], p->n[2], v)) continue;
       p->variant[WFS_K_YZ_PLANE] = v;
       rc = run_yz(p, buf, tab, s, 1);
