@@ -274,6 +274,19 @@ int wfs_gemm_bf16(const float* A, const float* B, float* C, const long dims[4], 
  * Workspaces (partials) need 8192 doubles.
  */
 int wfs_fno_bias_act(float* u, const float* bias, float* v, long C, long P, int act, void* stream);
+/*
+ * wfs_fno_layer (no reference counterpart): one network layer fused over the grid lines
+ * (x, y) of an n^3 grid, all channels:  u = W v + Z Gz + b;  vn = GELU(u) (act) or u;
+ * T = vn Fz.  W [C][cin], v [cin][n^3], Z [C][n^2][tm] (the layer's inverse spectral path
+ * before its z stage; NULL: no spectral term), Gz [tm][n], Fz [n][tm]; outputs u, vn
+ * [C][n^3] and T [C][n^2][tm] (the next layer's z stage), each optional (T needs vn).
+ * W, Gz, Fz enter the matrix cores as bf16, v, Z and vn as bf16 hi + lo parts, FP32
+ * accumulation.  n in {32, 64, 96, 128}; C, cin <= 32; tm a multiple of 4 up to 32;
+ * u, vn, Z 16-byte aligned.
+ */
+int wfs_fno_layer(const float* W, const float* v, const float* Z, const float* bias, const float* Gz,
+                  const float* Fz, float* u, float* vn, float* T, int cin, int C, int n, int tm, int act,
+                  void* stream);
 int wfs_fno_gelu_backward(const float* u, const float* dv, float* du, long n, void* stream);
 int wfs_fno_axpby(float* y, const float* x, float alpha, float beta, long n, void* stream);
 int wfs_fno_pack(const void* psi, int precision, const void* V, float pscale, float vscale, float* a, long P,

@@ -57,6 +57,25 @@ class EmulatedFNO(FNO3d):
         if not flags & 4:
             Cv.copy_(r)
 
+    def _layer(self, W, v, Z, bias, u, vn, T, cin, act):
+        """The fused layer as the GEMM composition it replaces: u = W v + Z Gz + b,
+        vn = GELU(u) or u, T = vn Fz (same operand splits)."""
+        C, n, P, tm = self.C, self.n, self.P, 2 * self.m
+        r = torch.empty(C * P, dtype=self.dtype, device=v.device)
+        self._gemm(W, v, r, 1, C, P, cin, (0, cin, 1), (0, P, 1), (0, P, 1), flags=2)
+        if Z is not None:
+            S = torch.empty_like(r)
+            self._gemm(Z, self.tab["Gz"], S, 1, C * n * n, n, tm, (0, tm, 1), (0, n, 1), (0, n, 1), flags=1)
+            r = r + S
+        r = (r.view(C, P) + bias[:C].view(C, 1)).reshape(-1)
+        if u is not None:
+            u.copy_(r)
+        g = fno_oracle.gelu_tanh(r) if act else r
+        if vn is not None:
+            vn.copy_(g)
+        if T is not None:
+            self._gemm(vn, self.tab["Fz"], T, 1, C * n * n, tm, n, (0, n, 1), (0, tm, 1), (0, tm, 1), flags=1)
+
     def _gelu_backward(self, u, dv, du):
         ut = u.detach().clone().requires_grad_(True)
         (g,) = torch.autograd.grad(fno_oracle.gelu_tanh(ut), ut, dv)

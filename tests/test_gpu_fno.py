@@ -122,6 +122,60 @@ def test_gemm_rejects_out_of_range_strides(dev):
         _abi.gemm_bf16(A, A, A, 1, 4, 4, 4, (0, 4, 1), (0, 4, 1), (0, 4, 1))
 
 
+@pytest.mark.parametrize("n,width,modes,layers", [(32, 8, 8, 2), (64, 16, 12, 3), (128, 32, 12, 4), (96, 32, 16, 2)])
+def test_fused_layers_match_gemm_composition(dev, n, width, modes, layers):
+    """The fused layer kernel (u = W v + Z Gz + b, GELU, next z stage per grid line) against the
+    network's unfused GEMM path on the same weights: forward output, the saved activations of a
+    training forward, and every gradient.  Both accumulate bf16 products in FP32 (in another
+    order), so they agree to FP32 rounding, not bit for bit."""
+    print(f"case: n={n} width={width} modes={modes} layers={layers}", flush=True)
+    nets = [FNO3d(n, width=width, modes=modes, layers=layers, seed=3, splits=64, fused=f) for f in (True, False)]
+    assert nets[0].fused and not nets[1].fused
+    nets[0].params["Q"].mul_(10.0)
+    nets[1].params["Q"].copy_(nets[0].params["Q"])
+    P = n ** 3
+    a = torch.from_numpy(np.random.default_rng(7).standard_normal(3 * P).astype(np.float32)).to(dev)
+    outs = [net.forward(a).clone() for net in nets]
+    e_inf = _rel(outs[0].cpu().numpy() - a[: 2 * P].cpu().numpy(), outs[1].cpu().numpy() - a[: 2 * P].cpu().numpy())
+    target = a[: 2 * P] + 0.1
+    grads = []
+    for net in nets:
+        out = net.forward(a, keep=True)
+        saved = [t.clone() for t in net._saved[1] + net._saved[2]]
+        dout = torch.empty_like(out)
+        net.zero_grad()
+        net.loss_and_grad(out, target, dout)
+        net.backward(dout)
+        grads.append(({k: g.clone() for k, g in net.grads.items()}, out.clone(), saved))
+    torch.cuda.synchronize()
+    e_keep = _rel(grads[0][1].cpu().numpy(), grads[1][1].cpu().numpy())
+    e_saved = max(_rel(x.cpu().numpy(), y.cpu().numpy()) for x, y in zip(grads[0][2], grads[1][2]))
+    g0 = {k: g.cpu().numpy() for k, g in grads[0][0].items()}
+    g1 = {k: g.cpu().numpy() for k, g in grads[1][0].items()}
+
+    def gnorm(k):
+        # a spectral weight is one complex number: its two gradients share one scale
+        if k[:2] in ("Rr", "Ri"):
+            return np.hypot(np.linalg.norm(g1["Rr" + k[2:]]), np.linalg.norm(g1["Ri" + k[2:]]))
+        return np.linalg.norm(g1[k])
+
+    e_grad = max(np.linalg.norm(g0[k] - g1[k]) / gnorm(k) for k in g1 if gnorm(k) > 0)
+    print(f"  fused vs GEMM path: forward (network term) {e_inf:.2e}, training forward {e_keep:.2e}, "
+          f"saved activations {e_saved:.2e}, gradients {e_grad:.2e}")
+    assert e_inf < 2e-5 and e_keep < 1e-6 and e_saved < 2e-5 and e_grad < 1e-3
+
+
+def test_fused_layer_limits(dev):
+    assert not FNO3d(16, width=8, modes=4, layers=1).fused          # n not a multiple of 32
+    assert not FNO3d(32, width=40, modes=4, layers=1).fused         # width above 32
+    assert not FNO3d(32, width=8, modes=18, layers=1).fused         # 2 modes above 32
+    z = torch.zeros(64, device=dev)
+    with pytest.raises(_abi.WfsError):
+        rc = _abi.lib().wfs_fno_layer(_abi.ptr(z), _abi.ptr(z), None, _abi.ptr(z), _abi.ptr(z), _abi.ptr(z),
+                                      None, _abi.ptr(z), None, 4, 4, 48, 8, 1, None)
+        _abi.check(rc, "wfs_fno_layer")
+
+
 FNO_CASES = [  # n, width, modes, layers
     (16, 8, 4, 3),
     (32, 8, 8, 2),

@@ -49,6 +49,7 @@ SIGNATURES = {
     "wfs_fno_bias_act": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_int, _vp]),
     "wfs_fno_gelu_backward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, _vp]),
     "wfs_fno_axpby": (ctypes.c_int, [_vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_long, _vp]),
+    "wfs_fno_layer": (ctypes.c_int, [_vp] * 9 + [ctypes.c_int] * 5 + [_vp]),
     "wfs_fno_pack": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_float, ctypes.c_float, _vp, ctypes.c_long,
                                     _vp]),
     "wfs_fno_unpack": (ctypes.c_int, [_vp, ctypes.c_float, _vp, ctypes.c_long, _vp]),
@@ -387,6 +388,32 @@ def gemm_bf16(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, batch: int, M: 
                               ptr(work) if work is not None else None, int(work.numel()) if work is not None else 0,
                               ptr(bias) if bias is not None else None, ptr(act) if act is not None else None,
                               stream_handle(stream)), "wfs_gemm_bf16")
+
+
+def fno_layer(W: torch.Tensor, v: torch.Tensor, Z: torch.Tensor | None, bias: torch.Tensor, Gz: torch.Tensor,
+              Fz: torch.Tensor, u: torch.Tensor | None, vn: torch.Tensor | None, T: torch.Tensor | None,
+              cin: int, C: int, n: int, tm: int, act: bool, stream=None) -> None:
+    """One Fourier-neural-operator layer fused over grid lines (``wfs_fno_layer``):
+    u = W v + Z Gz + b, vn = GELU(u) (act) or u, T = vn Fz, per line (x, y) of the n^3 grid.
+
+    W [C][cin], v [cin][n^3], Z [C][n^2][tm] (or None), bias [C], Gz [tm][n], Fz [n][tm];
+    outputs u, vn [C][n^3] and T [C][n^2][tm], each optional (T needs vn).  float32, contiguous.
+    """
+    P = n ** 3
+    need = [(W, C * cin, "W"), (v, cin * P, "v"), (bias, C, "bias"), (Gz, tm * n, "Gz"), (Fz, n * tm, "Fz")]
+    for t, size, name in need + [(Z, C * n * n * tm, "Z"), (u, C * P, "u"), (vn, C * P, "vn"),
+                                 (T, C * n * n * tm, "T")]:
+        if t is None:
+            continue
+        _require_device_any(t, name)
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous float32 tensor")
+        if t.numel() < size:
+            raise ValueError(f"{name} has {t.numel()} elements, needs {size}")
+    check(lib().wfs_fno_layer(ptr(W), ptr(v), ptr(Z) if Z is not None else None, ptr(bias), ptr(Gz), ptr(Fz),
+                              ptr(u) if u is not None else None, ptr(vn) if vn is not None else None,
+                              ptr(T) if T is not None else None, int(cin), int(C), int(n), int(tm),
+                              1 if act else 0, stream_handle(stream)), "wfs_fno_layer")
 
 
 def _require_device_any(t: torch.Tensor, name: str) -> None:

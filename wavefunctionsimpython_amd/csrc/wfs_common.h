@@ -302,15 +302,22 @@ __device__ __forceinline__ void column_dft(C (&a)[N / N2], const C (&w)[N / N2],
 }
 
 // GELU, tanh form: 0.5 u (1 + tanh(sqrt(2/pi) (u + 0.044715 u^3))), and its derivative
+// tanh(t) = 1 - 2 / (exp(2t) + 1), with exp(2t) = 2^(2 t log2 e) from the hardware
+// exponential (v_exp_f32) and the hardware reciprocal (v_rcp_f32): a few ulp of 1 in
+// absolute terms (what 1 + tanh in the GELU needs), saturating to +-1 for large |t|.
+__device__ __forceinline__ float tanh_hw(float t) {
+  const float e = __builtin_amdgcn_exp2f(t * 2.8853900817779268f);   // 2 log2(e)
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
 __device__ __forceinline__ float gelu_f(float u) {
   const float k = 0.7978845608028654f;
-  const float t = tanhf(k * (u + 0.044715f * u * u * u));
+  const float t = tanh_hw(k * (u + 0.044715f * u * u * u));
   return 0.5f * u * (1.0f + t);
 }
 __device__ __forceinline__ float gelu_grad(float u) {
   const float k = 0.7978845608028654f;
   const float a = k * (u + 0.044715f * u * u * u);
-  const float t = tanhf(a);
+  const float t = tanh_hw(a);
   const float da = k * (1.0f + 3.0f * 0.044715f * u * u);
   return 0.5f * (1.0f + t) + 0.5f * u * (1.0f - t * t) * da;
 }

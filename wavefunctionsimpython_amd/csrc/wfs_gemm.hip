@@ -31,11 +31,9 @@ namespace wfs {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// round to nearest even (gfx950 converts in hardware: v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short f32_to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+  return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 
 struct GemmArgs {
@@ -224,9 +222,249 @@ __global__ __launch_bounds__(256) void gemm_splitk_reduce_kernel(GemmArgs g) {
   if (lane == 0) epilogue(g, b, r / g.N, r % g.N, s);
 }
 
+// ---------------------------------------------------------------------------
+// One layer of the Fourier neural operator, fused over a grid line.  For the line
+// r = (x, y) (all n = 32 NZC points along z) and all channels:
+//   u  = W v + Z Gz + b        (W: C x cin, v: cin channels, Z: the layer's inverse
+//                               spectral path before its last (z) stage, C x tm per line)
+//   v' = GELU(u) or u
+//   T' = v' Fz                 (the next layer's first (z) spectral stage, C x tm per line)
+// which the unfused network computes as four GEMMs and writes S = Z Gz and u to HBM in
+// between.  Operand rounding is that of those GEMMs: W, Gz, Fz plain bf16, v, Z and v'
+// split into bf16 hi + lo; products accumulate in FP32 (in another order).
+//
+// One wavefront per line, no LDS and no barriers.  The transposed product is formed,
+// u^T = v^T W^T + Gz^T Z^T, so that (with the 32 x 32 x 8 matrix-core layout: lane l
+// holds A[l%32][4(l/32)..+3], B[4(l/32)..+3][l%32] and accumulator item i is
+// C[8(i/4)+4(l/32)+i%4][l%32]) every operand comes straight from memory or from the
+// previous product: the A fragment of v^T is four channel rows at one z (each load
+// instruction reads 2 x 128 contiguous bytes), W^T, Gz^T and Fz are per-lane
+// constants held in registers, Z^T is one 16-byte load per k step, and the
+// accumulator of a 32-point z chunk of u^T, items 4q..4q+3, is already the A
+// fragment of v' for k step q of T = v' Fz.  C, cin, tm <= 32.
+// ---------------------------------------------------------------------------
+struct LayerArgs {
+  const float* W;
+  const float* v;
+  const float* Z;      // may be null (no spectral term)
+  const float* bias;
+  const float* Gz;     // [tm][n]
+  const float* Fz;     // [n][tm]
+  float* u;            // may be null
+  float* vn;           // may be null
+  float* T;            // may be null
+  int cin, C, tm, act;
+  long rows;           // lines (n * n)
+};
+
+__device__ __forceinline__ void split4(const float (&x)[4], bf16x4& hi, bf16x4& lo) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const unsigned short h = f32_to_bf16(x[j]);
+    hi[j] = (short)h;
+    lo[j] = (short)f32_to_bf16(x[j] - __uint_as_float(unsigned(h) << 16));
+  }
+}
+
+template <int NZC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fno_layer_kernel(LayerArgs g) {
+  constexpr int n = 32 * NZC;
+  constexpr int GP = kBK + 4;                        // padded rows of the transform tables
+  __shared__ unsigned short GzT[n][GP];              // A = Gz^T: [z][j]
+  __shared__ unsigned short FzT[32][n + 4];          // B = Fz:   [j][z]
+  const int lane = threadIdx.x & 63;
+  const int lc = lane & 31;
+  const int kq = 4 * (lane >> 5);
+  const int C = g.C, cin = g.cin, tm = g.tm;
+  const long rows = g.rows;
+  const unsigned P = unsigned(rows * n);             // C * P * 4 bytes < 2^32 (n <= 128, C <= 32)
+  const int kin = (cin + 7) / 8;                     // k steps of W v
+  const int ktm = g.Z ? (tm + 7) / 8 : 0;            // k steps of Z Gz
+
+  for (int e = threadIdx.x; e < n * 32; e += blockDim.x) {
+    const int z = e / 32, j = e % 32;
+    GzT[z][j] = (g.Z && j < tm) ? f32_to_bf16(g.Gz[j * n + z]) : (unsigned short)0;
+    FzT[j][z] = (g.T && j < tm) ? f32_to_bf16(g.Fz[z * tm + j]) : (unsigned short)0;
+  }
+  bf16x4 Wf[4];                                      // B = W^T: (k = c_in, n = c_out = lc)
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 8 * s + kq + j;
+      Wf[s][j] = (short)((lc < C && k < cin) ? f32_to_bf16(g.W[lc * cin + k]) : 0);
+    }
+  const float bias_c = lc < C ? g.bias[lc] : 0.0f;
+  __syncthreads();
+
+  // 32-bit byte offsets from the buffers' base addresses (C * n^3 * 4 < 2^32); loads past
+  // the operand's extent read a clamped valid element and are zeroed, so no load is
+  // conditional
+  const char* __restrict__ vb = reinterpret_cast<const char*>(g.v);
+  const char* __restrict__ zb = reinterpret_cast<const char*>(g.Z);
+  char* __restrict__ ub = reinterpret_cast<char*>(g.u);
+  char* __restrict__ nb = reinterpret_cast<char*>(g.vn);
+  char* __restrict__ tb = reinterpret_cast<char*>(g.T);
+  const unsigned lcC = unsigned(lc < C ? lc : C - 1);
+  const long stride = long(gridDim.x) * (blockDim.x / 64);
+  // the A fragments of v^T for one 32-point chunk; the next chunk's (or the next line's
+  // first chunk's) are loaded while the current one is computed
+  auto load_v = [&](unsigned ru, int ch, float (&x)[16]) {
+    const unsigned zoff = ru * n + 32 * ch + lc;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = 8 * s + kq + j;
+        x[4 * s + j] = *reinterpret_cast<const float*>(vb + ((c < cin ? unsigned(c) : unsigned(cin - 1)) * P + zoff) * 4u);
+      }
+  };
+  float xv[16];
+  long r = long(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (r < rows) load_v(unsigned(r), 0, xv);
+  for (; r < rows; r += stride) {
+    const unsigned ru = unsigned(r);
+    bf16x4 Zh[4], Zl[4];                             // B = Z^T: (k = j, n = c = lc)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) { Zh[s] = bf16x4{0, 0, 0, 0}; Zl[s] = bf16x4{0, 0, 0, 0}; }
+    if (zb) {
+      // four consecutive j per lane: one 16-byte load (tm is a multiple of 4, so a quad lies
+      // wholly inside or wholly past the row; past it, the row's last quad is read and dropped)
+      const unsigned row = (lcC * unsigned(rows) + ru) * unsigned(tm);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int k0 = 8 * s + kq;
+        const float4 y = *reinterpret_cast<const float4*>(zb + (row + unsigned(k0 < tm ? k0 : tm - 4)) * 4u);
+        const bool ok = k0 < tm && lc < C;
+        float x[4] = {ok ? y.x : 0.0f, ok ? y.y : 0.0f, ok ? y.z : 0.0f, ok ? y.w : 0.0f};
+        split4(x, Zh[s], Zl[s]);
+      }
+    }
+    f32x16 accT;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) accT[i] = 0.0f;
+    const unsigned line = ru * n;
+    const long rnext = r + stride;
+    // chunk ch of this line from x (its loads were issued one chunk earlier)
+    auto process = [&](int ch, float (&x)[16]) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (8 * (i / 4) + kq + (i % 4) >= cin) x[i] = 0.0f;   // channels past cin
+      f32x16 acc;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        if (s < kin) {
+          bf16x4 ah, al;
+          float xs[4] = {x[4 * s], x[4 * s + 1], x[4 * s + 2], x[4 * s + 3]};
+          split4(xs, ah, al);
+          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, Wf[s], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, Wf[s], acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        if (s < ktm) {
+          bf16x4 a;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) a[j] = (short)GzT[32 * ch + lc][8 * s + kq + j];
+          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, Zl[s], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, Zh[s], acc, 0, 0, 0);
+        }
+      }
+      // lane: column c = lc; items 4q..4q+3: z = 32 ch + 8 q + kq + 0..3
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float val[4], gv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          val[j] = acc[4 * q + j] + bias_c;
+          gv[j] = lc < C ? (g.act ? gelu_f(val[j]) : val[j]) : 0.0f;
+        }
+        if (lc < C) {
+          const unsigned off = (unsigned(lc) * P + line + 32 * ch + 8 * q + kq) * 4u;
+          if (ub) *reinterpret_cast<float4*>(ub + off) = make_float4(val[0], val[1], val[2], val[3]);
+          if (nb) *reinterpret_cast<float4*>(nb + off) = make_float4(gv[0], gv[1], gv[2], gv[3]);
+        }
+        if (tb) {
+          bf16x4 ah, al, b;
+          split4(gv, ah, al);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) b[j] = (short)FzT[lc][32 * ch + 8 * q + kq + j];
+          accT = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, b, accT, 0, 0, 0);
+          accT = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, b, accT, 0, 0, 0);
+        }
+      }
+    };
+    // the chunk after ch: this line's ch + 1, or the next line's first
+    auto prefetch = [&](int ch, float (&x)[16]) {
+      if (ch + 1 < NZC) load_v(ru, ch + 1, x);
+      else if (rnext < rows) load_v(unsigned(rnext), 0, x);
+    };
+    // chunks in pairs through two buffers, so that no register copy waits for a load in flight
+    float xw[16];
+#pragma unroll 1
+    for (int ch = 0; ch < NZC; ch += 2) {
+      prefetch(ch, xw);
+      process(ch, xv);
+      if (ch + 1 < NZC) {
+        prefetch(ch + 1, xv);
+        process(ch + 1, xw);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) xv[i] = xw[i];   // odd chunk count: next line's first chunk
+      }
+    }
+    if (tb && lc < tm) {
+      // accumulator: column j = lc, rows c = 8(i/4) + 4(l/32) + i%4
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int c = 8 * (i / 4) + kq + (i % 4);
+        if (c < C)
+          *reinterpret_cast<float*>(tb + ((unsigned(c) * unsigned(rows) + ru) * unsigned(tm) + lc) * 4u) = accT[i];
+      }
+    }
+  }
+}
+
 }  // namespace wfs
 
 using namespace wfs;
+
+// layer of the Fourier neural operator fused over grid lines (see fno_layer_kernel)
+extern "C" int wfs_fno_layer(const float* W, const float* v, const float* Z, const float* bias, const float* Gz,
+                             const float* Fz, float* u, float* vn, float* T, int cin, int C, int n, int tm, int act,
+                             void* stream) {
+  if (!W || !v || !bias) { set_last_error("wfs_fno_layer: null argument"); return WFS_ERR_INVALID; }
+  if (n % 32 != 0 || n < 32 || n > 128 || C < 1 || C > 32 || cin < 1 || cin > 32 || tm < 4 || tm > 32 || tm % 4) {
+    set_last_error("wfs_fno_layer: needs n in {32, 64, 96, 128}, C, cin <= 32 and tm a multiple of 4 up to 32 "
+                   "(n=%d C=%d cin=%d tm=%d)", n, C, cin, tm);
+    return WFS_ERR_UNSUPPORTED;
+  }
+  if ((Z && !Gz) || (T && (!Fz || !vn)) || (!u && !vn)) {
+    set_last_error("wfs_fno_layer: Z needs Gz; T needs Fz and vn; one of u, vn is required");
+    return WFS_ERR_INVALID;
+  }
+  if ((reinterpret_cast<unsigned long>(u) % 16) != 0 || (reinterpret_cast<unsigned long>(vn) % 16) != 0 ||
+      (reinterpret_cast<unsigned long>(Z) % 16) != 0) {
+    set_last_error("wfs_fno_layer: u, vn and Z must be 16-byte aligned");
+    return WFS_ERR_INVALID;
+  }
+  LayerArgs g{W, v, Z, bias, Gz, Fz, u, vn, T, cin, C, tm, act, long(n) * n};
+  // one wavefront per line, four per block
+  const long want = (g.rows + 3) / 4;
+  const unsigned blocks = unsigned(want < 2048 ? want : 2048);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (n / 32) {
+    case 1: hipLaunchKernelGGL(fno_layer_kernel<1>, dim3(blocks), dim3(256), 0, s, g); break;
+    case 2: hipLaunchKernelGGL(fno_layer_kernel<2>, dim3(blocks), dim3(256), 0, s, g); break;
+    case 3: hipLaunchKernelGGL(fno_layer_kernel<3>, dim3(blocks), dim3(256), 0, s, g); break;
+    default: hipLaunchKernelGGL(fno_layer_kernel<4>, dim3(blocks), dim3(256), 0, s, g); break;
+  }
+  WFS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
 
 template <int WM, int WN>
 static void launch_tile(dim3 grid, hipStream_t s, const GemmArgs& g) {

@@ -74,7 +74,8 @@ class FNO3d:
 
     def __init__(self, n: int, width: int = 32, modes: int = 12, layers: int = 4, in_channels: int = 3,
                  out_channels: int = 2, residual: bool = True, seed: int = 0, device=None, splits: int = 2048,
-                 output_scale: float = 0.1, residual_scale: float = 1.0, split_activations: bool = True):
+                 output_scale: float = 0.1, residual_scale: float = 1.0, split_activations: bool = True,
+                 fused: bool = True):
         if modes % 2 or modes > n:
             raise ValueError("modes must be even and at most n")
         self.n, self.C, self.m, self.L = int(n), int(width), int(modes), int(layers)
@@ -89,6 +90,9 @@ class FNO3d:
         # activations enter the matrix cores as bf16 hi + lo parts, so the network is a smooth
         # function of its input to about 16 bits (parareal differences G(U') - G(U) need that)
         self.split_activations = bool(split_activations)
+        # layers as one fused kernel per grid line (wfs_fno_layer) where the sizes allow
+        self.fused = (bool(fused) and self.split_activations and self.n % 32 == 0 and 32 <= self.n <= 128
+                      and self.C <= 32 and self.cin <= 32 and 2 * self.m <= 32 and self.m % 2 == 0)
         rng = np.random.default_rng(seed)
         C, m3 = self.C, self.m ** 3
 
@@ -148,10 +152,20 @@ class FNO3d:
         """S[c][p] = Re IDFT_m(R_l (.) DFT_m(v)) (written, not accumulated)."""
         ws = self._workspace()
         C, n, m = self.C, self.n, self.m
+        tm = 2 * m
+        tb = self.tab
+        self._gemm(v, tb["Fz"], ws["T"], 1, C * n * n, tm, n, (0, n, 1), (0, tm, 1), (0, tm, 1), flags=SA)
+        self._spectral_mid(l)
+        self._gemm(ws["Z"], tb["Gz"], S, 1, C * n * n, n, tm, (0, tm, 1), (0, n, 1), (0, n, 1), flags=SA)
+
+    def _spectral_mid(self, l: int) -> None:
+        """Z = the spectral path of layer l between its z stages: T (the z-stage DFT of v, in the
+        workspace) -> y and x DFTs -> R_l -> inverse x and y DFTs -> Z (workspace)."""
+        ws = self._workspace()
+        C, n, m = self.C, self.n, self.m
         m2, m3, tm = m * m, m ** 3, 2 * m
         tb = self.tab
         T, U, X, O, Y, Z = ws["T"], ws["U"], ws["X"], ws["O"], ws["Y"], ws["Z"]
-        self._gemm(v, tb["Fz"], T, 1, C * n * n, tm, n, (0, n, 1), (0, tm, 1), (0, tm, 1), flags=SA)
         self._gemm(tb["F2"], T, U, C * n, tm, m, 2 * n, (0, 2 * n, 1), (2 * n * m, m, 1), (tm * m, m, 1), flags=SB)
         self._gemm(tb["F2"], U, X, C, tm, m2, 2 * n, (0, 2 * n, 1), (2 * n * m2, m2, 1), (tm * m2, m2, 1), flags=SB)
         Rr, Ri = self.params[f"Rr{l}"], self.params[f"Ri{l}"]
@@ -163,7 +177,11 @@ class FNO3d:
         self._gemm(Rr, Xi, Oi, m3, C, 1, C, sR, sX, sX, beta=1.0, flags=SB)
         self._gemm(tb["G2"], O, Y, C, 2 * n, m2, tm, (0, tm, 1), (tm * m2, m2, 1), (2 * n * m2, m2, 1), flags=SB)
         self._gemm(tb["G2"], Y, Z, C * n, 2 * n, m, tm, (0, tm, 1), (tm * m, m, 1), (2 * n * m, m, 1), flags=SB)
-        self._gemm(Z, tb["Gz"], S, 1, C * n * n, n, tm, (0, tm, 1), (0, n, 1), (0, n, 1), flags=SA)
+
+    def _layer(self, W, v, Z, bias, u, vn, T, cin: int, act: bool) -> None:
+        """u = W v + Z Gz + bias, vn = GELU(u) or u, T = vn Fz in one pass over the grid lines."""
+        _abi.fno_layer(W, v, Z, bias, self.tab["Gz"], self.tab["Fz"], u, vn, T, cin, self.C, self.n, 2 * self.m,
+                       act)
 
     def _spectral_backward(self, dS: torch.Tensor, l: int, v: torch.Tensor, dv: torch.Tensor) -> None:
         """dv += d/dv of the spectral path given dS; accumulates dR_l (needs X of this layer)."""
@@ -224,21 +242,36 @@ class FNO3d:
         saved_v: List[torch.Tensor] = []
         saved_u: List[torch.Tensor] = []
         v = torch.empty(C * P, dtype=self.dtype, device=self.device)
-        self._gemm(pr["P"], a, v, 1, C, P, self.cin, (0, self.cin, 1), (0, P, 1), (0, P, 1), flags=SB,
-                   bias=pr["pb"])
-        for l in range(self.L):
-            u = torch.empty(C * P, dtype=self.dtype, device=self.device)
-            self._spectral(v, l, u)
-            last = l == self.L - 1
-            vn = None if last else torch.empty(C * P, dtype=self.dtype, device=self.device)
-            # u = W v + S + b and v' = GELU(u) in the GEMM's epilogue; u is stored only when
-            # the backward pass needs it (or when it is the network's last activation)
-            self._gemm(pr[f"W{l}"], v, u, 1, C, P, C, (0, C, 1), (0, P, 1), (0, P, 1), beta=1.0,
-                       flags=SB | (NO_C if (not keep and not last) else 0), bias=pr[f"b{l}"], act=vn)
-            if keep:
-                saved_v.append(v)
-                saved_u.append(u)
-            v = u if last else vn
+        if self.fused:
+            # lift and the first z-stage DFT, then per layer the middle of the spectral path and one
+            # fused pass: u = W v + (last spectral stage) + b, v' = GELU(u), next z-stage DFT
+            self._layer(pr["P"], a, None, pr["pb"], None, v, ws["T"], self.cin, act=False)
+            for l in range(self.L):
+                self._spectral_mid(l)
+                last = l == self.L - 1
+                u = torch.empty(C * P, dtype=self.dtype, device=self.device) if (keep or last) else None
+                vn = None if last else torch.empty(C * P, dtype=self.dtype, device=self.device)
+                self._layer(pr[f"W{l}"], v, ws["Z"], pr[f"b{l}"], u, vn, None if last else ws["T"], C, act=True)
+                if keep:
+                    saved_v.append(v)
+                    saved_u.append(u)
+                v = u if last else vn
+        else:
+            self._gemm(pr["P"], a, v, 1, C, P, self.cin, (0, self.cin, 1), (0, P, 1), (0, P, 1), flags=SB,
+                       bias=pr["pb"])
+            for l in range(self.L):
+                u = torch.empty(C * P, dtype=self.dtype, device=self.device)
+                self._spectral(v, l, u)
+                last = l == self.L - 1
+                vn = None if last else torch.empty(C * P, dtype=self.dtype, device=self.device)
+                # u = W v + S + b and v' = GELU(u) in the GEMM's epilogue; u is stored only when
+                # the backward pass needs it (or when it is the network's last activation)
+                self._gemm(pr[f"W{l}"], v, u, 1, C, P, C, (0, C, 1), (0, P, 1), (0, P, 1), beta=1.0,
+                           flags=SB | (NO_C if (not keep and not last) else 0), bias=pr[f"b{l}"], act=vn)
+                if keep:
+                    saved_v.append(v)
+                    saved_u.append(u)
+                v = u if last else vn
         out = ws["out"]
         self._gemm(pr["Q"], v, out, 1, self.cout, P, C, (0, C, 1), (0, P, 1), (0, P, 1), flags=SB,
                    bias=pr["qb"])
@@ -302,9 +335,14 @@ class FNO3d:
         flops *= L
         flops += 2.0 * 2 * (C * P * self.cin + self.cout * P * C)
         act = 4.0 * C * P
-        # z-stage reads v; inverse z writes S; W-GEMM reads v and S and writes GELU(u) (u too at
-        # the last layer); the small truncated spectra are negligible
-        bytes_ = L * 5 * act + 4.0 * P * (self.cin + C) + 4.0 * P * (C + self.cout)
+        if self.fused:
+            # per layer one pass reads v and writes GELU(u) (u at the last layer); the lift reads a
+            # and writes v0; the projection reads v_L and writes out; truncated spectra negligible
+            bytes_ = L * 2 * act + 4.0 * P * (self.cin + C) + 4.0 * P * (C + self.cout)
+        else:
+            # z-stage reads v; inverse z writes S; W-GEMM reads v and S and writes GELU(u) (u too
+            # at the last layer); the small truncated spectra are negligible
+            bytes_ = L * 5 * act + 4.0 * P * (self.cin + C) + 4.0 * P * (C + self.cout)
         return {"flops": flops, "hbm_bytes": bytes_}
 
     def zero_grad(self) -> None:
