@@ -275,6 +275,13 @@ int wfs_gemm_bf16(const float* A, const float* B, float* C, const long dims[4], 
  */
 int wfs_fno_bias_act(float* u, const float* bias, float* v, long C, long P, int act, void* stream);
 /*
+ * wfs_fno_modes (no reference counterpart): complex per-mode channel mixing
+ * O[o][q] = sum_i R[q][i][o] X[i][q] over m3 modes, R [m3][C][C] as separate real and
+ * imaginary arrays, X and O [C][2][m3] (real parts, then imaginary parts); R rounded to
+ * bf16, X as bf16 hi + lo, FP32 accumulation (the operand rounding of wfs_gemm_bf16).
+ */
+int wfs_fno_modes(const float* Rr, const float* Ri, const float* X, float* O, int m3, int C, void* stream);
+/*
  * wfs_fno_layer (no reference counterpart): one network layer fused over the grid lines
  * (x, y) of an n^3 grid, all channels:  u = W v + Z Gz + b;  vn = GELU(u) (act) or u;
  * T = vn Fz.  W [C][cin], v [cin][n^3], Z [C][n^2][tm] (the layer's inverse spectral path

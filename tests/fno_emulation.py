@@ -57,6 +57,16 @@ class EmulatedFNO(FNO3d):
         if not flags & 4:
             Cv.copy_(r)
 
+    def _modes(self, Rr, Ri, X, O):
+        """The per-mode complex mixing as the four GEMMs it replaces (same operand splits)."""
+        C, m3 = self.C, self.m ** 3
+        Xr, Xi, Or, Oi = X, X[m3:], O, O[m3:]
+        sR, sX = (C * C, 1, C), (1, 2 * m3, 0)
+        self._gemm(Rr, Xr, Or, m3, C, 1, C, sR, sX, sX, flags=2)
+        self._gemm(Ri, Xi, Or, m3, C, 1, C, sR, sX, sX, alpha=-1.0, beta=1.0, flags=2)
+        self._gemm(Ri, Xr, Oi, m3, C, 1, C, sR, sX, sX, flags=2)
+        self._gemm(Rr, Xi, Oi, m3, C, 1, C, sR, sX, sX, beta=1.0, flags=2)
+
     def _layer(self, W, v, Z, bias, u, vn, T, cin, act):
         """The fused layer as the GEMM composition it replaces: u = W v + Z Gz + b,
         vn = GELU(u) or u, T = vn Fz (same operand splits)."""

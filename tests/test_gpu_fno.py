@@ -165,6 +165,36 @@ def test_fused_layers_match_gemm_composition(dev, n, width, modes, layers):
     assert e_inf < 2e-5 and e_keep < 1e-6 and e_saved < 2e-5 and e_grad < 1e-3
 
 
+def test_mode_mixing_kernel_matches_gemms(dev):
+    """wfs_fno_modes against the four strided GEMMs it replaces (same bf16 operand rounding,
+    FP32 accumulation in another order) and against float64 on the rounded operands."""
+    C, m3 = 32, 1728
+    rng = np.random.default_rng(8)
+    Rr, Ri = (torch.from_numpy(rng.standard_normal(m3 * C * C).astype(np.float32) / C).to(dev) for _ in range(2))
+    X = torch.from_numpy(rng.standard_normal(2 * C * m3).astype(np.float32)).to(dev)
+    O = torch.empty_like(X)
+    _abi.fno_modes(Rr, Ri, X, O, m3, C)
+    O2 = torch.empty_like(X)
+    Xr, Xi, Or, Oi = X, X[m3:], O2, O2[m3:]
+    sR, sX = (C * C, 1, C), (1, 2 * m3, 0)
+    _abi.gemm_bf16(Rr, Xr, Or, m3, C, 1, C, sR, sX, sX, flags=_abi.GEMM_SPLIT_B)
+    _abi.gemm_bf16(Ri, Xi, Or, m3, C, 1, C, sR, sX, sX, alpha=-1.0, beta=1.0, flags=_abi.GEMM_SPLIT_B)
+    _abi.gemm_bf16(Ri, Xr, Oi, m3, C, 1, C, sR, sX, sX, flags=_abi.GEMM_SPLIT_B)
+    _abi.gemm_bf16(Rr, Xi, Oi, m3, C, 1, C, sR, sX, sX, beta=1.0, flags=_abi.GEMM_SPLIT_B)
+    torch.cuda.synchronize()
+    R = (_np_bf16(Rr.cpu().numpy()) + 1j * _np_bf16(Ri.cpu().numpy())).reshape(m3, C, C)
+    xn = X.cpu().numpy().reshape(C, 2, m3)
+    xs = [_np_bf16(p) + _np_bf16(p - _np_bf16(p)) for p in (xn[:, 0], xn[:, 1])]
+    ref = np.einsum("qio,iq->oq", R, xs[0] + 1j * xs[1])
+    got = O.cpu().numpy().reshape(C, 2, m3)
+    gemm = O2.cpu().numpy().reshape(C, 2, m3)
+    scale = np.abs(ref).max()
+    e_ref = max(np.abs(got[:, 0] - ref.real).max(), np.abs(got[:, 1] - ref.imag).max()) / scale
+    e_gemm = np.abs(got - gemm).max() / scale
+    print(f"mode mixing: vs float64 on rounded operands {e_ref:.2e}, vs GEMM path {e_gemm:.2e}")
+    assert e_ref < 1e-6 and e_gemm < 1e-6
+
+
 def test_fused_layer_limits(dev):
     assert not FNO3d(16, width=8, modes=4, layers=1).fused          # n not a multiple of 32
     assert not FNO3d(32, width=40, modes=4, layers=1).fused         # width above 32

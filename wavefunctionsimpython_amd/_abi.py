@@ -50,6 +50,7 @@ SIGNATURES = {
     "wfs_fno_gelu_backward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, _vp]),
     "wfs_fno_axpby": (ctypes.c_int, [_vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_long, _vp]),
     "wfs_fno_layer": (ctypes.c_int, [_vp] * 9 + [ctypes.c_int] * 5 + [_vp]),
+    "wfs_fno_modes": (ctypes.c_int, [_vp] * 4 + [ctypes.c_int] * 2 + [_vp]),
     "wfs_fno_pack": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_float, ctypes.c_float, _vp, ctypes.c_long,
                                     _vp]),
     "wfs_fno_unpack": (ctypes.c_int, [_vp, ctypes.c_float, _vp, ctypes.c_long, _vp]),
@@ -414,6 +415,19 @@ def fno_layer(W: torch.Tensor, v: torch.Tensor, Z: torch.Tensor | None, bias: to
                               ptr(u) if u is not None else None, ptr(vn) if vn is not None else None,
                               ptr(T) if T is not None else None, int(cin), int(C), int(n), int(tm),
                               1 if act else 0, stream_handle(stream)), "wfs_fno_layer")
+
+
+def fno_modes(Rr: torch.Tensor, Ri: torch.Tensor, X: torch.Tensor, O: torch.Tensor, m3: int, C: int,
+              stream=None) -> None:
+    """Complex per-mode channel mixing O[o][q] = sum_i R[q][i][o] X[i][q] (``wfs_fno_modes``):
+    R [m3][C][C] real and imaginary parts, X and O [C][2][m3]; float32, contiguous."""
+    for t, size, name in ((Rr, m3 * C * C, "Rr"), (Ri, m3 * C * C, "Ri"), (X, 2 * C * m3, "X"),
+                          (O, 2 * C * m3, "O")):
+        _require_device_any(t, name)
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() < size:
+            raise ValueError(f"{name} must be a contiguous float32 tensor of at least {size} elements")
+    check(lib().wfs_fno_modes(ptr(Rr), ptr(Ri), ptr(X), ptr(O), int(m3), int(C), stream_handle(stream)),
+          "wfs_fno_modes")
 
 
 def _require_device_any(t: torch.Tensor, name: str) -> None:

@@ -170,13 +170,22 @@ class FNO3d:
         self._gemm(tb["F2"], U, X, C, tm, m2, 2 * n, (0, 2 * n, 1), (2 * n * m2, m2, 1), (tm * m2, m2, 1), flags=SB)
         Rr, Ri = self.params[f"Rr{l}"], self.params[f"Ri{l}"]
         Xr, Xi, Or, Oi = X, X[m3:], O, O[m3:]
+        self._modes(Rr, Ri, X, O)
+        self._gemm(tb["G2"], O, Y, C, 2 * n, m2, tm, (0, tm, 1), (tm * m2, m2, 1), (2 * n * m2, m2, 1), flags=SB)
+        self._gemm(tb["G2"], Y, Z, C * n, 2 * n, m, tm, (0, tm, 1), (tm * m, m, 1), (2 * n * m, m, 1), flags=SB)
+
+    def _modes(self, Rr, Ri, X, O) -> None:
+        """O = R (.) X per mode, complex (Or = Rr Xr - Ri Xi, Oi = Ri Xr + Rr Xi); X, O [C][2][m^3]."""
+        C, m3 = self.C, self.m ** 3
+        if self.split_activations:
+            _abi.fno_modes(Rr, Ri, X, O, m3, C)
+            return
+        Xr, Xi, Or, Oi = X, X[m3:], O, O[m3:]
         sR, sX = (C * C, 1, C), (1, 2 * m3, 0)
         self._gemm(Rr, Xr, Or, m3, C, 1, C, sR, sX, sX, flags=SB)
         self._gemm(Ri, Xi, Or, m3, C, 1, C, sR, sX, sX, alpha=-1.0, beta=1.0, flags=SB)
         self._gemm(Ri, Xr, Oi, m3, C, 1, C, sR, sX, sX, flags=SB)
         self._gemm(Rr, Xi, Oi, m3, C, 1, C, sR, sX, sX, beta=1.0, flags=SB)
-        self._gemm(tb["G2"], O, Y, C, 2 * n, m2, tm, (0, tm, 1), (tm * m2, m2, 1), (2 * n * m2, m2, 1), flags=SB)
-        self._gemm(tb["G2"], Y, Z, C * n, 2 * n, m, tm, (0, tm, 1), (tm * m, m, 1), (2 * n * m, m, 1), flags=SB)
 
     def _layer(self, W, v, Z, bias, u, vn, T, cin: int, act: bool) -> None:
         """u = W v + Z Gz + bias, vn = GELU(u) or u, T = vn Fz in one pass over the grid lines."""
