@@ -433,20 +433,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
 //   O[o][q] = sum_i R[q][i][o] X[i][q]     (complex: Or = Rr Xr - Ri Xi, Oi = Ri Xr + Rr Xi)
 // for m3 modes q and C channels, with the GEMM path's operand rounding (R plain bf16,
 // X as bf16 hi + lo, FP32 accumulation).  The GEMM path issues it as four strided
-// batched GEMMs with N = 1 (one 32 x 32 tile per mode for one useful column); here a
-// thread owns one (mode, output channel) pair.  Layouts: R [m3][C][C] ([q][i][o]),
+// batched GEMMs with N = 1 (one 32 x 32 tile per mode for one useful column); here four
+// lanes own one (mode, output channel) pair.  Layouts: R [m3][C][C] ([q][i][o]),
 // X and O [C][2][m3] (real parts, then imaginary parts).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void fno_modes_kernel(const float* __restrict__ Rr, const float* __restrict__ Ri,
                                                         const float* __restrict__ X, float* __restrict__ O,
                                                         int m3, int C) {
+  // four lanes per (mode, output channel): lane h sums input channels i = h, h + 4, ...,
+  // then the four partial sums are added across the lanes (xor 1, xor 2)
   const long t = long(blockIdx.x) * 256 + threadIdx.x;
-  if (t >= long(m3) * C) return;
-  const int q = int(t / C), o = int(t % C);
+  const long pair = t >> 2;
+  const int h = int(t & 3);
+  const bool live = pair < long(m3) * C;
+  const long pp = live ? pair : 0;
+  const int q = int(pp / C), o = int(pp % C);
   const float* __restrict__ rr = Rr + long(q) * C * C + o;
   const float* __restrict__ ri = Ri + long(q) * C * C + o;
   float srr = 0.0f, sii = 0.0f, sir = 0.0f, sri = 0.0f;   // Rr Xr, Ri Xi, Ri Xr, Rr Xi
-  for (int i = 0; i < C; ++i) {
+  for (int i = h; i < C; i += 4) {
     const float a = __uint_as_float(unsigned(f32_to_bf16(rr[long(i) * C])) << 16);
     const float b = __uint_as_float(unsigned(f32_to_bf16(ri[long(i) * C])) << 16);
     const float xr = X[long(i) * 2 * m3 + q], xi = X[long(i) * 2 * m3 + m3 + q];
@@ -459,8 +464,17 @@ __global__ __launch_bounds__(256) void fno_modes_kernel(const float* __restrict_
     sir = __builtin_fmaf(b, xrh, __builtin_fmaf(b, xrl, sir));
     sri = __builtin_fmaf(a, xih, __builtin_fmaf(a, xil, sri));
   }
-  O[long(o) * 2 * m3 + q] = srr - sii;
-  O[long(o) * 2 * m3 + m3 + q] = sir + sri;
+#pragma unroll
+  for (int off = 1; off < 4; off <<= 1) {
+    srr += __shfl_xor(srr, off, 64);
+    sii += __shfl_xor(sii, off, 64);
+    sir += __shfl_xor(sir, off, 64);
+    sri += __shfl_xor(sri, off, 64);
+  }
+  if (live && h == 0) {
+    O[long(o) * 2 * m3 + q] = srr - sii;
+    O[long(o) * 2 * m3 + m3 + q] = sir + sri;
+  }
 }
 
 }  // namespace wfs
@@ -470,7 +484,7 @@ using namespace wfs;
 extern "C" int wfs_fno_modes(const float* Rr, const float* Ri, const float* X, float* O, int m3, int C,
                              void* stream) {
   if (!Rr || !Ri || !X || !O || m3 < 1 || C < 1) { set_last_error("wfs_fno_modes: bad argument"); return WFS_ERR_INVALID; }
-  const long total = long(m3) * C;
+  const long total = long(m3) * C * 4;
   hipLaunchKernelGGL(fno_modes_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), Rr, Ri, X, O, m3, C);
   WFS_HIP_CHECK(hipGetLastError());
