@@ -214,7 +214,10 @@ int wfs_potential_operator(long count, const void* V, double dt, int precision, 
 #define WFS_K_HORNER 8
 #define WFS_K_OBSERVABLES 9
 #define WFS_K_YZ_PLANE 10
-#define WFS_K_COUNT 11
+#define WFS_K_YZ_SLAB 11
+#define WFS_K_X_SLAB 12
+#define WFS_K_SLAB_EDGE 13
+#define WFS_K_COUNT 14
 int wfs_plan_set_profiling(wfs_plan* plan, int enable);
 
 /*
@@ -230,6 +233,14 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * inverse / potential / z forward and y forward passes of each step as one
  * kernel on whole x planes held in LDS; variant 0 runs the three passes.  The
  * results are identical; the autotuner keeps the faster.
+ * WFS_K_YZ_SLAB: variant 1 (default where supported: complex64 grids with
+ * ny = nz = 256 and nx <= 256) runs a step as two launches, the y transform
+ * split in a length-64 stage fused with the z work (kernel class
+ * WFS_K_YZ_SLAB; the entry and exit launches of a call count as
+ * WFS_K_SLAB_EDGE) and a length-4 stage fused with the x pass (WFS_K_X_SLAB);
+ * variant 0 runs the four axis passes.  The results agree to float rounding
+ * (the length-256 y transform is factored differently); the autotuner keeps
+ * the faster.
  */
 int wfs_plan_autotune(wfs_plan* plan, void* stream);
 int wfs_plan_set_variant(wfs_plan* plan, int kernel_class, int variant);

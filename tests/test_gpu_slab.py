@@ -1,0 +1,236 @@
+"""Two-launch split step for complex64 grids with ny = nz = 256 (the slab scheme).
+
+The length-256 y transform is split in a length-64 stage on the rows y = 4 a + b (fused with
+the z inverse / potential / z forward work) and a length-4 stage fused with the x pass.  Its
+arithmetic differs from the four axis passes only in how the y transform is factored, so the
+two paths agree to float rounding; both are pinned to the FP64 oracle at the complex64 bar.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from wavefunctionsimpython_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+C64, C128 = _abi.WFS_C64, _abi.WFS_C128
+DX = (0.3, 0.25, 0.35)
+DT = 0.01
+TOL_ORACLE = 1e-5     # complex64 state vs the FP64 oracle, relative L2
+TOL_PASSES = 2e-6     # slab scheme vs the four axis passes, relative L2
+
+
+def _inputs(shape, seed):
+    """a smooth packet with a random phase ripple and a complex potential (CAP-like imaginary part)"""
+    rng = np.random.default_rng(seed)
+    psi = oracle.gaussian_wave_packet(shape, DX, (1.2, 6.0, 8.0), (0.0, 1.0, -2.0), (1.0, 0.5, -0.8),
+                                      np.complex128)
+    psi = psi * np.exp(1j * 0.3 * rng.standard_normal(shape))
+    V = (rng.standard_normal(shape) * 0.5 + 1j * rng.uniform(-0.05, 0.0, shape)).astype(np.complex64)
+    return psi, V
+
+
+# (yz_slab, x_slab) variants: one block per row group / persistent; 16 / 8 z points per x block
+FORMS = [(1, 1), (2, 2)]
+
+
+def _run(shape, slab, psi0, V=None, axes=None, batch=1, steps=5, dev="cuda:0"):
+    """slab: False (four axis passes) or a (yz_slab, x_slab) variant pair"""
+    p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER, batch=batch)
+    yz, xv = slab if slab else (0, 1)
+    p.set_variant("yz_slab", yz)
+    p.set_variant("x_slab", xv)
+    psi = torch.as_tensor(psi0).to(dev).to(torch.complex64)
+    if axes is not None:
+        p.set_potential_axes(*axes)
+        p.step(psi, None, steps)
+    else:
+        p.step(psi, torch.as_tensor(V).to(dev), steps)
+    torch.cuda.synchronize()
+    return psi
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("nx", [8, 16, 32])
+def test_slab_matches_oracle_table_potential(dev, nx, form):
+    shape = (nx, 256, 256)
+    psi0, V = _inputs(shape, 100 + nx)
+    print(f"case: slab {form}, table potential, shape {shape}, 5 steps", flush=True)
+    got = _run(shape, form, psi0, V).cpu().numpy()
+    ref = oracle.split_step(psi0, V, DX, DT, 1.0, nsteps=5)
+    err = _rel(got, ref)
+    ref_passes = _run(shape, False, psi0, V).cpu().numpy()
+    err_passes = _rel(got, ref_passes)
+    print(f"  rel L2 vs oracle {err:.3e}, vs four passes {err_passes:.3e}, "
+          f"four passes vs oracle {_rel(ref_passes, ref):.3e}")
+    assert err < TOL_ORACLE
+    assert err_passes < TOL_PASSES
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_slab_matches_oracle_separable_potential(dev, form):
+    shape = (16, 256, 256)
+    psi0, _ = _inputs(shape, 7)
+    rng = np.random.default_rng(8)
+    axes = [rng.uniform(-1.0, 1.0, (1, m)) for m in shape]
+    V = (axes[0][0][:, None, None] + axes[1][0][None, :, None] + axes[2][0][None, None, :])
+    print(f"case: slab {form}, separable potential, shape {shape}, 6 steps", flush=True)
+    got = _run(shape, form, psi0, axes=axes, steps=6).cpu().numpy()
+    ref = oracle.split_step(psi0, V, DX, DT, 1.0, nsteps=6)
+    err = _rel(got, ref)
+    print(f"  rel L2 vs oracle {err:.3e}")
+    assert err < TOL_ORACLE
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_slab_single_step_and_entry_exit(dev, form):
+    """one step: entry, one y-z launch with its x launch, exit"""
+    shape = (8, 256, 256)
+    psi0, V = _inputs(shape, 3)
+    got = _run(shape, form, psi0, V, steps=1).cpu().numpy()
+    ref = oracle.split_step(psi0, V, DX, DT, 1.0, nsteps=1)
+    err = _rel(got, ref)
+    print(f"case: slab, 1 step, shape {shape}: rel L2 vs oracle {err:.3e}")
+    assert err < TOL_ORACLE
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_slab_batched_bit_identical_to_single_runs(dev, form):
+    batch = 3
+    shape = (8, 256, 256)
+    rng = np.random.default_rng(21)
+    axes = [rng.uniform(-1.0, 1.0, (batch, m)) for m in shape]
+    psis = [_inputs(shape, 30 + i)[0] for i in range(batch)]
+    stacked = np.stack(psis)
+    print(f"case: slab {form}, separable batched, batch {batch}, shape {shape}", flush=True)
+    both = _run(shape, form, stacked, axes=axes, batch=batch)
+    for i in range(batch):
+        one = _run(shape, form, psis[i], axes=[a[i:i + 1] for a in axes])
+        assert torch.equal(both[i], one), f"packet {i}"
+
+
+def test_slab_full_grid_against_passes_and_norm(dev):
+    """256^3 (the benchmark grid): no absorber, so the norm is kept; slab vs four passes"""
+    shape = (256, 256, 256)
+    psi0 = oracle.gaussian_wave_packet(shape, (80 / 256,) * 3, (4.0, 4.0, 4.0), (-16.0, 0.0, 0.0),
+                                       (2.0, 0.0, 0.0), np.complex64)
+    rng = np.random.default_rng(5)
+    V = (rng.uniform(0.0, 1.0, shape)).astype(np.complex64)
+    p_dx = (80 / 256,) * 3
+    out = {}
+    for form in [(0, 1)] + FORMS:
+        p = _abi.Plan(shape, p_dx, 0.005, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+        p.set_variant("yz_slab", form[0])
+        p.set_variant("x_slab", form[1])
+        psi = torch.from_numpy(psi0).to(dev)
+        p.step(psi, torch.from_numpy(V).to(dev), 20)
+        torch.cuda.synchronize()
+        out[form] = psi
+    a = out[(0, 1)]
+    n0 = float(np.sum(np.abs(psi0.astype(np.complex128)) ** 2))
+    for form in FORMS:
+        b = out[form]
+        rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a))
+        n1 = float(torch.sum(torch.abs(b.to(torch.complex128)) ** 2))
+        print(f"case: 256^3 x 20 steps, slab {form}: vs passes rel L2 {rel:.3e}, norm drift {abs(n1 / n0 - 1):.3e}")
+        assert rel < TOL_PASSES
+        assert abs(n1 / n0 - 1) < 1e-5
+    # the forms differ only in how blocks are scheduled: identical results
+    assert torch.equal(out[FORMS[0]], out[FORMS[1]])
+
+
+def test_slab_defaults_and_limits(dev):
+    def variant(shape, precision=C64, batch=1):
+        return _abi.Plan(shape, DX, DT, 1.0, precision, _abi.WFS_METHOD_FOURIER, batch=batch).variants()["yz_slab"]
+    assert variant((256, 256, 256)) == 1
+    assert variant((16, 256, 256), batch=4) == 1
+    assert variant((256, 256, 256), C128) == 0
+    assert variant((512, 256, 256)) == 0
+    assert variant((256, 128, 128)) == 0
+    assert variant((256, 256, 128)) == 0
+    p = _abi.Plan((64, 128, 128), DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    with pytest.raises(_abi.WfsError):
+        p.set_variant("yz_slab", 1)
+    q = _abi.Plan((16, 256, 256), DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    with pytest.raises(_abi.WfsError):
+        q.set_variant("yz_slab", 3)
+    q.set_variant("yz_slab", 0)
+    assert q.variants()["yz_slab"] == 0
+
+
+def test_slab_autotune_and_profile_classes(dev):
+    shape = (32, 256, 256)
+    psi0, V = _inputs(shape, 44)
+    p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    chosen = p.autotune()
+    print(f"autotuned yz_slab at {shape}: {chosen['yz_slab']}")
+    assert chosen["yz_slab"] in (0, 1, 2) and chosen["x_slab"] in (1, 2)
+    p.set_variant("yz_slab", 1)
+    p.set_profiling(True)
+    psi = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+    p.step(psi, torch.from_numpy(V).to(dev), 4)
+    prof = p.profile(reset=True)
+    p.set_profiling(False)
+    assert prof["yz_slab"][0] == 4 and prof["x_slab"][0] == 5 and prof["slab_edge"][0] == 2
+    for k in ("y_inv", "y_fwd", "z_inv_v_fwd", "x_fwd_k_inv"):
+        assert prof.get(k, (0, 0.0))[0] == 0
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_slab_graph_replays_match_eager(dev, form):
+    shape = (16, 256, 256)
+    psi0, V = _inputs(shape, 55)
+    Vd = torch.from_numpy(V).to(dev)
+    eager_plan = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    graph_plan = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    assert eager_plan.variants()["yz_slab"] == 1
+    for pl in (eager_plan, graph_plan):
+        pl.set_variant("yz_slab", form[0])
+        pl.set_variant("x_slab", form[1])
+    eager_plan.set_potential(Vd)
+    graph_plan.set_potential(Vd)
+    eager = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+    graphed = eager.clone()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with _abi.Graph(s) as g:
+        graph_plan.step(graphed, None, 3, stream=s)
+    for replay in range(3):
+        eager_plan.step(eager, None, 3)
+        torch.cuda.synchronize()
+        g.launch()
+        s.synchronize()
+        junk = torch.ones(1000, device=dev).sum().item()
+        assert junk == 1000.0
+        assert torch.equal(eager, graphed), f"replay {replay}"
+
+
+def test_potential_table_order_follows_the_scheme(dev):
+    """The plan stores the phase table in the order its scheme reads (C order for the axis
+    passes, lines order for the slab launch) and rebuilds the other order on a switch: a run
+    that switches after set_potential equals, bit for bit, a run that never switched."""
+    shape = (8, 256, 256)
+    psi0, V = _inputs(shape, 66)
+    Vd = torch.from_numpy(V).to(dev)
+    for first, then in ((0, 1), (1, 0), (2, 0), (0, 2)):
+        switched = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+        switched.set_variant("yz_slab", first)
+        switched.set_potential(Vd)
+        switched.set_variant("yz_slab", then)
+        direct = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+        direct.set_variant("yz_slab", then)
+        direct.set_potential(Vd)
+        a = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+        b = a.clone()
+        switched.step(a, None, 3)
+        direct.step(b, None, 3)
+        torch.cuda.synchronize()
+        print(f"case: table set under yz_slab {first}, stepped under {then}", flush=True)
+        assert torch.equal(a, b)

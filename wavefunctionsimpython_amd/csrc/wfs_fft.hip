@@ -16,7 +16,7 @@
 // where K holds the kinetic phase and the 1/(Nx Ny Nz) normalisation of the
 // round trip.  The reference performs the same arithmetic as four cuFFT calls
 // and three elementwise kernels per step (sources/sim_state.py:641-697).
-#include "wfs_internal.h"
+#include "wfs_yz_lines.h"
 
 namespace wfs {
 
@@ -145,223 +145,21 @@ __global__ __launch_bounds__(N2 * G) void fft_pass_kernel(cx<R>* __restrict__ da
 // host-side dispatch
 // ---------------------------------------------------------------------------
 
-template <int N> struct split_of;
-template <> struct split_of<8>   { static constexpr int n2 = 2; };
-template <> struct split_of<16>  { static constexpr int n2 = 4; };
-template <> struct split_of<32>  { static constexpr int n2 = 4; };
-template <> struct split_of<64>  { static constexpr int n2 = 8; };
-template <> struct split_of<96>  { static constexpr int n2 = 4; };
-template <> struct split_of<128> { static constexpr int n2 = 8; };
-template <> struct split_of<192> { static constexpr int n2 = 8; };
-template <> struct split_of<256> { static constexpr int n2 = 16; };
-template <> struct split_of<512> { static constexpr int n2 = 16; };
-
-// ---------------------------------------------------------------------------
-// Fused y-z pass on whole x planes (ny = nz = N).  One block holds one plane,
-// N*N points, in registers and moves it through LDS:
-//   y inverse -> transpose -> z inverse -> potential phase -> z forward
-//   -> transpose -> y forward
-// Each transform is column_dft with the split, table twiddles and operation
-// order of the axis passes y_inv, z_inv_v_fwd and y_fwd, so the result is the
-// same; the plane crosses HBM once (16 B per point for complex64) instead of
-// three times (48 B per point).
-// Thread t owns lines c0 + h N/H (h < H; columns z for the y transforms, rows y
-// for the z transforms) as thread u = t / (N/H) of each, N1 points per line:
-// the same mapping for both axes, so with N/H a multiple of 64 the table
-// twiddles of both transforms are wavefront-uniform (scalar loads).  The
-// exchanges and transpositions go through LDS in H rounds of N/H lines: the
-// LDS holds N/H * (N + 1) points (the transposition, padded; the exchanges
-// take N/H * N).  For N = 128 complex64 and H = 2: 66 KiB and 512 threads, two
-// blocks per CU, so one block's loads and stores overlap the other's
-// arithmetic.
-// ---------------------------------------------------------------------------
-template <typename R, int N, int H>
-constexpr long yz_plane_lds_bytes() {
-  return long(N / H) * (N + 1) * long(sizeof(cx<R>));
-}
-
-// An opaque copy of v: the compiler recomputes what derives from it instead of
-// keeping values from an earlier stage alive (register pressure).
-__device__ __forceinline__ int opaque_v(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
-template <typename R, int N, int H>
-__global__ __launch_bounds__(N * split_of<N>::n2 / H)
-__attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_plane_kernel(cx<R>* __restrict__ data,
-                                                                           const YzPlaneParams p) {
-  constexpr int N2 = split_of<N>::n2;
-  constexpr int N1 = N / N2;
-  constexpr int NH = N / H;                  // lines per round
-  constexpr int N1H = N1 / H;                // points of a line per round of the transposition
-  constexpr int TS = N + 1;                  // padded row stride of the transposition
-  static_assert(N1 % H == 0 && N2 >= H, "bad plane split");
-  constexpr bool UNIFORM = (NH % 64) == 0;   // u is the same across a wavefront
-  using C = cx<R>;
-  using CC = const __attribute__((address_space(4))) C;
-  __shared__ C lds[NH * TS];
-
-  const int tid = threadIdx.x;
-  const long plane = blockIdx.x;
-  C* __restrict__ blk = data + plane * long(N) * N;
-
-  C a[H][N1];
-  C w[N1];
-  auto thread_u = [&]() -> int {
-    if constexpr (UNIFORM) return __builtin_amdgcn_readfirstlane(tid / NH);
-    else return opaque_v(tid) / NH;
-  };
-  auto load_w = [&](const void* twp, int u) {
-    if constexpr (UNIFORM) {
-      CC* tw = (CC*)(twp);
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = mk<R>(tw[(u * k1) % N].x, tw[(u * k1) % N].y);
-    } else {
-      const C* __restrict__ tw = reinterpret_cast<const C*>(twp);
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(u * k1) % N];
-    }
-  };
-  // the exchange of column_dft for the lines of one round
-  auto exchange = [&](C (&line)[N1], int u, auto sign_c) {
-    const int c0 = opaque_v(tid) % NH;
-    column_dft_second<N, N2, decltype(sign_c)::value>(line, lds, u,
-                                                      [&](int k1, int n2) { return (k1 * N2 + n2) * NH + c0; });
-  };
-  // one transform of every line, round by round (one round's temporaries live at a time)
-  auto transform = [&](const void* twp, auto sign_c) {
-    const int u = thread_u();
-#pragma unroll
-    for (int h = 0; h < H; ++h) {
-      if (h) __syncthreads();
-      load_w(twp, u);
-      column_dft_first<N, N2, decltype(sign_c)::value>(a[h], w);
-      exchange(a[h], u, sign_c);
-    }
-  };
-  // transpositions between the column (y) and row (z) lines, N/H rows a round
-  auto cols_to_rows = [&]() {
-    const int t = opaque_v(tid);
-    const int c0 = t % NH, u = t / NH;
-    C b[H][N1];
-#pragma unroll
-    for (int r = 0; r < H; ++r) {
-      if (r) __syncthreads();
-#pragma unroll
-      for (int h = 0; h < H; ++h)
-#pragma unroll
-        for (int i = 0; i < N1H; ++i) lds[(u + N2 * i) * TS + c0 + h * NH] = a[h][r * N1H + i];
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < N1; ++i) b[r][i] = lds[c0 * TS + u + N2 * i];
-    }
-#pragma unroll
-    for (int r = 0; r < H; ++r)
-#pragma unroll
-      for (int i = 0; i < N1; ++i) a[r][i] = b[r][i];
-  };
-  auto rows_to_cols = [&]() {
-    const int t = opaque_v(tid);
-    const int c0 = t % NH, u = t / NH;
-    C b[H][N1];
-#pragma unroll
-    for (int r = 0; r < H; ++r) {
-      if (r) __syncthreads();
-#pragma unroll
-      for (int i = 0; i < N1; ++i) lds[c0 * TS + u + N2 * i] = a[r][i];
-      __syncthreads();
-#pragma unroll
-      for (int h = 0; h < H; ++h)
-#pragma unroll
-        for (int i = 0; i < N1H; ++i) b[h][r * N1H + i] = lds[(u + N2 * i) * TS + c0 + h * NH];
-    }
-#pragma unroll
-    for (int h = 0; h < H; ++h)
-#pragma unroll
-      for (int i = 0; i < N1; ++i) a[h][i] = b[h][i];
-  };
-
-  {
-    const int c0 = tid % NH, u = tid / NH;
-#pragma unroll
-    for (int h = 0; h < H; ++h)
-#pragma unroll
-      for (int i = 0; i < N1; ++i) a[h][i] = blk[unsigned(c0 + h * NH + (u + N2 * i) * N)];
-  }
-
-  // y inverse on columns
-  transform(p.tw_yi, std::integral_constant<int, +1>{});
-  __syncthreads();
-  cols_to_rows();
-  __syncthreads();
-
-  // z inverse, potential phase and z forward on rows y = c0 + r N/H (z = u + N2 i).  Each
-  // row round is finished (exchange, potential, first forward step) before the next one's
-  // exchange, so that only one round's potential factors are held.
-  {
-    const int u = thread_u();
-#pragma unroll
-    for (int r = 0; r < H; ++r) {
-      if (r) __syncthreads();
-      load_w(p.tw_z, u);
-      column_dft_first<N, N2, +1>(a[r], w);
-      exchange(a[r], u, std::integral_constant<int, +1>{});
-      const int c0 = opaque_v(tid) % NH;
-      if (p.spx) {
-        const long b = plane / p.nx;
-        const C* __restrict__ px = reinterpret_cast<const C*>(p.spx);
-        const C* __restrict__ py = reinterpret_cast<const C*>(p.spy) + b * N;
-        const C* __restrict__ pzb = reinterpret_cast<const C*>(p.spz) + b * N;
-        const C pxy = cmul(px[plane], py[c0 + r * NH]);
-        const int ur = opaque_v(u);     // separate loads per round (not held across rounds)
-#pragma unroll
-        for (int i = 0; i < N1; ++i) {
-          a[r][i] = cmul(a[r][i], cmul(pzb[ur + N2 * i], pxy));
-          if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");   // at most 4 factors in flight
-        }
-      } else {
-        const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + plane * long(N) * N;
-#pragma unroll
-        for (int i = 0; i < N1; ++i) {
-          a[r][i] = cmul(a[r][i], pvb[unsigned((c0 + r * NH) * N + u + N2 * i)]);
-          if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");
-        }
-      }
-      load_w(p.tw_z, u);
-      column_dft_first<N, N2, -1>(a[r], w);
-    }
-#pragma unroll
-    for (int r = 0; r < H; ++r) {
-      __syncthreads();
-      exchange(a[r], u, std::integral_constant<int, -1>{});
-    }
-  }
-  __syncthreads();
-  rows_to_cols();
-  __syncthreads();
-
-  // y forward on columns
-  transform(p.tw_yf, std::integral_constant<int, -1>{});
-  {
-    const int t = opaque_v(tid);
-    const int c0 = t % NH, u = t / NH;
-#pragma unroll
-    for (int h = 0; h < H; ++h)
-#pragma unroll
-      for (int i = 0; i < N1; ++i) blk[unsigned(c0 + h * NH + (u + N2 * i) * N)] = a[h][i];
-  }
-}
-
+// Fused y-z pass on whole x planes (ny = nz = N): yz_lines_kernel with one residue class
+// (B = 1) and LY = N, i.e. the plane's y inverse, z inverse, potential phase, z forward and
+// y forward in one launch, bit for bit the three axis passes.  The plane crosses HBM once
+// (16 B per point for complex64) instead of three times (48 B per point).  For N = 128
+// complex64 and H = 2: 66 KiB of LDS and 512 threads, two blocks per CU, so one block's
+// loads and stores overlap the other's arithmetic.
 template <typename R, int N, int H>
 static int launch_yz_plane_n(void* data, const YzPlaneParams& p, hipStream_t s) {
-  static_assert(yz_plane_lds_bytes<R, N, H>() <= 160 * 1024, "plane does not fit in LDS");
+  static_assert(yz_lines_lds_bytes<R, N, N, H>() <= 160 * 1024, "plane does not fit in LDS");
   if (p.planes <= 0 || p.planes > 0x7fffffffL) {
     set_last_error("yz plane pass: bad plane count %ld", p.planes);
     return WFS_ERR_INVALID;
   }
-  hipLaunchKernelGGL((yz_plane_kernel<R, N, H>), dim3(unsigned(p.planes)), dim3(N * split_of<N>::n2 / H), 0, s,
-                     reinterpret_cast<cx<R>*>(data), p);
+  hipLaunchKernelGGL((yz_lines_kernel<R, N, N, 1, H, LINES_FULL>), dim3(unsigned(p.planes)),
+                     dim3(yz_lines_threads<N, N, H>()), 0, s, reinterpret_cast<cx<R>*>(data), p);
   return int(hipGetLastError());
 }
 

@@ -36,6 +36,26 @@ int launch_fft_pass(int precision, int n, int layout, int mode, void* data, cons
                     int variant, hipStream_t s);
 // Fused y-z pass of the split step on whole x planes (ny == nz, plane held in LDS):
 // y inverse, z inverse, potential phase, z forward, y forward, in place.
+// Order of a potential phase table read by yz_lines_kernel in its z stage: the table of
+// row group g (LY rows of NZ points) holds, at (r * n1 + i) * threads + t, the factor of row
+// (t mod rh) + r rh, z = (t div rh) + n2z i: the points thread t multiplies in round r, so
+// that each wavefront reads one contiguous run.  Group g covers x plane g div b (over all
+// packets) and rows y = b * row + (g mod b).
+struct LinesOrder {
+  long ny = 0, nz = 0;
+  int ly = 0, b = 1, rh = 0, n2z = 0, n1 = 0, threads = 0;
+};
+// natural (C-order) index of element o of a lines-ordered table
+__host__ __device__ inline long lines_source(long o, const LinesOrder& m) {
+  const long per = long(m.ly) * m.nz;
+  const long g = o / per, rem = o % per;
+  const long ri = rem / m.threads;
+  const int t = int(rem % m.threads);
+  const int r = int(ri / m.n1), i = int(ri % m.n1);
+  const long row = long(m.b) * ((t % m.rh) + long(r) * m.rh) + g % m.b;
+  return (g / m.b) * m.ny * m.nz + row * m.nz + (t / m.rh) + long(m.n2z) * i;
+}
+
 struct YzPlaneParams {
   const void* tw_yi = nullptr;  // y twiddles of the inverse y transform (state precision)
   const void* tw_yf = nullptr;  // y twiddles of the forward y transform
@@ -51,8 +71,41 @@ struct YzPlaneParams {
 bool yz_plane_supported(int precision, int ny, int nz, int halves);
 int launch_yz_plane(int precision, int n, int halves, void* data, const YzPlaneParams& p, hipStream_t s);
 
+// Two-pass split step for complex64 grids with ny = nz = 256 (wfs_slab.hip): the y
+// transform split in a length-64 stage (on rows y = 4 a + b, with the z work) and a length-4
+// stage (with the x pass); see there.
+enum {
+  LINES_FULL = 0,    // y inverse, z inverse, potential phase, z forward, y forward
+  LINES_ENTRY = 1,   // z forward, y forward (position space in)
+  LINES_EXIT = 2     // y inverse, z inverse (position space out)
+};
+struct XSlabParams {
+  const void* tw_x = nullptr;   // forward twiddles exp(-2 pi i j / nx)
+  const void* tw_y = nullptr;   // forward twiddles exp(-2 pi i j / ny) (the y-stage twiddles)
+  const void* kx = nullptr;     // per-axis kinetic factors, normalisation folded in
+  const void* ky = nullptr;
+  const void* kz = nullptr;
+  long packets = 0;             // grids stacked along x
+};
+bool slab_supported(int precision, const int n[3]);
+// the potential table order of the slab scheme's y-z launch
+LinesOrder slab_lines_order();
+// YzPlaneParams.planes = packets * nx; variant 2 = persistent (one block per CU walking the
+// row groups, loading the next group during the current one's arithmetic), else one block
+// per group
+int launch_yz_slab(int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s);
+// variant 2: 8 z points per block, else 16
+int launch_x_slab(int nx, int variant, void* data, const XSlabParams& p, hipStream_t s);
+
 // pv = cexp_i(-dt V) elementwise in the state precision
 int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s);
+// the same table written in lines order: pv[o] = cexp_i(-dt V[lines_source(o)])
+int launch_potential_phase_lines(int precision, long count, const float2* V, double dt, void* pv,
+                                 const LinesOrder& m, hipStream_t s);
+// move a table between natural and lines order (to_lines: dst[o] = src[lines_source(o)],
+// else dst[lines_source(o)] = src[o])
+int launch_table_reorder(int precision, long count, const void* src, void* dst, const LinesOrder& m, bool to_lines,
+                         hipStream_t s);
 
 // power series: one Horner stage  u_out = psi + (-i dt / n) H u_in
 int launch_horner_pair(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,

@@ -24,7 +24,10 @@ struct wfs_plan {
   double* plane_work;     // plane-current partial records (grown on demand)
   long plane_work_len;
   double* plane_out;      // 7 doubles
-  void* pv;               // potential phase table cexp_i(-dt V), state precision
+  void* pv;               // potential phase table cexp_i(-dt V), state precision, C order
+  void* pv_lines;         // the same table in the slab scheme's lines order (LinesOrder)
+  bool pv_nat_ok;         // pv holds the current potential
+  bool pv_lines_ok;       // pv_lines holds the current potential
   bool pv_set;
   void* sep[3];           // separable potential: per-axis exp(-i dt f_d), [ntab][batch][n_d]
   bool sep_set;           // the step uses sep[] instead of pv
@@ -242,14 +245,57 @@ int run_pass(wfs_plan* p, int axis, int mode, void* data, const void* pv, const 
 
 size_t grid_points(const wfs_plan* p) { return size_t(p->batch) * p->n[0] * p->n[1] * p->n[2]; }
 
+bool slab_on(const wfs_plan* p) { return p->variant[WFS_K_YZ_SLAB] != 0; }
+
+int alloc_table(wfs_plan* p, void** t) {
+  if (*t) return 0;
+  hipError_t e = hipMalloc(t, grid_points(p) * p->elem);
+  if (e != hipSuccess) { *t = nullptr; set_last_error("hipMalloc for the potential phase table failed"); return WFS_ERR_ALLOC; }
+  return 0;
+}
+
+// the phase table in the order the current scheme reads (the other order is rebuilt from it
+// on demand)
 int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
-  if (!p->pv) {
-    hipError_t e = hipMalloc(&p->pv, grid_points(p) * p->elem);
-    if (e != hipSuccess) { p->pv = nullptr; set_last_error("hipMalloc for the potential phase table failed"); return WFS_ERR_ALLOC; }
+  const long count = long(grid_points(p));
+  if (slab_on(p)) {
+    WFS_TRY(alloc_table(p, &p->pv_lines));
+    WFS_TRY(launch_potential_phase_lines(p->precision, count, static_cast<const float2*>(V), p->dt, p->pv_lines,
+                                         slab_lines_order(), s));
+    p->pv_lines_ok = true;
+    p->pv_nat_ok = false;
+  } else {
+    WFS_TRY(alloc_table(p, &p->pv));
+    WFS_TRY(launch_potential_phase(p->precision, count, static_cast<const float2*>(V), p->dt, p->pv, s));
+    p->pv_nat_ok = true;
+    p->pv_lines_ok = false;
   }
-  WFS_TRY(launch_potential_phase(p->precision, long(grid_points(p)), static_cast<const float2*>(V), p->dt, p->pv, s));
   p->pv_set = true;
   p->sep_set = false;
+  return 0;
+}
+
+// the table the next steps read: C order for the axis passes and the plane pass, lines order
+// for the slab scheme
+int current_table(wfs_plan* p, hipStream_t s, const void** out) {
+  *out = nullptr;
+  if (p->sep_set) return 0;
+  const long count = long(grid_points(p));
+  if (slab_on(p)) {
+    if (!p->pv_lines_ok) {
+      WFS_TRY(alloc_table(p, &p->pv_lines));
+      WFS_TRY(launch_table_reorder(p->precision, count, p->pv, p->pv_lines, slab_lines_order(), true, s));
+      p->pv_lines_ok = true;
+    }
+    *out = p->pv_lines;
+  } else {
+    if (!p->pv_nat_ok) {
+      WFS_TRY(alloc_table(p, &p->pv));
+      WFS_TRY(launch_table_reorder(p->precision, count, p->pv_lines, p->pv, slab_lines_order(), false, s));
+      p->pv_nat_ok = true;
+    }
+    *out = p->pv;
+  }
   return 0;
 }
 
@@ -305,12 +351,60 @@ int run_yz(wfs_plan* p, void* data, const void* pv, hipStream_t s, int sel) {
   return run_pass(p, 1, MODE_FWD, data, nullptr, nullptr, 1.0, s, sel);
 }
 
+// Slab scheme (ny = nz = 256): the y-z launch of one step on rows y = 4 a + b (mode
+// LINES_FULL), or the entry (z, y forward) / exit (y, z inverse) launch of a call
+int run_slab_lines(wfs_plan* p, int mode, void* data, const void* pv, hipStream_t s, int sel) {
+  YzPlaneParams pp;
+  pp.tw_yi = table_copy(p, p->tw[1], 1, sel);
+  pp.tw_yf = pp.tw_yi;
+  pp.tw_z = table_copy(p, p->tw[2], 2, sel);
+  if (mode == LINES_FULL) {
+    if (p->sep_set) {
+      const size_t per = size_t(p->batch) * p->elem;
+      const int c = sel % p->ntab;
+      pp.spx = static_cast<const char*>(p->sep[0]) + size_t(c) * per * p->n[0];
+      pp.spy = static_cast<const char*>(p->sep[1]) + size_t(c) * per * p->n[1];
+      pp.spz = static_cast<const char*>(p->sep[2]) + size_t(c) * per * p->n[2];
+    } else {
+      pp.pv = pv;
+    }
+  }
+  pp.nx = p->n[0];
+  pp.planes = long(p->batch) * p->n[0];
+  ProfileScope scope(p, mode == LINES_FULL ? WFS_K_YZ_SLAB : WFS_K_SLAB_EDGE, s);
+  return launch_yz_slab(mode, p->variant[WFS_K_YZ_SLAB], data, pp, s);
+}
+
+// Slab scheme: length-4 y stages around x forward -> K -> x inverse
+int run_x_slab(wfs_plan* p, void* data, const void* const* ktab, hipStream_t s, int sel) {
+  XSlabParams xp;
+  xp.tw_x = table_copy(p, p->tw[0], 0, sel);
+  xp.tw_y = table_copy(p, p->tw[1], 1, sel);
+  xp.kx = table_copy(p, ktab[0], 0, sel);
+  xp.ky = table_copy(p, ktab[1], 1, sel);
+  xp.kz = table_copy(p, ktab[2], 2, sel);
+  xp.packets = p->batch;
+  ProfileScope scope(p, WFS_K_X_SLAB, s);
+  return launch_x_slab(p->n[0], p->variant[WFS_K_X_SLAB], data, xp, s);
+}
+
 int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
   if (V) WFS_TRY(set_potential(p, V, s));
   if (!p->pv_set) { set_last_error("wfs_step: no potential (pass V or call wfs_set_potential first)"); return WFS_ERR_INVALID; }
-  const void* pv = p->pv;
+  const void* pv = nullptr;
+  WFS_TRY(current_table(p, s, &pv));
   const void* const* kh = p->khalf;
   const void* const* kf = p->kfull;
+  if (p->variant[WFS_K_YZ_SLAB] != 0) {
+    // two launches per step; between them y holds the half-transformed Y[b][ka]
+    WFS_TRY(run_slab_lines(p, LINES_ENTRY, psi, nullptr, s, 0));
+    WFS_TRY(run_x_slab(p, psi, kh, s, 0));
+    for (int step = 1; step <= nsteps; ++step) {
+      WFS_TRY(run_slab_lines(p, LINES_FULL, psi, pv, s, step));
+      WFS_TRY(run_x_slab(p, psi, step == nsteps ? kh : kf, s, step));
+    }
+    return run_slab_lines(p, LINES_EXIT, psi, nullptr, s, nsteps);
+  }
   // every pass of step k uses table copy k (mod ntab); the entry passes use copy 0
   WFS_TRY(run_pass(p, 2, MODE_FWD, psi, nullptr, nullptr, 1.0, s, 0));
   WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s, 0));
@@ -384,6 +478,7 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
   if (p->plane_work) (void)hipFree(p->plane_work);
   if (p->plane_out) (void)hipFree(p->plane_out);
   if (p->pv) (void)hipFree(p->pv);
+  if (p->pv_lines) (void)hipFree(p->pv_lines);
   if (p->pending) {
     for (int k = 0; k < WFS_K_COUNT; ++k)
       for (auto& pr : p->pending[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -434,6 +529,7 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->variant[WFS_K_YZ_PLANE] =
       (yz_plane_supported(precision, n[1], n[2], 2) && n[1] == 128 && long(batch) * n[0] >= 512) ? 2
       : yz_plane_supported(precision, n[1], n[2], 1) ? 1 : 0;
+  p->variant[WFS_K_YZ_SLAB] = slab_supported(precision, n) ? 1 : 0;
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;
@@ -549,6 +645,10 @@ extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
     set_last_error("wfs_plan_set_variant: fused plane pass variant %d unsupported for %d x %d planes", variant, p->n[1], p->n[2]);
     return WFS_ERR_UNSUPPORTED;
   }
+  if (kc == WFS_K_YZ_SLAB && variant > 0 && !slab_supported(p->precision, p->n)) {
+    set_last_error("wfs_plan_set_variant: the slab scheme needs complex64, ny = nz = 256 and nx <= 256");
+    return WFS_ERR_UNSUPPORTED;
+  }
   p->variant[kc] = variant;
   return 0;
 }
@@ -620,6 +720,50 @@ extern "C" int wfs_plan_autotune(wfs_plan* p, void* stream) {
       if (rc == 0 && ms < best) { best = ms; best_v = v; }
     }
     p->variant[WFS_K_YZ_PLANE] = rc == 0 ? best_v : keep;
+  }
+  // slab scheme (two launches per step) against the tuned four axis passes: first the x
+  // launch's block width, then one step of each y-z form and of the passes
+  if (rc == 0 && slab_supported(p->precision, p->n)) {
+    {
+      const int keep = p->variant[WFS_K_X_SLAB];
+      float best = 1e30f;
+      int best_v = keep;
+      for (int v = 1; v < 3 && rc == 0; ++v) {
+        p->variant[WFS_K_X_SLAB] = v;
+        rc = run_x_slab(p, buf, p->kfull, s, 1);
+        (void)hipEventRecord(a, s);
+        for (int r = 0; r < 3 && rc == 0; ++r) rc = run_x_slab(p, buf, p->kfull, s, 1);
+        (void)hipEventRecord(b, s);
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (rc == 0 && ms < best) { best = ms; best_v = v; }
+      }
+      p->variant[WFS_K_X_SLAB] = rc == 0 ? best_v : keep;
+    }
+    const int keep = p->variant[WFS_K_YZ_SLAB];
+    float best = 1e30f;
+    int best_v = keep;
+    for (int v = 0; v < 3 && rc == 0; ++v) {
+      p->variant[WFS_K_YZ_SLAB] = v;
+      auto one_step = [&]() -> int {
+        if (v) {
+          WFS_TRY(run_slab_lines(p, LINES_FULL, buf, tab, s, 1));
+          return run_x_slab(p, buf, p->kfull, s, 1);
+        }
+        WFS_TRY(run_yz(p, buf, tab, s, 1));
+        return run_pass(p, 0, MODE_FKI, buf, nullptr, p->kfull, 1.0, s, 1);
+      };
+      rc = one_step();
+      (void)hipEventRecord(a, s);
+      for (int r = 0; r < 3 && rc == 0; ++r) rc = one_step();
+      (void)hipEventRecord(b, s);
+      (void)hipEventSynchronize(b);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (rc == 0 && ms < best) { best = ms; best_v = v; }
+    }
+    p->variant[WFS_K_YZ_SLAB] = rc == 0 ? best_v : keep;
   }
   p->profiling = prof;
   (void)hipEventDestroy(a);

@@ -265,16 +265,41 @@ __global__ __launch_bounds__(kGenThreads) void kinetic_table_kernel(cx<R>* __res
 
 // cexp_i(-dt V) = (cos(-dt Vr), sin(-dt Vr)) exp(dt Vi), FP64 then rounded
 template <typename R>
+__device__ __forceinline__ cx<R> potential_factor(float2 v, double dt) {
+  double s, c;
+  sincos(-dt * double(v.x), &s, &c);
+  const double a = exp(dt * double(v.y));
+  return mk<R>(R(c * a), R(s * a));
+}
+
+template <typename R>
 __global__ __launch_bounds__(kGenThreads) void potential_table_kernel(cx<R>* __restrict__ out,
                                                                       const float2* __restrict__ V, long n,
                                                                       double dt) {
   const long i = long(blockIdx.x) * kGenThreads + threadIdx.x;
   if (i >= n) return;
-  const float2 v = V[i];
-  double s, c;
-  sincos(-dt * double(v.x), &s, &c);
-  const double a = exp(dt * double(v.y));
-  out[i] = mk<R>(R(c * a), R(s * a));
+  out[i] = potential_factor<R>(V[i], dt);
+}
+
+// the same factors in the order the slab scheme's y-z launch reads them
+template <typename R>
+__global__ __launch_bounds__(kGenThreads) void potential_table_lines_kernel(cx<R>* __restrict__ out,
+                                                                            const float2* __restrict__ V, long n,
+                                                                            double dt, LinesOrder m) {
+  const long o = long(blockIdx.x) * kGenThreads + threadIdx.x;
+  if (o >= n) return;
+  out[o] = potential_factor<R>(V[lines_source(o, m)], dt);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kGenThreads) void table_reorder_kernel(cx<R>* __restrict__ dst,
+                                                                    const cx<R>* __restrict__ src, long n,
+                                                                    LinesOrder m, int to_lines) {
+  const long o = long(blockIdx.x) * kGenThreads + threadIdx.x;
+  if (o >= n) return;
+  const long j = lines_source(o, m);
+  if (to_lines) dst[o] = src[j];
+  else dst[j] = src[o];
 }
 
 }  // namespace
@@ -431,6 +456,37 @@ int launch_potential_phase(int precision, long count, const float2* V, double dt
   else
     hipLaunchKernelGGL(potential_table_kernel<float>, dim3(blocks_for(count)), dim3(kGenThreads), 0, s,
                        static_cast<float2*>(pv), V, count, dt);
+  return int(hipGetLastError());
+}
+
+static bool lines_order_ok(long count, const LinesOrder& m) {
+  return m.ly > 0 && m.nz > 0 && m.b > 0 && m.rh > 0 && m.n1 > 0 && m.threads > 0 &&
+         count % (long(m.ly) * m.nz) == 0 && long(m.ly) * m.b == m.ny;
+}
+
+int launch_potential_phase_lines(int precision, long count, const float2* V, double dt, void* pv,
+                                 const LinesOrder& m, hipStream_t s) {
+  if (count <= 0) return 0;
+  if (!lines_order_ok(count, m)) { set_last_error("potential table: bad lines order"); return WFS_ERR_INVALID; }
+  if (precision == WFS_C128)
+    hipLaunchKernelGGL(potential_table_lines_kernel<double>, dim3(blocks_for(count)), dim3(kGenThreads), 0, s,
+                       static_cast<double2*>(pv), V, count, dt, m);
+  else
+    hipLaunchKernelGGL(potential_table_lines_kernel<float>, dim3(blocks_for(count)), dim3(kGenThreads), 0, s,
+                       static_cast<float2*>(pv), V, count, dt, m);
+  return int(hipGetLastError());
+}
+
+int launch_table_reorder(int precision, long count, const void* src, void* dst, const LinesOrder& m, bool to_lines,
+                         hipStream_t s) {
+  if (count <= 0) return 0;
+  if (!lines_order_ok(count, m)) { set_last_error("potential table: bad lines order"); return WFS_ERR_INVALID; }
+  if (precision == WFS_C128)
+    hipLaunchKernelGGL(table_reorder_kernel<double>, dim3(blocks_for(count)), dim3(kGenThreads), 0, s,
+                       static_cast<double2*>(dst), static_cast<const double2*>(src), count, m, int(to_lines));
+  else
+    hipLaunchKernelGGL(table_reorder_kernel<float>, dim3(blocks_for(count)), dim3(kGenThreads), 0, s,
+                       static_cast<float2*>(dst), static_cast<const float2*>(src), count, m, int(to_lines));
   return int(hipGetLastError());
 }
 }  // namespace wfs

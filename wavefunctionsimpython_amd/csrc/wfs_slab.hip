@@ -1,0 +1,241 @@
+// Two-pass split step for complex64 grids with ny = nz = 256 (gfx950).
+//
+// A 256^2 y-z plane (512 KiB) does not fit in one CU, so the y transform is split in two
+// stages, ny = LY * B with LY = 64, B = 4, y = B a + b, ky = ka + LY kb:
+//   X[ka + LY kb] = sum_b W_B^{b kb} W_NY^{b ka} sum_a x[B a + b] W_LY^{a ka}
+// and each stage rides on a pass that is needed anyway:
+//
+//   yz_lines_kernel<NZ 256, LY 64, B 4>  (one block per x plane and residue b: the 64 rows
+//     y = 4 a + b with all z, 128 KiB):  length-64 inverse y stage over a, z inverse,
+//     potential phase, z forward, length-64 forward y stage
+//   x_slab_kernel  (one block per packet, ka and 16 consecutive z: the 4 rows y = 4 ka + b
+//     with all x):  twiddles W_NY^{b ka} and length-4 forward y stage over b, x forward,
+//     kinetic phase K(kx, ky, kz), x inverse, length-4 inverse y stage and twiddles
+//
+// Between the passes the y axis holds the half-transformed Y[b][ka] at row B ka + b.  The
+// y-z launch reads the potential phase table in lines order (LinesOrder), one contiguous run
+// per wavefront load.  A step
+// moves 16 B per point in each pass (+8 for a potential table) against 64 (+8) for the four
+// axis passes.  The arithmetic differs from the axis passes only in how the length-256 y
+// transform is factored, so the results agree with theirs to float rounding, not bit for bit.
+#include "wfs_yz_lines.h"
+
+#include <algorithm>
+
+namespace wfs {
+
+namespace {
+
+constexpr int kSlabNY = 256, kSlabNZ = 256, kSlabLY = 64, kSlabB = kSlabNY / kSlabLY;
+
+// Lane pairs: the partner lanes tid and tid ^ ZB share the (x, z) points of a block; lane
+// q = (tid / ZB) % 2 holds rows b = q and q + 2 (and, after the forward y stage, ky
+// columns kb = q and q + 2).  The length-4 y stage is a 2-point stage in each lane, one
+// complex value swapped with the partner, and a second 2-point stage:
+//   forward, Z[b] = W_NY^{b ka} Y[b]:  E_q[k'] = Z[q] + (-1)^k' Z[q + 2],
+//            X[k' + 2 k''] = E_0[k'] + (-1)^k'' W_4^{k'} E_1[k']
+//   inverse: F_q[beta] = X[q] + (-1)^beta X[q + 2],
+//            out[b] = F_0[b mod 2] + W_4^{-b} F_1[b mod 2],  Y'[b] = W_NY^{-b ka} out[b]
+// Lane q keeps the k' = q (beta = q) values and receives the partner's.
+template <typename R>
+__device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
+  return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+template <typename R, int NX, int NY, int NZ, int B, int ZB>
+__global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>* __restrict__ data,
+                                                                           const XSlabParams p) {
+  constexpr int N2 = split_of<NX>::n2;
+  constexpr int N1 = NX / N2;
+  constexpr int A = NY / B;
+  constexpr int TZ = NZ / ZB;
+  static_assert(B == 4, "the lane-pair y stage is written for B = 4");
+  static_assert(N1 * N2 == NX && N1 % N2 == 0 && NZ % ZB == 0 && NY % B == 0 && 2 * ZB <= 64, "bad slab split");
+  using C = cx<R>;
+  using CC = const __attribute__((address_space(4))) C;
+  constexpr int LINES = 2 * ZB;            // x lines per round: (q, z)
+  __shared__ C lds[LINES * NX];
+
+  const int tid = threadIdx.x;
+  const int zl = tid % ZB;                 // z within the block's segment
+  const int q = (tid / ZB) % 2;            // lane-pair half
+  const int t = tid / (2 * ZB);            // thread of the x line: x = t + N2 i
+  const int line = q * ZB + zl;
+  const long bid = blockIdx.x;
+  const int zt = int(bid % TZ);
+  const int ka = int((bid / TZ) % A);
+  const long pk = bid / (long(TZ) * A);
+  C* __restrict__ blk = data + pk * (long(NX) * NY * NZ) + long(B * ka) * NZ + long(zt) * ZB;
+  constexpr unsigned XS = unsigned(NY) * NZ;   // x stride; offsets stay below 2^32 (NX <= 512)
+  // element i of row q + 2 j
+  auto off = [&](int i, int j) { return unsigned(zl) + unsigned(q + 2 * j) * NZ + unsigned(t + N2 * i) * XS; };
+
+  // y-stage twiddles W_NY^{b ka} of this lane's rows b = q, q + 2
+  C wq[2];
+  {
+    CC* twy = (CC*)(p.tw_y);
+    C wb[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) wb[b] = mk<R>(twy[(b * ka) % NY].x, twy[(b * ka) % NY].y);
+    wq[0] = q ? wb[1] : wb[0];
+    wq[1] = q ? wb[3] : wb[2];
+  }
+
+  C a[2][N1];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[j][i] = blk[off(i, j)];
+
+  // forward y stage: a[0] <- X[q], a[1] <- X[q + 2]
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    const C z0 = cmul(a[0][i], wq[0]);
+    const C z1 = cmul(a[1][i], wq[1]);
+    const C e0 = cadd(z0, z1), e1 = csub(z0, z1);
+    const C keep = q ? e1 : e0;
+    const C recv = shfl_xor_c<R>(q ? e0 : e1, ZB);
+    const C P = q ? recv : keep;                         // E_0[q]
+    const C Q = q ? keep : recv;                         // E_1[q]
+    const C Qt = q ? mk<R>(Q.y, -Q.x) : Q;               // W_4^q E_1[q], W_4 = -i
+    a[0][i] = cadd(P, Qt);
+    a[1][i] = csub(P, Qt);
+  }
+
+  // x forward, kinetic phase, x inverse on the columns ky = ka + A kb, kb = q + 2 j
+  const C* __restrict__ tw = reinterpret_cast<const C*>(p.tw_x);
+  C w[N1];
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(t * k1) % NX];
+  auto lidx = [&](int k1, int n2) { return (k1 * N2 + n2) * LINES + line; };
+  const C* __restrict__ kx = reinterpret_cast<const C*>(p.kx);
+  const C* __restrict__ ky = reinterpret_cast<const C*>(p.ky);
+  const C* __restrict__ kz = reinterpret_cast<const C*>(p.kz);
+  // one column at a time (the other one's points are the only extra registers)
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    if (j) __syncthreads();
+    column_dft<NX, N2, -1>(a[j], w, lds, t, lidx);
+    const C kyz = cmul(ky[ka + A * (q + 2 * j)], kz[zt * ZB + zl]);
+#pragma unroll
+    for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(kx[t + N2 * i], kyz));
+    __syncthreads();
+    column_dft<NX, N2, +1>(a[j], w, lds, t, lidx);
+  }
+
+  // inverse y stage: a[0] <- Y'[q], a[1] <- Y'[q + 2]
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    const C f0 = cadd(a[0][i], a[1][i]), f1 = csub(a[0][i], a[1][i]);
+    const C keep = q ? f1 : f0;
+    const C recv = shfl_xor_c<R>(q ? f0 : f1, ZB);
+    const C P = q ? recv : keep;                         // F_0[q]
+    const C Q = q ? keep : recv;                         // F_1[q]
+    const C Qt = q ? mk<R>(-Q.y, Q.x) : Q;               // W_4^{-q} F_1[q], W_4^{-1} = +i
+    a[0][i] = cmulc(cadd(P, Qt), wq[0]);
+    a[1][i] = cmulc(csub(P, Qt), wq[1]);
+  }
+
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < N1; ++i) blk[off(i, j)] = a[j][i];
+}
+
+// compute units of the current device (cached per device)
+int cu_count() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cache[dev] > 0) return cache[dev];
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+  cache[dev] = n;
+  return n;
+}
+
+// persistent: one block per CU walking the row groups, else one block per group
+template <int MODE, bool PERSIST>
+int launch_lines_m(void* data, const YzPlaneParams& p, hipStream_t s) {
+  static_assert(yz_lines_lds_bytes<float, kSlabNZ, kSlabLY, 1>() <= 160 * 1024, "rows do not fit in LDS");
+  const long groups = p.planes * kSlabB;
+  const long grid = PERSIST ? std::min<long>(groups, cu_count()) : groups;
+  hipLaunchKernelGGL((yz_lines_kernel<float, kSlabNZ, kSlabLY, kSlabB, 1, MODE, PERSIST, true>), dim3(unsigned(grid)),
+                     dim3(yz_lines_threads<kSlabNZ, kSlabLY, 1>()), 0, s, reinterpret_cast<float2*>(data), p);
+  return int(hipGetLastError());
+}
+
+template <int MODE>
+int launch_lines_mode(int variant, void* data, const YzPlaneParams& p, hipStream_t s) {
+  return variant == 2 ? launch_lines_m<MODE, true>(data, p, s) : launch_lines_m<MODE, false>(data, p, s);
+}
+
+template <int NX, int ZB>
+int launch_x_slab_z(void* data, const XSlabParams& p, hipStream_t s) {
+  constexpr long per_packet = long(kSlabNY / kSlabB) * (kSlabNZ / ZB);
+  hipLaunchKernelGGL((x_slab_kernel<float, NX, kSlabNY, kSlabNZ, kSlabB, ZB>), dim3(unsigned(p.packets * per_packet)),
+                     dim3(2 * ZB * split_of<NX>::n2), 0, s, reinterpret_cast<float2*>(data), p);
+  return int(hipGetLastError());
+}
+
+// variant 2: 8 z points per block (256 threads at nx = 256, two blocks per CU), else 16
+template <int NX>
+int launch_x_slab_n(int variant, void* data, const XSlabParams& p, hipStream_t s) {
+  return variant == 2 ? launch_x_slab_z<NX, 8>(data, p, s) : launch_x_slab_z<NX, 16>(data, p, s);
+}
+
+}  // namespace
+
+// complex64, ny = nz = 256, nx a supported FFT length up to 256 (at 512 the x lines of a
+// lane pair do not fit in registers)
+bool slab_supported(int precision, const int n[3]) {
+  return precision == WFS_C64 && n[1] == kSlabNY && n[2] == kSlabNZ && n[0] <= 256 && fft_length_supported(n[0]);
+}
+
+LinesOrder slab_lines_order() {
+  LinesOrder m;
+  m.ny = kSlabNY;
+  m.nz = kSlabNZ;
+  m.ly = kSlabLY;
+  m.b = kSlabB;
+  m.rh = kSlabLY;
+  m.n2z = split_of<kSlabNZ>::n2;
+  m.n1 = kSlabNZ / split_of<kSlabNZ>::n2;
+  m.threads = yz_lines_threads<kSlabNZ, kSlabLY, 1>();
+  return m;
+}
+
+int launch_yz_slab(int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s) {
+  if (p.planes <= 0 || p.planes * kSlabB > 0x7fffffffL) {
+    set_last_error("slab y-z pass: bad plane count %ld", p.planes);
+    return WFS_ERR_INVALID;
+  }
+  if (mode == LINES_FULL) return launch_lines_mode<LINES_FULL>(variant, data, p, s);
+  if (mode == LINES_ENTRY) return launch_lines_mode<LINES_ENTRY>(variant, data, p, s);
+  if (mode == LINES_EXIT) return launch_lines_mode<LINES_EXIT>(variant, data, p, s);
+  set_last_error("slab y-z pass: bad mode %d", mode);
+  return WFS_ERR_INVALID;
+}
+
+int launch_x_slab(int nx, int variant, void* data, const XSlabParams& p, hipStream_t s) {
+  const long blocks = p.packets * long(kSlabNY / kSlabB) * (kSlabNZ / 8);
+  if (p.packets <= 0 || blocks > 0x7fffffffL) {
+    set_last_error("slab x pass: bad packet count %ld", p.packets);
+    return WFS_ERR_INVALID;
+  }
+  switch (nx) {
+    case 8:   return launch_x_slab_n<8>(variant, data, p, s);
+    case 16:  return launch_x_slab_n<16>(variant, data, p, s);
+    case 32:  return launch_x_slab_n<32>(variant, data, p, s);
+    case 64:  return launch_x_slab_n<64>(variant, data, p, s);
+    case 96:  return launch_x_slab_n<96>(variant, data, p, s);
+    case 128: return launch_x_slab_n<128>(variant, data, p, s);
+    case 192: return launch_x_slab_n<192>(variant, data, p, s);
+    case 256: return launch_x_slab_n<256>(variant, data, p, s);
+    default:
+      set_last_error("slab x pass: unsupported length %d", nx);
+      return WFS_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace wfs
