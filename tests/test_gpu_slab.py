@@ -149,8 +149,8 @@ def test_slab_full_grid_against_passes_and_norm(dev):
 def test_slab_defaults_and_limits(dev):
     def variant(shape, precision=C64, batch=1):
         return _abi.Plan(shape, DX, DT, 1.0, precision, _abi.WFS_METHOD_FOURIER, batch=batch).variants()["yz_slab"]
-    assert variant((256, 256, 256)) == 1
-    assert variant((16, 256, 256), batch=4) == 1
+    assert variant((256, 256, 256)) == 2
+    assert variant((16, 256, 256), batch=4) == 2
     assert variant((256, 256, 256), C128) == 0
     assert variant((512, 256, 256)) == 0
     assert variant((256, 128, 128)) == 0
@@ -190,7 +190,7 @@ def test_slab_graph_replays_match_eager(dev, form):
     Vd = torch.from_numpy(V).to(dev)
     eager_plan = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
     graph_plan = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
-    assert eager_plan.variants()["yz_slab"] == 1
+    assert eager_plan.variants()["yz_slab"] == 2
     for pl in (eager_plan, graph_plan):
         pl.set_variant("yz_slab", form[0])
         pl.set_variant("x_slab", form[1])

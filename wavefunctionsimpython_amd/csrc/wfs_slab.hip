@@ -42,6 +42,10 @@ __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
   return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
 
+// ZB = 8: a block reads and writes 64-byte halves of 128-byte lines; the two blocks of a line
+// get the block indices 16 g + x and 16 g + 8 + x, which the dispatcher sends to the same XCD
+// (blocks go round-robin over the 8 XCDs) one after the other, so the second half comes from
+// that XCD's L2.
 template <typename R, int NX, int NY, int NZ, int B, int ZB>
 __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>* __restrict__ data,
                                                                            const XSlabParams p) {
@@ -61,7 +65,8 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   const int q = (tid / ZB) % 2;            // lane-pair half
   const int t = tid / (2 * ZB);            // thread of the x line: x = t + N2 i
   const int line = q * ZB + zl;
-  const long bid = blockIdx.x;
+  long bid = blockIdx.x;
+  if constexpr (ZB == 8) bid = 2 * (8 * (bid / 16) + bid % 8) + (bid / 8) % 2;
   const int zt = int(bid % TZ);
   const int ka = int((bid / TZ) % A);
   const long pk = bid / (long(TZ) * A);
@@ -102,24 +107,36 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     a[1][i] = csub(P, Qt);
   }
 
-  // x forward, kinetic phase, x inverse on the columns ky = ka + A kb, kb = q + 2 j
+  // x forward, kinetic phase, x inverse on the columns ky = ka + A kb, kb = q + 2 j, one
+  // column at a time: the table twiddles and kinetic factors are loaded where they are used
+  // (opaque indices), so that only the other column's points stay live besides the transform
   const C* __restrict__ tw = reinterpret_cast<const C*>(p.tw_x);
-  C w[N1];
-#pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(t * k1) % NX];
   auto lidx = [&](int k1, int n2) { return (k1 * N2 + n2) * LINES + line; };
   const C* __restrict__ kx = reinterpret_cast<const C*>(p.kx);
   const C* __restrict__ ky = reinterpret_cast<const C*>(p.ky);
   const C* __restrict__ kz = reinterpret_cast<const C*>(p.kz);
-  // one column at a time (the other one's points are the only extra registers)
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
+    C w[N1];
     if (j) __syncthreads();
-    column_dft<NX, N2, -1>(a[j], w, lds, t, lidx);
-    const C kyz = cmul(ky[ka + A * (q + 2 * j)], kz[zt * ZB + zl]);
+    {
+      const int tt = opaque_v(t);
 #pragma unroll
-    for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(kx[t + N2 * i], kyz));
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(tt * k1) % NX];
+    }
+    column_dft<NX, N2, -1>(a[j], w, lds, t, lidx);
+    {
+      const int tt = opaque_v(t);
+      const C kyz = cmul(ky[ka + A * (q + 2 * j)], kz[zt * ZB + zl]);
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(kx[tt + N2 * i], kyz));
+    }
     __syncthreads();
+    {
+      const int tt = opaque_v(t);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(tt * k1) % NX];
+    }
     column_dft<NX, N2, +1>(a[j], w, lds, t, lidx);
   }
 
@@ -136,10 +153,16 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     a[1][i] = cmulc(csub(P, Qt), wq[1]);
   }
 
+  // store addresses recomputed from an opaque copy of the thread index, so the load
+  // addresses do not stay live through the transforms
+  {
+    const int tt = opaque_v(tid);
+    const unsigned base = unsigned(tt % ZB) + unsigned((tt / ZB) % 2) * NZ + unsigned(tt / (2 * ZB)) * XS;
 #pragma unroll
-  for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int i = 0; i < N1; ++i) blk[off(i, j)] = a[j][i];
+      for (int i = 0; i < N1; ++i) blk[base + unsigned(2 * j) * NZ + unsigned(N2 * i) * XS] = a[j][i];
+  }
 }
 
 // compute units of the current device (cached per device)
@@ -173,6 +196,7 @@ int launch_lines_mode(int variant, void* data, const YzPlaneParams& p, hipStream
 template <int NX, int ZB>
 int launch_x_slab_z(void* data, const XSlabParams& p, hipStream_t s) {
   constexpr long per_packet = long(kSlabNY / kSlabB) * (kSlabNZ / ZB);
+  static_assert(per_packet % 16 == 0, "the XCD pairing needs a multiple of 16 blocks");
   hipLaunchKernelGGL((x_slab_kernel<float, NX, kSlabNY, kSlabNZ, kSlabB, ZB>), dim3(unsigned(p.packets * per_packet)),
                      dim3(2 * ZB * split_of<NX>::n2), 0, s, reinterpret_cast<float2*>(data), p);
   return int(hipGetLastError());
