@@ -57,9 +57,11 @@ __host__ __device__ inline long lines_source(long o, const LinesOrder& m) {
 }
 
 struct YzPlaneParams {
-  const void* tw_yi = nullptr;  // y twiddles of the inverse y transform (state precision)
-  const void* tw_yf = nullptr;  // y twiddles of the forward y transform
-  const void* tw_z = nullptr;   // z twiddles (both z transforms)
+  // y twiddles of the inverse / forward y transforms and the z twiddles, state precision:
+  // the length-N axis tables, or (slab scheme) each thread's twiddles [u][k1]
+  const void* tw_yi = nullptr;
+  const void* tw_yf = nullptr;
+  const void* tw_z = nullptr;
   const void* pv = nullptr;     // potential phase table, state layout (null when spx is set)
   const void* spx = nullptr;    // separable potential: per-axis phase factors [packet][n_d]
   const void* spy = nullptr;
@@ -88,6 +90,12 @@ struct XSlabParams {
   long packets = 0;             // grids stacked along x
 };
 bool slab_supported(int precision, const int n[3]);
+// thread geometry of the slab scheme's y-z launch: y lines split over uy threads, z lines
+// over uz threads, n1 points per thread; y-line twiddle stride b in the ny table
+struct SlabRows {
+  int b, uy, uz, n1;
+};
+SlabRows slab_rows();
 // the potential table order of the slab scheme's y-z launch
 LinesOrder slab_lines_order();
 // YzPlaneParams.planes = packets * nx; variant 2 = persistent (one block per CU walking the

@@ -29,6 +29,7 @@ struct wfs_plan {
   bool pv_nat_ok;         // pv holds the current potential
   bool pv_lines_ok;       // pv_lines holds the current potential
   bool pv_set;
+  void* tw_rows[2];       // slab scheme: the y and z twiddles of each thread's lines, [ntab][u][k1]
   void* sep[3];           // separable potential: per-axis exp(-i dt f_d), [ntab][batch][n_d]
   bool sep_set;           // the step uses sep[] instead of pv
   int variant[WFS_K_COUNT];   // tile variant per kernel class (wfs_plan_autotune)
@@ -111,6 +112,24 @@ int upload_dithered(void** dst, const std::vector<double>& re, const std::vector
       h[c * n + j].x = dither_part(re[j], long(j), 0, c);
       h[c * n + j].y = dither_part(im[j], long(j), 1, c);
     }
+  return upload(dst, h.data(), h.size() * sizeof(float2));
+}
+
+// The twiddles W_L^{u k1} = W_N^{(M u k1) mod N} (u < U, k1 < K) of the length-L lines
+// of the slab scheme's y-z launch, each copy laid out [u][k1] so that a thread's K twiddles
+// are contiguous (wide scalar loads); entry for entry the values of the dithered N-table.
+int upload_rows_table(void** dst, const std::vector<double>& re, const std::vector<double>& im, int M, int U,
+                      int K) {
+  const long n = long(re.size());
+  std::vector<float2> h(size_t(kDither) * U * K);
+  for (int c = 0; c < kDither; ++c)
+    for (int u = 0; u < U; ++u)
+      for (int k1 = 0; k1 < K; ++k1) {
+        const long j = (long(M) * u * k1) % n;
+        float2& e = h[(size_t(c) * U + u) * K + k1];
+        e.x = dither_part(re[j], j, 0, c);
+        e.y = dither_part(im[j], j, 1, c);
+      }
   return upload(dst, h.data(), h.size() * sizeof(float2));
 }
 
@@ -355,9 +374,11 @@ int run_yz(wfs_plan* p, void* data, const void* pv, hipStream_t s, int sel) {
 // LINES_FULL), or the entry (z, y forward) / exit (y, z inverse) launch of a call
 int run_slab_lines(wfs_plan* p, int mode, void* data, const void* pv, hipStream_t s, int sel) {
   YzPlaneParams pp;
-  pp.tw_yi = table_copy(p, p->tw[1], 1, sel);
+  const SlabRows g = slab_rows();
+  const int c = sel % p->ntab;
+  pp.tw_yi = static_cast<const char*>(p->tw_rows[0]) + size_t(c) * g.uy * g.n1 * p->elem;
   pp.tw_yf = pp.tw_yi;
-  pp.tw_z = table_copy(p, p->tw[2], 2, sel);
+  pp.tw_z = static_cast<const char*>(p->tw_rows[1]) + size_t(c) * g.uz * g.n1 * p->elem;
   if (mode == LINES_FULL) {
     if (p->sep_set) {
       const size_t per = size_t(p->batch) * p->elem;
@@ -479,6 +500,8 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
   if (p->plane_out) (void)hipFree(p->plane_out);
   if (p->pv) (void)hipFree(p->pv);
   if (p->pv_lines) (void)hipFree(p->pv_lines);
+  for (int k = 0; k < 2; ++k)
+    if (p->tw_rows[k]) (void)hipFree(p->tw_rows[k]);
   if (p->pending) {
     for (int k = 0; k < WFS_K_COUNT; ++k)
       for (auto& pr : p->pending[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -545,6 +568,12 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
     }
     rc = upload_dithered(&p->tw[a], re, im, precision);
     if (rc) break;
+    if (a > 0 && slab_supported(precision, n)) {
+      const SlabRows g = slab_rows();
+      rc = a == 1 ? upload_rows_table(&p->tw_rows[0], re, im, g.b, g.uy, g.n1)
+                  : upload_rows_table(&p->tw_rows[1], re, im, 1, g.uz, g.n1);
+      if (rc) break;
+    }
     kinetic_table(n[a], dx[a], dt, mass, 2.0, re, im);
     rc = upload_dithered(&p->kfull[a], re, im, precision);
     if (rc) break;

@@ -216,6 +216,15 @@ bool slab_supported(int precision, const int n[3]) {
   return precision == WFS_C64 && n[1] == kSlabNY && n[2] == kSlabNZ && n[0] <= 256 && fft_length_supported(n[0]);
 }
 
+SlabRows slab_rows() {
+  SlabRows g;
+  g.b = kSlabB;
+  g.n1 = kSlabNZ / split_of<kSlabNZ>::n2;
+  g.uy = kSlabLY / g.n1;
+  g.uz = split_of<kSlabNZ>::n2;
+  return g;
+}
+
 LinesOrder slab_lines_order() {
   LinesOrder m;
   m.ny = kSlabNY;

@@ -61,7 +61,9 @@ constexpr long yz_lines_lds_bytes() {
 // and loads the next group's points while it transforms the current one (H = 1 only), so
 // that one block per CU still overlaps its memory traffic with its arithmetic.
 // PV_LINES: the potential table is in lines order (LinesOrder, wfs_internal.h), read as one
-// contiguous run per wavefront; else in the state's C order.
+// contiguous run per wavefront, and the twiddle tables hold each thread's twiddles
+// contiguously ([u][k1], u the thread's index within a line); else the potential table is
+// in the state's C order and the twiddles come from the length-N axis tables.
 template <typename R, int NZ, int LY, int B, int H, int MODE, bool PERSIST = false, bool PV_LINES = false>
 __global__ __launch_bounds__((yz_lines_threads<NZ, LY, H>()))
 __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_lines_kernel(cx<R>* __restrict__ data,
@@ -104,7 +106,15 @@ __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_lines_kernel(cx<R>*
   // twiddles W_L^{u k1} of a length-L line from the length-(M L) table: entry M u k1 mod (M L)
   auto load_w = [&](const void* twp, int u, auto uniform_c, auto m_c, auto len_c) {
     constexpr int M = decltype(m_c)::value, LEN = decltype(len_c)::value;
-    if constexpr (decltype(uniform_c)::value) {
+    if constexpr (PV_LINES && decltype(uniform_c)::value) {
+      CC* tw = (CC*)(twp) + u * N1;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = mk<R>(tw[k1].x, tw[k1].y);
+    } else if constexpr (PV_LINES) {
+      const C* __restrict__ tw = reinterpret_cast<const C*>(twp) + u * N1;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[k1];
+    } else if constexpr (decltype(uniform_c)::value) {
       CC* tw = (CC*)(twp);
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) w[k1] = mk<R>(tw[(M * u * k1) % LEN].x, tw[(M * u * k1) % LEN].y);
