@@ -125,8 +125,10 @@ int launch_horner_stage(int precision, int batch, int nx, int ny, int nz, const 
 
 // observables over one box of every packet: writes 7 doubles per packet to out (device)
 int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const double dx[3], const void* psi,
-                       const wfs_box& box, double* partials, long max_partials, double* out, long out_stride,
-                       hipStream_t s);
+                       const wfs_box* boxes, int nb, double* partials, long max_partials, double* out,
+                       long out_stride, hipStream_t s);
+// boxes measured in one launch_box_moments call
+constexpr int kBoxesPerPass = 4;
 long box_moments_partials_needed(const wfs_box& box);
 
 // probability current through a plane: out7[0] = flux (FP64), density optional [cu][cv]

@@ -10,21 +10,24 @@
 // which the reference computes in two kernels with float atomics
 // (volume_probability.cu:29-41, expected_location.cu:44-66).
 //
-// Each block owns a run of (x, y) rows of the box; a wavefront reduces with
-// cross-lane shuffles, waves combine through LDS, and the block writes one
-// partial record.  A second kernel (one block per packet) sums the partials in
+// Up to four boxes are measured in one pass over their bounding box (each point
+// read once, added to every box that holds it).  Each block owns 32 (x, y) rows,
+// a wavefront one row at a time (lanes along z, coalesced); the lanes reduce with
+// cross-lane shuffles at the end, waves combine through LDS, and the block
+// writes one partial record per box.  A second kernel (one block per packet) sums the partials in
 // a fixed order, so results are bit-reproducible run to run.  A stack of
 // independent grids of the same shape is handled by one launch
 // (blockIdx.y = packet).
 #include "wfs_internal.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace wfs {
 
 constexpr int kObsThreads = 256;
 constexpr int kObsQ = 7;
-constexpr int kRowsPerBlock = 4;
+constexpr int kRowsPerBlock = 32;   // rows (x, y) per block; wave w takes rows w, w + 4, ...
 
 __device__ __forceinline__ double simpson_s(int i, int n) {
   if (i == 0 || i == n - 1) return 1.0;
@@ -37,77 +40,124 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Up to kBoxesPerPass boxes measured in one pass over their bounding box: every point is
+// read once and adds to each box that holds it, with that box's Simpson weights.
+struct BoxSet {
+  int n;
+  int lo[3], cnt[3];                      // bounding box of the set
+  int bot[kBoxesPerPass][3], num[kBoxesPerPass][3];
+};
+
 template <typename R>
 __global__ __launch_bounds__(kObsThreads) void box_moments_kernel(const cx<R>* __restrict__ psi, int nx, int ny,
                                                                   int nz, double dx, double dy, double dz,
-                                                                  wfs_box box, double* __restrict__ partials) {
-  const long nrows = long(box.count[0]) * box.count[1];
+                                                                  BoxSet set, double* __restrict__ partials) {
+  constexpr int kWaves = kObsThreads / 64;
+  constexpr int NB = kBoxesPerPass;
+  const long nrows = long(set.cnt[0]) * set.cnt[1];
   // packet blockIdx.y of a stack of independent grids
   psi += long(blockIdx.y) * nx * ny * nz;
-  partials += long(blockIdx.y) * gridDim.x * kObsQ;
-  const long row0 = long(blockIdx.x) * kRowsPerBlock;
-  double acc[kObsQ];
-#pragma unroll
-  for (int q = 0; q < kObsQ; ++q) acc[q] = 0.0;
-
-  for (int rr = 0; rr < kRowsPerBlock; ++rr) {
-    const long row = row0 + rr;
-    if (row >= nrows) break;
-    const int ix = int(row / box.count[1]);
-    const int iy = int(row % box.count[1]);
-    const double wxy = simpson_s(ix, box.count[0]) * simpson_s(iy, box.count[1]);
-    const double rx = dx * double(box.bottom[0] + ix - nx / 2);
-    const double ry = dy * double(box.bottom[1] + iy - ny / 2);
-    const cx<R>* rowp = psi + (long(box.bottom[0] + ix) * ny + (box.bottom[1] + iy)) * long(nz) + box.bottom[2];
-    double s0 = 0.0, sz1 = 0.0, sz2 = 0.0;
-    for (int iz = threadIdx.x; iz < box.count[2]; iz += kObsThreads) {
-      const cx<R> v = rowp[iz];
-      const double rho = double(v.x) * double(v.x) + double(v.y) * double(v.y);
-      const double wr = simpson_s(iz, box.count[2]) * rho;
-      const double rz = dz * double(box.bottom[2] + iz - nz / 2);
-      s0 += wr;
-      sz1 += wr * rz;
-      sz2 += wr * rz * rz;
-    }
-    const double w = wxy;
-    acc[0] += w * s0;
-    acc[1] += w * rx * s0;
-    acc[2] += w * ry * s0;
-    acc[3] += w * sz1;
-    acc[4] += w * rx * rx * s0;
-    acc[5] += w * ry * ry * s0;
-    acc[6] += w * sz2;
-  }
-
-  __shared__ double red[kObsThreads / 64][kObsQ];
+  partials += long(blockIdx.y) * gridDim.x * set.n * kObsQ;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  const long row0 = long(blockIdx.x) * kRowsPerBlock;
+  // lane partial sums; every term is linear in the lane's z sums, so the row weights
+  // (uniform over the wavefront) multiply them before the one reduction at the end
+  double acc[NB][kObsQ];
 #pragma unroll
-  for (int q = 0; q < kObsQ; ++q) {
-    const double v = wave_sum(acc[q]);
-    if (lane == 0) red[wave][q] = v;
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int q = 0; q < kObsQ; ++q) acc[b][q] = 0.0;
+
+  for (int rr = wave; rr < kRowsPerBlock; rr += kWaves) {
+    const long row = row0 + rr;
+    if (row >= nrows) break;
+    const int gx = set.lo[0] + int(row / set.cnt[1]);
+    const int gy = set.lo[1] + int(row % set.cnt[1]);
+    double wxy[NB];
+    bool any = false;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int ix = gx - set.bot[b][0], iy = gy - set.bot[b][1];
+      const bool in = b < set.n && ix >= 0 && ix < set.num[b][0] && iy >= 0 && iy < set.num[b][1];
+      wxy[b] = in ? simpson_s(ix, set.num[b][0]) * simpson_s(iy, set.num[b][1]) : 0.0;
+      any = any || in;
+    }
+    if (!any) continue;
+    const double rx = dx * double(gx - nx / 2);
+    const double ry = dy * double(gy - ny / 2);
+    const cx<R>* rowp = psi + (long(gx) * ny + gy) * long(nz);
+    double s0[NB], sz1[NB], sz2[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) s0[b] = sz1[b] = sz2[b] = 0.0;
+    // 256 points of the row at a time: four loads issued together (indices clamped into the
+    // row, the excess lanes masked afterwards), so that each wavefront has four in flight
+    const int zend = set.lo[2] + set.cnt[2];
+    for (int z0 = set.lo[2]; z0 < zend; z0 += 256) {
+      cx<R> v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = rowp[min(z0 + lane + 64 * u, zend - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gz = z0 + lane + 64 * u;
+        const double rho = gz < zend ? double(v[u].x) * double(v[u].x) + double(v[u].y) * double(v[u].y) : 0.0;
+        const double rz = dz * double(gz - nz / 2);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int iz = gz - set.bot[b][2];
+          const bool in = wxy[b] != 0.0 && iz >= 0 && iz < set.num[b][2];
+          const double wr = in ? simpson_s(iz, set.num[b][2]) * rho : 0.0;
+          s0[b] += wr;
+          sz1[b] += wr * rz;
+          sz2[b] += wr * rz * rz;
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const double w = wxy[b];
+      acc[b][0] += w * s0[b];
+      acc[b][1] += w * rx * s0[b];
+      acc[b][2] += w * ry * s0[b];
+      acc[b][3] += w * sz1[b];
+      acc[b][4] += w * rx * rx * s0[b];
+      acc[b][5] += w * ry * ry * s0[b];
+      acc[b][6] += w * sz2[b];
+    }
+  }
+
+  __shared__ double red[kWaves][NB * kObsQ];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b >= set.n) break;
+#pragma unroll
+    for (int q = 0; q < kObsQ; ++q) {
+      const double v = wave_sum(acc[b][q]);
+      if (lane == 0) red[wave][b * kObsQ + q] = v;
+    }
   }
   __syncthreads();
-  if (threadIdx.x < kObsQ) {
+  if (threadIdx.x < set.n * kObsQ) {
     double v = 0.0;
 #pragma unroll
-    for (int w = 0; w < kObsThreads / 64; ++w) v += red[w][threadIdx.x];
-    partials[long(blockIdx.x) * kObsQ + threadIdx.x] = v;
+    for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];
+    partials[long(blockIdx.x) * set.n * kObsQ + threadIdx.x] = v;
   }
 }
 
+// one block per (box, packet): the box's records summed in a fixed order
 __global__ __launch_bounds__(kObsThreads) void sum_partials_kernel(const double* __restrict__ partials, long nparts,
-                                                                   double scale, double* __restrict__ out,
+                                                                   int nb, double scale, double* __restrict__ out,
                                                                    long out_stride) {
-  // one block per packet
-  partials += long(blockIdx.x) * nparts * kObsQ;
-  out += long(blockIdx.x) * out_stride;
+  const int box = blockIdx.x;
+  partials += long(blockIdx.y) * nparts * nb * kObsQ + long(box) * kObsQ;
+  out += long(blockIdx.y) * out_stride + long(box) * kObsQ;
   double acc[kObsQ];
 #pragma unroll
   for (int q = 0; q < kObsQ; ++q) acc[q] = 0.0;
-  for (long b = threadIdx.x; b < nparts; b += kObsThreads) {
+  for (long r = threadIdx.x; r < nparts; r += kObsThreads) {
 #pragma unroll
-    for (int q = 0; q < kObsQ; ++q) acc[q] += partials[b * kObsQ + q];
+    for (int q = 0; q < kObsQ; ++q) acc[q] += partials[r * nb * kObsQ + q];
   }
   __shared__ double red[kObsThreads / 64][kObsQ];
   const int lane = threadIdx.x & 63;
@@ -128,32 +178,51 @@ __global__ __launch_bounds__(kObsThreads) void sum_partials_kernel(const double*
 
 long box_moments_partials_needed(const wfs_box& box) {
   const long nrows = long(box.count[0]) * box.count[1];
-  return ((nrows + kRowsPerBlock - 1) / kRowsPerBlock) * kObsQ;
+  return ((nrows + kRowsPerBlock - 1) / kRowsPerBlock) * kObsQ * kBoxesPerPass;
 }
 
-// out[b * out_stride + q], q < 7, for every packet b < batch
+// out[p * out_stride + 7 b + q], q < 7, for the nb <= kBoxesPerPass boxes and every packet p
 int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const double dx[3], const void* psi,
-                       const wfs_box& box, double* partials, long max_partials, double* out, long out_stride,
-                       hipStream_t s) {
-  const long nrows = long(box.count[0]) * box.count[1];
+                       const wfs_box* boxes, int nb, double* partials, long max_partials, double* out,
+                       long out_stride, hipStream_t s) {
+  if (nb < 1 || nb > kBoxesPerPass) {
+    set_last_error("box moments: %d boxes in one pass (1..%d)", nb, kBoxesPerPass);
+    return WFS_ERR_INVALID;
+  }
+  BoxSet set{};
+  set.n = nb;
+  for (int a = 0; a < 3; ++a) {
+    int lo = boxes[0].bottom[a], hi = boxes[0].bottom[a] + boxes[0].count[a];
+    for (int b = 1; b < nb; ++b) {
+      lo = std::min(lo, boxes[b].bottom[a]);
+      hi = std::max(hi, boxes[b].bottom[a] + boxes[b].count[a]);
+    }
+    set.lo[a] = lo;
+    set.cnt[a] = hi - lo;
+    for (int b = 0; b < nb; ++b) {
+      set.bot[b][a] = boxes[b].bottom[a];
+      set.num[b][a] = boxes[b].count[a];
+    }
+  }
+  const long nrows = long(set.cnt[0]) * set.cnt[1];
   const long nblocks = (nrows + kRowsPerBlock - 1) / kRowsPerBlock;
-  if (nblocks * kObsQ * batch > max_partials) {
+  if (nblocks * kObsQ * nb * batch > max_partials) {
     set_last_error("observable workspace too small");
     return WFS_ERR_INVALID;
   }
   // the Simpson weight products carry 1/27 and the volume element dx dy dz
-  const double scale = dx[0] * dx[1] * dx[2] / 27.0;
+  const double sc = dx[0] * dx[1] * dx[2] / 27.0;
   if (precision == WFS_C128) {
     hipLaunchKernelGGL(box_moments_kernel<double>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
-                       static_cast<const double2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], box, partials);
+                       static_cast<const double2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], set, partials);
   } else {
     hipLaunchKernelGGL(box_moments_kernel<float>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
-                       static_cast<const float2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], box, partials);
+                       static_cast<const float2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], set, partials);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(batch)), dim3(kObsThreads), 0, s, partials, nblocks, scale, out,
-                     out_stride);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(nb), unsigned(batch)), dim3(kObsThreads), 0, s, partials,
+                     nblocks, nb, sc, out, out_stride);
   return int(hipGetLastError());
 }
 
@@ -306,7 +375,7 @@ int launch_plane_current(int precision, int nx, int ny, int nz, const double dx[
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
   // Simpson 1/9 and the pixel area dW dH
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kObsThreads), 0, s, partials, nblocks,
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kObsThreads), 0, s, partials, nblocks, 1,
                      p.dWd * p.dHd / 9.0, out7, 7L);
   return int(hipGetLastError());
 }

@@ -1,6 +1,7 @@
 // Plans, the time-step driver and the remaining C ABI entry points.
 #include "wfs_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -637,9 +638,13 @@ extern "C" int wfs_observables(wfs_plan* p, const void* psi, const wfs_box* boxe
         return WFS_ERR_INVALID;
       }
     }
+  }
+  // up to kBoxesPerPass boxes per pass over their bounding box
+  for (int b0 = 0; b0 < nboxes; b0 += kBoxesPerPass) {
+    const int nb = std::min(kBoxesPerPass, nboxes - b0);
     ProfileScope scope(p, WFS_K_OBSERVABLES, s);
-    const int rc = launch_box_moments(p->precision, p->batch, p->n[0], p->n[1], p->n[2], p->dx, psi, bx,
-                                      p->obs_work, p->obs_work_len, out + 7 * b, 7L * nboxes, s);
+    const int rc = launch_box_moments(p->precision, p->batch, p->n[0], p->n[1], p->n[2], p->dx, psi, boxes + b0,
+                                      nb, p->obs_work, p->obs_work_len, out + 7 * b0, 7L * nboxes, s);
     if (rc) return rc;
   }
   return 0;
