@@ -105,9 +105,24 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
   __shared__ C sw[3][WY][WZ];
 
   const int tid = threadIdx.x;
-  const int z0 = blockIdx.x * TZ, y0 = blockIdx.y * TY;
-  const int packet = blockIdx.z / ntiles;
-  const int x0 = (blockIdx.z % ntiles) * tile_x;
+  // Blocks are dispatched round-robin over the 8 XCDs in linear order.  Give each XCD a
+  // contiguous run of tiles (z fastest, then y, then x slab): the tiles of one XCD march the
+  // same x planes together, so the y/z halo lines one tile fetches are L2 hits for its
+  // neighbours instead of second HBM reads (the order changes nothing in the arithmetic).
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  {
+    const long total = long(gridDim.x) * gridDim.y * gridDim.z;
+    if (total % 8 == 0) {
+      const long lin = blockIdx.x + long(gridDim.x) * (blockIdx.y + long(gridDim.y) * blockIdx.z);
+      const long logical = (lin % 8) * (total / 8) + lin / 8;
+      bx = int(logical % gridDim.x);
+      by = int((logical / gridDim.x) % gridDim.y);
+      bz = int(logical / (long(gridDim.x) * gridDim.y));
+    }
+  }
+  const int z0 = bx * TZ, y0 = by * TY;
+  const int packet = bz / ntiles;
+  const int x0 = (bz % ntiles) * tile_x;
   int x1 = x0 + tile_x;
   if (x1 > nx) x1 = nx;
   const long poff = long(packet) * nx * ny * nz;
