@@ -231,6 +231,8 @@ def batch_bench(steps, warmup):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "48 harmonic packets, grids {64,96,128,192}^3 drawn with default_rng(0), dt=0.01, "
                                "dx=0.5", "packets_per_grid": sizes, "graph_nodes": g.node_count()},
+        "variants_per_grid": {str(c.key[0][0]): {k: v for k, v in c.plan.variants().items()
+                                                 if k in ("yz_plane", "yz_slab", "x_slab")} for c in batch.classes},
         "unbatched_eager_single_step_calls": {"value": pts * seq_steps / seq, "unit": "grid-pt-steps/s",
                                               "steps": seq_steps},
     }

@@ -1,6 +1,6 @@
-"""Two-launch split step for complex64 grids with ny = nz = 256 (the slab scheme).
+"""Two-launch split step for complex64 grids with ny = nz = 256 or 192 (the slab scheme).
 
-The length-256 y transform is split in a length-64 stage on the rows y = 4 a + b (fused with
+The length-n y transform is split in a length-n/4 stage on the rows y = 4 a + b (fused with
 the z inverse / potential / z forward work) and a length-4 stage fused with the x pass.  Its
 arithmetic differs from the four axis passes only in how the y transform is factored, so the
 two paths agree to float rounding; both are pinned to the FP64 oracle at the complex64 bar.
@@ -58,9 +58,9 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("form", FORMS)
-@pytest.mark.parametrize("nx", [8, 16, 32])
-def test_slab_matches_oracle_table_potential(dev, nx, form):
-    shape = (nx, 256, 256)
+@pytest.mark.parametrize("nx,n", [(8, 256), (16, 256), (32, 256), (8, 192), (16, 192), (96, 192)])
+def test_slab_matches_oracle_table_potential(dev, nx, n, form):
+    shape = (nx, n, n)
     psi0, V = _inputs(shape, 100 + nx)
     print(f"case: slab {form}, table potential, shape {shape}, 5 steps", flush=True)
     got = _run(shape, form, psi0, V).cpu().numpy()
@@ -75,8 +75,9 @@ def test_slab_matches_oracle_table_potential(dev, nx, form):
 
 
 @pytest.mark.parametrize("form", FORMS)
-def test_slab_matches_oracle_separable_potential(dev, form):
-    shape = (16, 256, 256)
+@pytest.mark.parametrize("n", [192, 256])
+def test_slab_matches_oracle_separable_potential(dev, n, form):
+    shape = (16, n, n)
     psi0, _ = _inputs(shape, 7)
     rng = np.random.default_rng(8)
     axes = [rng.uniform(-1.0, 1.0, (1, m)) for m in shape]
@@ -102,9 +103,10 @@ def test_slab_single_step_and_entry_exit(dev, form):
 
 
 @pytest.mark.parametrize("form", FORMS)
-def test_slab_batched_bit_identical_to_single_runs(dev, form):
+@pytest.mark.parametrize("n", [192, 256])
+def test_slab_batched_bit_identical_to_single_runs(dev, n, form):
     batch = 3
-    shape = (8, 256, 256)
+    shape = (8, n, n)
     rng = np.random.default_rng(21)
     axes = [rng.uniform(-1.0, 1.0, (batch, m)) for m in shape]
     psis = [_inputs(shape, 30 + i)[0] for i in range(batch)]
@@ -116,14 +118,15 @@ def test_slab_batched_bit_identical_to_single_runs(dev, form):
         assert torch.equal(both[i], one), f"packet {i}"
 
 
-def test_slab_full_grid_against_passes_and_norm(dev):
-    """256^3 (the benchmark grid): no absorber, so the norm is kept; slab vs four passes"""
-    shape = (256, 256, 256)
-    psi0 = oracle.gaussian_wave_packet(shape, (80 / 256,) * 3, (4.0, 4.0, 4.0), (-16.0, 0.0, 0.0),
+@pytest.mark.parametrize("n", [192, 256])
+def test_slab_full_grid_against_passes_and_norm(dev, n):
+    """n^3 (256^3 is the benchmark grid): no absorber, so the norm is kept; slab vs four passes"""
+    shape = (n, n, n)
+    psi0 = oracle.gaussian_wave_packet(shape, (80 / n,) * 3, (4.0, 4.0, 4.0), (-16.0, 0.0, 0.0),
                                        (2.0, 0.0, 0.0), np.complex64)
     rng = np.random.default_rng(5)
     V = (rng.uniform(0.0, 1.0, shape)).astype(np.complex64)
-    p_dx = (80 / 256,) * 3
+    p_dx = (80 / n,) * 3
     out = {}
     for form in [(0, 1)] + FORMS:
         p = _abi.Plan(shape, p_dx, 0.005, 1.0, C64, _abi.WFS_METHOD_FOURIER)
@@ -139,7 +142,7 @@ def test_slab_full_grid_against_passes_and_norm(dev):
         b = out[form]
         rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a))
         n1 = float(torch.sum(torch.abs(b.to(torch.complex128)) ** 2))
-        print(f"case: 256^3 x 20 steps, slab {form}: vs passes rel L2 {rel:.3e}, norm drift {abs(n1 / n0 - 1):.3e}")
+        print(f"case: {n}^3 x 20 steps, slab {form}: vs passes rel L2 {rel:.3e}, norm drift {abs(n1 / n0 - 1):.3e}")
         assert rel < TOL_PASSES
         assert abs(n1 / n0 - 1) < 1e-5
     # the forms differ only in how blocks are scheduled: identical results
@@ -151,6 +154,8 @@ def test_slab_defaults_and_limits(dev):
         return _abi.Plan(shape, DX, DT, 1.0, precision, _abi.WFS_METHOD_FOURIER, batch=batch).variants()["yz_slab"]
     assert variant((256, 256, 256)) == 2
     assert variant((16, 256, 256), batch=4) == 2
+    assert variant((192, 192, 192)) == 1
+    assert variant((64, 192, 192), batch=3) == 1
     assert variant((256, 256, 256), C128) == 0
     assert variant((512, 256, 256)) == 0
     assert variant((256, 128, 128)) == 0

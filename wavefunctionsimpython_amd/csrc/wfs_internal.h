@@ -73,9 +73,9 @@ struct YzPlaneParams {
 bool yz_plane_supported(int precision, int ny, int nz, int halves);
 int launch_yz_plane(int precision, int n, int halves, void* data, const YzPlaneParams& p, hipStream_t s);
 
-// Two-pass split step for complex64 grids with ny = nz = 256 (wfs_slab.hip): the y
-// transform split in a length-64 stage (on rows y = 4 a + b, with the z work) and a length-4
-// stage (with the x pass); see there.
+// Two-launch split step for complex64 grids with ny = nz = 256 or 192 (wfs_slab.hip): the y
+// transform split in a length-ny/4 stage (on rows y = 4 a + b, with the z work) and a
+// length-4 stage (with the x pass); see there.
 enum {
   LINES_FULL = 0,    // y inverse, z inverse, potential phase, z forward, y forward
   LINES_ENTRY = 1,   // z forward, y forward (position space in)
@@ -95,15 +95,15 @@ bool slab_supported(int precision, const int n[3]);
 struct SlabRows {
   int b, uy, uz, n1;
 };
-SlabRows slab_rows();
-// the potential table order of the slab scheme's y-z launch
-LinesOrder slab_lines_order();
+SlabRows slab_rows(int n);
+// the potential table order of the slab scheme's y-z launch (ny = nz = n)
+LinesOrder slab_lines_order(int n);
 // YzPlaneParams.planes = packets * nx; variant 2 = persistent (one block per CU walking the
 // row groups, loading the next group during the current one's arithmetic), else one block
 // per group
-int launch_yz_slab(int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s);
+int launch_yz_slab(int n, int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s);
 // variant 2: 8 z points per block, else 16
-int launch_x_slab(int nx, int variant, void* data, const XSlabParams& p, hipStream_t s);
+int launch_x_slab(int nx, int n, int variant, void* data, const XSlabParams& p, hipStream_t s);
 
 // pv = cexp_i(-dt V) elementwise in the state precision
 int launch_potential_phase(int precision, long count, const float2* V, double dt, void* pv, hipStream_t s);

@@ -281,7 +281,7 @@ int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
   if (slab_on(p)) {
     WFS_TRY(alloc_table(p, &p->pv_lines));
     WFS_TRY(launch_potential_phase_lines(p->precision, count, static_cast<const float2*>(V), p->dt, p->pv_lines,
-                                         slab_lines_order(), s));
+                                         slab_lines_order(p->n[1]), s));
     p->pv_lines_ok = true;
     p->pv_nat_ok = false;
   } else {
@@ -304,14 +304,14 @@ int current_table(wfs_plan* p, hipStream_t s, const void** out) {
   if (slab_on(p)) {
     if (!p->pv_lines_ok) {
       WFS_TRY(alloc_table(p, &p->pv_lines));
-      WFS_TRY(launch_table_reorder(p->precision, count, p->pv, p->pv_lines, slab_lines_order(), true, s));
+      WFS_TRY(launch_table_reorder(p->precision, count, p->pv, p->pv_lines, slab_lines_order(p->n[1]), true, s));
       p->pv_lines_ok = true;
     }
     *out = p->pv_lines;
   } else {
     if (!p->pv_nat_ok) {
       WFS_TRY(alloc_table(p, &p->pv));
-      WFS_TRY(launch_table_reorder(p->precision, count, p->pv_lines, p->pv, slab_lines_order(), false, s));
+      WFS_TRY(launch_table_reorder(p->precision, count, p->pv_lines, p->pv, slab_lines_order(p->n[1]), false, s));
       p->pv_nat_ok = true;
     }
     *out = p->pv;
@@ -375,7 +375,7 @@ int run_yz(wfs_plan* p, void* data, const void* pv, hipStream_t s, int sel) {
 // LINES_FULL), or the entry (z, y forward) / exit (y, z inverse) launch of a call
 int run_slab_lines(wfs_plan* p, int mode, void* data, const void* pv, hipStream_t s, int sel) {
   YzPlaneParams pp;
-  const SlabRows g = slab_rows();
+  const SlabRows g = slab_rows(p->n[1]);
   const int c = sel % p->ntab;
   pp.tw_yi = static_cast<const char*>(p->tw_rows[0]) + size_t(c) * g.uy * g.n1 * p->elem;
   pp.tw_yf = pp.tw_yi;
@@ -394,7 +394,7 @@ int run_slab_lines(wfs_plan* p, int mode, void* data, const void* pv, hipStream_
   pp.nx = p->n[0];
   pp.planes = long(p->batch) * p->n[0];
   ProfileScope scope(p, mode == LINES_FULL ? WFS_K_YZ_SLAB : WFS_K_SLAB_EDGE, s);
-  return launch_yz_slab(mode, p->variant[WFS_K_YZ_SLAB], data, pp, s);
+  return launch_yz_slab(p->n[1], mode, p->variant[WFS_K_YZ_SLAB], data, pp, s);
 }
 
 // Slab scheme: length-4 y stages around x forward -> K -> x inverse
@@ -407,7 +407,7 @@ int run_x_slab(wfs_plan* p, void* data, const void* const* ktab, hipStream_t s, 
   xp.kz = table_copy(p, ktab[2], 2, sel);
   xp.packets = p->batch;
   ProfileScope scope(p, WFS_K_X_SLAB, s);
-  return launch_x_slab(p->n[0], p->variant[WFS_K_X_SLAB], data, xp, s);
+  return launch_x_slab(p->n[0], p->n[1], p->variant[WFS_K_X_SLAB], data, xp, s);
 }
 
 int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
@@ -553,10 +553,12 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->variant[WFS_K_YZ_PLANE] =
       (yz_plane_supported(precision, n[1], n[2], 2) && n[1] == 128 && long(batch) * n[0] >= 512) ? 2
       : yz_plane_supported(precision, n[1], n[2], 1) ? 1 : 0;
-  // slab scheme at ny = nz = 256 (complex64): persistent y-z launch, 8-z x blocks (measured
-  // fastest at 256^3: 90 and 66 us against 115 and 79 for the other forms)
-  p->variant[WFS_K_YZ_SLAB] = slab_supported(precision, n) ? 2 : 0;
-  if (slab_supported(precision, n)) p->variant[WFS_K_X_SLAB] = 2;
+  // slab scheme at ny = nz = 256 or 192 (complex64).  At 256 the persistent y-z launch and
+  // 8-z x blocks (measured fastest at 256^3: 90 and 66 us against 115 and 79 for the other
+  // forms); at 192 one block per row group (two 72 KiB blocks per CU) and 16-z x blocks
+  // (192^3 x 15: 662 and 460 us; 8-z x blocks 514 us)
+  p->variant[WFS_K_YZ_SLAB] = slab_supported(precision, n) ? (n[1] == 256 ? 2 : 1) : 0;
+  if (slab_supported(precision, n)) p->variant[WFS_K_X_SLAB] = n[1] == 256 ? 2 : 1;
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;
@@ -570,7 +572,7 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
     rc = upload_dithered(&p->tw[a], re, im, precision);
     if (rc) break;
     if (a > 0 && slab_supported(precision, n)) {
-      const SlabRows g = slab_rows();
+      const SlabRows g = slab_rows(n[1]);
       rc = a == 1 ? upload_rows_table(&p->tw_rows[0], re, im, g.b, g.uy, g.n1)
                   : upload_rows_table(&p->tw_rows[1], re, im, 1, g.uz, g.n1);
       if (rc) break;
@@ -683,7 +685,7 @@ extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
     return WFS_ERR_UNSUPPORTED;
   }
   if (kc == WFS_K_YZ_SLAB && variant > 0 && !slab_supported(p->precision, p->n)) {
-    set_last_error("wfs_plan_set_variant: the slab scheme needs complex64, ny = nz = 256 and nx <= 256");
+    set_last_error("wfs_plan_set_variant: the slab scheme needs complex64, ny = nz in {192, 256} and nx <= 256");
     return WFS_ERR_UNSUPPORTED;
   }
   p->variant[kc] = variant;

@@ -1,15 +1,16 @@
-// Two-pass split step for complex64 grids with ny = nz = 256 (gfx950).
+// Two-launch split step for complex64 grids with ny = nz = 256 or 192 (gfx950).
 //
-// A 256^2 y-z plane (512 KiB) does not fit in one CU, so the y transform is split in two
-// stages, ny = LY * B with LY = 64, B = 4, y = B a + b, ky = ka + LY kb:
+// A 256^2 y-z plane (512 KiB) or a 192^2 one (288 KiB) does not fit in one CU, so the y
+// transform is split in two stages, ny = LY * B with B = 4 (LY = 64 or 48), y = B a + b,
+// ky = ka + LY kb:
 //   X[ka + LY kb] = sum_b W_B^{b kb} W_NY^{b ka} sum_a x[B a + b] W_LY^{a ka}
 // and each stage rides on a pass that is needed anyway:
 //
-//   yz_lines_kernel<NZ 256, LY 64, B 4>  (one block per x plane and residue b: the 64 rows
-//     y = 4 a + b with all z, 128 KiB):  length-64 inverse y stage over a, z inverse,
-//     potential phase, z forward, length-64 forward y stage
-//   x_slab_kernel  (one block per packet, ka and 16 consecutive z: the 4 rows y = 4 ka + b
-//     with all x):  twiddles W_NY^{b ka} and length-4 forward y stage over b, x forward,
+//   yz_lines_kernel<NZ, LY, B 4>  (one block per x plane and residue b: the LY rows
+//     y = 4 a + b with all z, 128 KiB at 256, 72 KiB at 192):  length-LY inverse y stage
+//     over a, z inverse, potential phase, z forward, length-LY forward y stage
+//   x_slab_kernel  (one block per packet, ka and 8 or 16 consecutive z: the 4 rows
+//     y = 4 ka + b with all x):  twiddles W_NY^{b ka} and length-4 forward y stage over b, x forward,
 //     kinetic phase K(kx, ky, kz), x inverse, length-4 inverse y stage and twiddles
 //
 // Between the passes the y axis holds the half-transformed Y[b][ka] at row B ka + b.  The
@@ -26,7 +27,7 @@ namespace wfs {
 
 namespace {
 
-constexpr int kSlabNY = 256, kSlabNZ = 256, kSlabLY = 64, kSlabB = kSlabNY / kSlabLY;
+constexpr int kSlabB = 4;   // y rows y = 4 a + b: the length-4 stage rides on the x launch
 
 // Lane pairs: the partner lanes tid and tid ^ ZB share the (x, z) points of a block; lane
 // q = (tid / ZB) % 2 holds rows b = q and q + 2 (and, after the forward y stage, ky
@@ -178,97 +179,120 @@ int cu_count() {
 }
 
 // persistent: one block per CU walking the row groups, else one block per group
-template <int MODE, bool PERSIST>
+template <int N, int MODE, bool PERSIST>
 int launch_lines_m(void* data, const YzPlaneParams& p, hipStream_t s) {
-  static_assert(yz_lines_lds_bytes<float, kSlabNZ, kSlabLY, 1>() <= 160 * 1024, "rows do not fit in LDS");
+  constexpr int LY = N / kSlabB;
+  static_assert(yz_lines_lds_bytes<float, N, LY, 1>() <= 160 * 1024, "rows do not fit in LDS");
   const long groups = p.planes * kSlabB;
   const long grid = PERSIST ? std::min<long>(groups, cu_count()) : groups;
-  hipLaunchKernelGGL((yz_lines_kernel<float, kSlabNZ, kSlabLY, kSlabB, 1, MODE, PERSIST, true>), dim3(unsigned(grid)),
-                     dim3(yz_lines_threads<kSlabNZ, kSlabLY, 1>()), 0, s, reinterpret_cast<float2*>(data), p);
+  hipLaunchKernelGGL((yz_lines_kernel<float, N, LY, kSlabB, 1, MODE, PERSIST, true>), dim3(unsigned(grid)),
+                     dim3(yz_lines_threads<N, LY, 1>()), 0, s, reinterpret_cast<float2*>(data), p);
   return int(hipGetLastError());
 }
 
-template <int MODE>
-int launch_lines_mode(int variant, void* data, const YzPlaneParams& p, hipStream_t s) {
-  return variant == 2 ? launch_lines_m<MODE, true>(data, p, s) : launch_lines_m<MODE, false>(data, p, s);
-}
-
-template <int NX, int ZB>
-int launch_x_slab_z(void* data, const XSlabParams& p, hipStream_t s) {
-  constexpr long per_packet = long(kSlabNY / kSlabB) * (kSlabNZ / ZB);
-  static_assert(per_packet % 16 == 0, "the XCD pairing needs a multiple of 16 blocks");
-  hipLaunchKernelGGL((x_slab_kernel<float, NX, kSlabNY, kSlabNZ, kSlabB, ZB>), dim3(unsigned(p.packets * per_packet)),
-                     dim3(2 * ZB * split_of<NX>::n2), 0, s, reinterpret_cast<float2*>(data), p);
-  return int(hipGetLastError());
-}
-
-// variant 2: 8 z points per block (256 threads at nx = 256, two blocks per CU), else 16
-template <int NX>
-int launch_x_slab_n(int variant, void* data, const XSlabParams& p, hipStream_t s) {
-  return variant == 2 ? launch_x_slab_z<NX, 8>(data, p, s) : launch_x_slab_z<NX, 16>(data, p, s);
-}
-
-}  // namespace
-
-// complex64, ny = nz = 256, nx a supported FFT length up to 256 (at 512 the x lines of a
-// lane pair do not fit in registers)
-bool slab_supported(int precision, const int n[3]) {
-  return precision == WFS_C64 && n[1] == kSlabNY && n[2] == kSlabNZ && n[0] <= 256 && fft_length_supported(n[0]);
-}
-
-SlabRows slab_rows() {
-  SlabRows g;
-  g.b = kSlabB;
-  g.n1 = kSlabNZ / split_of<kSlabNZ>::n2;
-  g.uy = kSlabLY / g.n1;
-  g.uz = split_of<kSlabNZ>::n2;
-  return g;
-}
-
-LinesOrder slab_lines_order() {
-  LinesOrder m;
-  m.ny = kSlabNY;
-  m.nz = kSlabNZ;
-  m.ly = kSlabLY;
-  m.b = kSlabB;
-  m.rh = kSlabLY;
-  m.n2z = split_of<kSlabNZ>::n2;
-  m.n1 = kSlabNZ / split_of<kSlabNZ>::n2;
-  m.threads = yz_lines_threads<kSlabNZ, kSlabLY, 1>();
-  return m;
-}
-
-int launch_yz_slab(int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s) {
-  if (p.planes <= 0 || p.planes * kSlabB > 0x7fffffffL) {
-    set_last_error("slab y-z pass: bad plane count %ld", p.planes);
-    return WFS_ERR_INVALID;
-  }
-  if (mode == LINES_FULL) return launch_lines_mode<LINES_FULL>(variant, data, p, s);
-  if (mode == LINES_ENTRY) return launch_lines_mode<LINES_ENTRY>(variant, data, p, s);
-  if (mode == LINES_EXIT) return launch_lines_mode<LINES_EXIT>(variant, data, p, s);
+template <int N>
+int launch_lines_n(int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s) {
+  const bool persist = variant == 2;
+  if (mode == LINES_FULL)
+    return persist ? launch_lines_m<N, LINES_FULL, true>(data, p, s) : launch_lines_m<N, LINES_FULL, false>(data, p, s);
+  if (mode == LINES_ENTRY)
+    return persist ? launch_lines_m<N, LINES_ENTRY, true>(data, p, s) : launch_lines_m<N, LINES_ENTRY, false>(data, p, s);
+  if (mode == LINES_EXIT)
+    return persist ? launch_lines_m<N, LINES_EXIT, true>(data, p, s) : launch_lines_m<N, LINES_EXIT, false>(data, p, s);
   set_last_error("slab y-z pass: bad mode %d", mode);
   return WFS_ERR_INVALID;
 }
 
-int launch_x_slab(int nx, int variant, void* data, const XSlabParams& p, hipStream_t s) {
-  const long blocks = p.packets * long(kSlabNY / kSlabB) * (kSlabNZ / 8);
-  if (p.packets <= 0 || blocks > 0x7fffffffL) {
-    set_last_error("slab x pass: bad packet count %ld", p.packets);
-    return WFS_ERR_INVALID;
-  }
+template <int NX, int N, int ZB>
+int launch_x_slab_z(void* data, const XSlabParams& p, hipStream_t s) {
+  constexpr long per_packet = long(N / kSlabB) * (N / ZB);
+  static_assert(per_packet % 16 == 0, "the XCD pairing needs a multiple of 16 blocks");
+  hipLaunchKernelGGL((x_slab_kernel<float, NX, N, N, kSlabB, ZB>), dim3(unsigned(p.packets * per_packet)),
+                     dim3(2 * ZB * split_of<NX>::n2), 0, s, reinterpret_cast<float2*>(data), p);
+  return int(hipGetLastError());
+}
+
+// variant 2: 8 z points per block (256 threads at nx = 256), else 16
+template <int N>
+int launch_x_slab_ny(int nx, int variant, void* data, const XSlabParams& p, hipStream_t s) {
+#define WFS_X_SLAB_CASE(NX)                                                                              \
+  case NX:                                                                                               \
+    return variant == 2 ? launch_x_slab_z<NX, N, 8>(data, p, s) : launch_x_slab_z<NX, N, 16>(data, p, s);
   switch (nx) {
-    case 8:   return launch_x_slab_n<8>(variant, data, p, s);
-    case 16:  return launch_x_slab_n<16>(variant, data, p, s);
-    case 32:  return launch_x_slab_n<32>(variant, data, p, s);
-    case 64:  return launch_x_slab_n<64>(variant, data, p, s);
-    case 96:  return launch_x_slab_n<96>(variant, data, p, s);
-    case 128: return launch_x_slab_n<128>(variant, data, p, s);
-    case 192: return launch_x_slab_n<192>(variant, data, p, s);
-    case 256: return launch_x_slab_n<256>(variant, data, p, s);
+    WFS_X_SLAB_CASE(8)
+    WFS_X_SLAB_CASE(16)
+    WFS_X_SLAB_CASE(32)
+    WFS_X_SLAB_CASE(64)
+    WFS_X_SLAB_CASE(96)
+    WFS_X_SLAB_CASE(128)
+    WFS_X_SLAB_CASE(192)
+    WFS_X_SLAB_CASE(256)
     default:
       set_last_error("slab x pass: unsupported length %d", nx);
       return WFS_ERR_UNSUPPORTED;
   }
+#undef WFS_X_SLAB_CASE
+}
+
+}  // namespace
+
+// complex64, ny = nz in {192, 256}, nx a supported FFT length up to 256 (at 512 the x lines
+// of a lane pair do not fit in registers)
+bool slab_supported(int precision, const int n[3]) {
+  return precision == WFS_C64 && n[1] == n[2] && (n[1] == 192 || n[1] == 256) && n[0] <= 256 &&
+         fft_length_supported(n[0]);
+}
+
+template <int N>
+static SlabRows slab_rows_n() {
+  SlabRows g;
+  g.b = kSlabB;
+  g.n1 = N / split_of<N>::n2;
+  g.uy = (N / kSlabB) / g.n1;
+  g.uz = split_of<N>::n2;
+  return g;
+}
+
+SlabRows slab_rows(int n) { return n == 192 ? slab_rows_n<192>() : slab_rows_n<256>(); }
+
+template <int N>
+static LinesOrder slab_lines_order_n() {
+  constexpr int LY = N / kSlabB;
+  LinesOrder m;
+  m.ny = N;
+  m.nz = N;
+  m.ly = LY;
+  m.b = kSlabB;
+  m.rh = LY;
+  m.n2z = split_of<N>::n2;
+  m.n1 = N / split_of<N>::n2;
+  m.threads = yz_lines_threads<N, LY, 1>();
+  return m;
+}
+
+LinesOrder slab_lines_order(int n) { return n == 192 ? slab_lines_order_n<192>() : slab_lines_order_n<256>(); }
+
+int launch_yz_slab(int n, int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s) {
+  if (p.planes <= 0 || p.planes * kSlabB > 0x7fffffffL) {
+    set_last_error("slab y-z pass: bad plane count %ld", p.planes);
+    return WFS_ERR_INVALID;
+  }
+  if (n == 256) return launch_lines_n<256>(mode, variant, data, p, s);
+  if (n == 192) return launch_lines_n<192>(mode, variant, data, p, s);
+  set_last_error("slab y-z pass: unsupported plane %d", n);
+  return WFS_ERR_UNSUPPORTED;
+}
+
+int launch_x_slab(int nx, int n, int variant, void* data, const XSlabParams& p, hipStream_t s) {
+  const long blocks = p.packets * long(n / kSlabB) * (n / 8);
+  if (p.packets <= 0 || blocks > 0x7fffffffL) {
+    set_last_error("slab x pass: bad packet count %ld", p.packets);
+    return WFS_ERR_INVALID;
+  }
+  if (n == 256) return launch_x_slab_ny<256>(nx, variant, data, p, s);
+  if (n == 192) return launch_x_slab_ny<192>(nx, variant, data, p, s);
+  set_last_error("slab x pass: unsupported plane %d", n);
+  return WFS_ERR_UNSUPPORTED;
 }
 
 }  // namespace wfs
