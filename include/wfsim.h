@@ -233,18 +233,20 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * inverse / potential / z forward and y forward passes of each step as one
  * kernel on whole x planes held in LDS; variant 0 runs the three passes.  The
  * results are identical; the autotuner keeps the faster.
- * WFS_K_YZ_SLAB: variants 1 and 2 (2 is the default where supported:
- * complex64 grids with ny = nz = 256 and nx <= 256) run a step as two
- * launches, the y transform split in a length-64 stage fused with the z work
+ * WFS_K_YZ_SLAB: variants 1 and 2 (the default where supported: complex64
+ * grids with ny = nz = 256 or 192 and nx <= 256; 2 at 256, 1 at 192) run a
+ * step as two launches, the y transform split in a length-ny/4 stage fused
+ * with the z work
  * (kernel class WFS_K_YZ_SLAB; the entry and exit launches of a call count
  * as WFS_K_SLAB_EDGE) and a length-4 stage fused with the x pass
  * (WFS_K_X_SLAB); variant 2 runs the y-z launch as one persistent block per
  * CU that loads the next row group during the current one's arithmetic;
  * variant 0 runs the four axis passes.  The results agree to float rounding
  * (the length-256 y transform is factored differently); the slab variants
- * among themselves are bit-identical.  WFS_K_X_SLAB: variant 1 puts 16 z
- * points in an x block, variant 2 (default) 8, pairing the two blocks of a
- * 128-byte line on one XCD.  The autotuner keeps the fastest.
+ * among themselves are bit-identical.  WFS_K_X_SLAB: variant 1 (default at
+ * 192) puts 16 z points in an x block, variant 2 (default at 256) 8, pairing
+ * the two blocks of a 128-byte line on one XCD.  The autotuner keeps the
+ * fastest.
  */
 int wfs_plan_autotune(wfs_plan* plan, void* stream);
 int wfs_plan_set_variant(wfs_plan* plan, int kernel_class, int variant);
