@@ -65,15 +65,10 @@ def build_workload(dev, n=N):
     return shape, dx, psi, V, boxes
 
 
-def run_chunks(plan, psi, V, boxes, steps, out):
-    k = 0
-    chunk = 0
-    while k < steps:
-        s = min(OBS_EVERY, steps - k)
-        plan.observables(psi, boxes, out=out[chunk % out.shape[0]])
-        plan.step(psi, V, s)
-        k += s
-        chunk += 1
+def run_chunks(plan, psi, V, boxes, steps, out=None):
+    """steps steps, the boxes measured before every OBS_EVERY-th (core_sim's measure-then-evolve
+    order): one wfs_step_observe call.  Returns the observations (float64, device)."""
+    return plan.step_observe(psi, V, steps, OBS_EVERY, boxes, out=out)
 
 
 def gpu_bench(steps, warmup):
@@ -84,7 +79,8 @@ def gpu_bench(steps, warmup):
     plan = _abi.Plan(shape, dx, DT, 1.0, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER)
     variants = plan.autotune()
     plan.set_potential(V)          # phase table cexp(-i dt V), kept by the plan
-    out = torch.zeros((64, len(boxes), 7), dtype=torch.float64, device=dev)
+    nobs = max((max(steps, warmup, 40) + OBS_EVERY - 1) // OBS_EVERY, 1)
+    out = torch.zeros((nobs, len(boxes), 7), dtype=torch.float64, device=dev)
     run_chunks(plan, psi, None, boxes, warmup, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -92,7 +88,7 @@ def gpu_bench(steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
-    norm = float(out[0, 0, 0].item())
+    norm = float(out[0, 0, 0].item())     # first observation of the timed run: the state after warm-up
 
     # instrumented window: HIP events around every launch, same stream, same workload
     plan.set_profiling(True)

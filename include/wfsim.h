@@ -132,6 +132,22 @@ int wfs_observables(wfs_plan* plan, const void* psi, const wfs_box* boxes, int n
                     void* stream);
 
 /*
+ * nsteps steps with the box observables of the state before steps 0,
+ * observe_every, 2 observe_every, ... (< nsteps) written to
+ * out[o][packet][box][7] (device; o < ceil(nsteps / observe_every)): the
+ * results of alternating wfs_observables and wfs_step(observe_every) calls,
+ * which is how core_sim.run_iteration measures before each step
+ * (sources/core_sim.py:19-65, sources/measurement.py:644-704) with the step
+ * count between measurements of the SURVEY's wfs_step(..., observe_every, ...)
+ * boundary.  With the slab scheme (WFS_K_YZ_SLAB) the exit launch of one
+ * stretch, the observation and the entry launch of the next run as one launch;
+ * the state is bit-identical to the alternating calls and the observables
+ * agree to FP64 rounding (summation order).  V as for wfs_step.
+ */
+int wfs_step_observe(wfs_plan* plan, void* psi, const void* V, int nsteps, int observe_every,
+                     const wfs_box* boxes, int nboxes, double* out, void* stream);
+
+/*
  * Probability current through a plane (sources/measurement.py:190-302,
  * sources/cuda_kernels/probability_current_density.cu:4-218).  count[2] are
  * the (odd) pixel counts, size[2] the plane's width and height in Bohr radii,

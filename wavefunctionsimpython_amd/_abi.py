@@ -67,6 +67,8 @@ SIGNATURES = {
     "wfs_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "wfs_fft3d": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "wfs_observables": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(WfsBox), ctypes.c_int, _vp, _vp]),
+    "wfs_step_observe": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(WfsBox),
+                                        ctypes.c_int, _vp, _vp]),
     "wfs_plane_current": (ctypes.c_int, [_vp, _vp, _c_dbl3, _c_dbl3, ctypes.c_double * 2, ctypes.c_int * 2, _vp,
                                          _vp, _vp]),
     "wfs_init_gaussian": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_int, _c_dbl3, _c_dbl3, _c_dbl3, _vp, _vp]),
@@ -269,9 +271,7 @@ class Plan:
         check_state(data, self.shape, self.precision, "data")
         check(lib().wfs_fft3d(self._h, ptr(data), int(direction), stream_handle(stream)), "wfs_fft3d")
 
-    def observables(self, psi: torch.Tensor, boxes, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
-        """Moments of |psi|^2 per box: device float64 tensor (nboxes, 7), or (batch, nboxes, 7)."""
-        check_state(psi, self.shape, self.precision)
+    def _boxes(self, boxes):
         nb = len(boxes)
         arr = (WfsBox * max(nb, 1))()
         for i, (bottom, count) in enumerate(boxes):
@@ -281,6 +281,43 @@ class Plan:
                     raise ValueError(f"box {i} axis {a}: [{b}, {b + c}) lies outside the grid of {self.n[a]}")
                 arr[i].bottom[a] = b
                 arr[i].count[a] = c
+        return nb, arr
+
+    def step_observe(self, psi: torch.Tensor, V: torch.Tensor | None, nsteps: int, observe_every: int, boxes,
+                     out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """nsteps steps, measuring the boxes before steps 0, observe_every, ... (< nsteps):
+        device float64 tensor (ceil(nsteps / observe_every), [batch,] nboxes, 7).  Same state
+        and observables as alternating observables() and step(observe_every) calls."""
+        check_state(psi, self.shape, self.precision)
+        if V is None and self.method == WFS_METHOD_POWER_SERIES:
+            V = self._V_ref
+        if V is not None:
+            check_potential(V, self.shape)
+        elif not self._potential_set:
+            raise ValueError("no potential: pass V or call set_potential() first")
+        if nsteps < 0 or observe_every < 1:
+            raise ValueError("nsteps must be >= 0 and observe_every >= 1")
+        nb, arr = self._boxes(boxes)
+        nobs = (nsteps + observe_every - 1) // observe_every
+        per = (nb, 7) if self.batch == 1 else (self.batch, nb, 7)
+        if out is None:
+            out = torch.empty((nobs,) + per, dtype=torch.float64, device=psi.device)
+        elif (out.dtype != torch.float64 or out.numel() < nobs * 7 * nb * self.batch or not out.is_contiguous()
+              or out.device != psi.device):
+            raise ValueError("out must be a contiguous float64 device tensor with 7 entries per box and observation")
+        check(lib().wfs_step_observe(self._h, ptr(psi), ptr(V) if V is not None else None, int(nsteps),
+                                     int(observe_every), arr, nb, ptr(out) if nb else None, stream_handle(stream)),
+              "wfs_step_observe")
+        if V is not None:
+            self._potential_set = True
+            if self.method == WFS_METHOD_POWER_SERIES:
+                self._V_ref = V
+        return out
+
+    def observables(self, psi: torch.Tensor, boxes, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Moments of |psi|^2 per box: device float64 tensor (nboxes, 7), or (batch, nboxes, 7)."""
+        check_state(psi, self.shape, self.precision)
+        nb, arr = self._boxes(boxes)
         if out is None:
             shape = (nb, 7) if self.batch == 1 else (self.batch, nb, 7)
             out = torch.empty(shape, dtype=torch.float64, device=psi.device)

@@ -56,12 +56,26 @@ __host__ __device__ inline long lines_source(long o, const LinesOrder& m) {
   return (g / m.b) * m.ny * m.nz + row * m.nz + (t / m.rh) + long(m.n2z) * i;
 }
 
+// boxes measured in one launch_box_moments call (and one LINES_OBSERVE launch)
+constexpr int kBoxesPerPass = 4;
+// Simpson box moments taken inside a LINES_OBSERVE launch: one record of 7 sums per row group
+// and box, partials[group][box][7] (the summing kernel applies dx dy dz / 27)
+struct LinesMoments {
+  int nb = 0;
+  int bot[kBoxesPerPass][3] = {}, num[kBoxesPerPass][3] = {};
+  int n[3] = {};
+  double dx[3] = {};
+  double* partials = nullptr;
+};
+
 struct YzPlaneParams {
   // y twiddles of the inverse / forward y transforms and the z twiddles, state precision:
   // the length-N axis tables, or (slab scheme) each thread's twiddles [u][k1]
   const void* tw_yi = nullptr;
   const void* tw_yf = nullptr;
   const void* tw_z = nullptr;
+  const void* tw_zf = nullptr;  // LINES_OBSERVE: z twiddles of the forward z transform
+  LinesMoments mom;             // LINES_OBSERVE: boxes and partial records
   const void* pv = nullptr;     // potential phase table, state layout (null when spx is set)
   const void* spx = nullptr;    // separable potential: per-axis phase factors [packet][n_d]
   const void* spy = nullptr;
@@ -79,7 +93,8 @@ int launch_yz_plane(int precision, int n, int halves, void* data, const YzPlaneP
 enum {
   LINES_FULL = 0,    // y inverse, z inverse, potential phase, z forward, y forward
   LINES_ENTRY = 1,   // z forward, y forward (position space in)
-  LINES_EXIT = 2     // y inverse, z inverse (position space out)
+  LINES_EXIT = 2,    // y inverse, z inverse (position space out)
+  LINES_OBSERVE = 3  // y inverse, z inverse, box moments of the position-space rows, z forward, y forward
 };
 struct XSlabParams {
   const void* tw_x = nullptr;   // forward twiddles exp(-2 pi i j / nx)
@@ -127,9 +142,10 @@ int launch_horner_stage(int precision, int batch, int nx, int ny, int nz, const 
 int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const double dx[3], const void* psi,
                        const wfs_box* boxes, int nb, double* partials, long max_partials, double* out,
                        long out_stride, hipStream_t s);
-// boxes measured in one launch_box_moments call
-constexpr int kBoxesPerPass = 4;
 long box_moments_partials_needed(const wfs_box& box);
+// out[p * out_stride + 7 b + q] = scale * sum over the nparts records partials[p][r][b][q]
+int launch_sum_partials(const double* partials, long nparts, int nb, int batch, double scale, double* out,
+                        long out_stride, hipStream_t s);
 
 // probability current through a plane: out7[0] = flux (FP64), density optional [cu][cv]
 long plane_current_partials_needed(int cu, int cv);

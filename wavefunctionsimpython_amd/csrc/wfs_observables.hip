@@ -176,6 +176,13 @@ __global__ __launch_bounds__(kObsThreads) void sum_partials_kernel(const double*
   }
 }
 
+int launch_sum_partials(const double* partials, long nparts, int nb, int batch, double scale, double* out,
+                        long out_stride, hipStream_t s) {
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(nb), unsigned(batch)), dim3(kObsThreads), 0, s, partials,
+                     nparts, nb, scale, out, out_stride);
+  return int(hipGetLastError());
+}
+
 long box_moments_partials_needed(const wfs_box& box) {
   const long nrows = long(box.count[0]) * box.count[1];
   return ((nrows + kRowsPerBlock - 1) / kRowsPerBlock) * kObsQ * kBoxesPerPass;

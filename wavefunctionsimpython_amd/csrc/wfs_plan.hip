@@ -27,6 +27,8 @@ struct wfs_plan {
   double* plane_out;      // 7 doubles
   void* pv;               // potential phase table cexp_i(-dt V), state precision, C order
   void* pv_lines;         // the same table in the slab scheme's lines order (LinesOrder)
+  double* obs_lines;      // slab scheme: box moment records of an observing launch (lazy)
+  long obs_lines_len;
   bool pv_nat_ok;         // pv holds the current potential
   bool pv_lines_ok;       // pv_lines holds the current potential
   bool pv_set;
@@ -410,6 +412,52 @@ int run_x_slab(wfs_plan* p, void* data, const void* const* ktab, hipStream_t s, 
   return launch_x_slab(p->n[0], p->n[1], p->variant[WFS_K_X_SLAB], data, xp, s);
 }
 
+// Slab scheme: the exit launch of one call (table copy sel_exit), the box moments of the
+// position-space state, and the entry launch of the next call (copy 0) as one launch, then
+// the records summed into out[packet][box][7]
+int run_slab_observe(wfs_plan* p, void* data, int sel_exit, const wfs_box* boxes, int nb, double* out,
+                     hipStream_t s) {
+  const SlabRows g = slab_rows(p->n[1]);
+  const long groups = long(p->batch) * p->n[0] * g.b;
+  const long need = groups * kBoxesPerPass * 7;
+  if (p->obs_lines_len < need) {
+    if (p->obs_lines) (void)hipFree(p->obs_lines);
+    p->obs_lines = nullptr;
+    p->obs_lines_len = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->obs_lines), size_t(need) * sizeof(double));
+    if (e != hipSuccess) { set_last_error("hipMalloc for observing records failed"); return WFS_ERR_ALLOC; }
+    p->obs_lines_len = need;
+  }
+  YzPlaneParams pp;
+  const size_t ysz = size_t(g.uy) * g.n1 * p->elem, zsz = size_t(g.uz) * g.n1 * p->elem;
+  pp.tw_yi = static_cast<const char*>(p->tw_rows[0]) + size_t(sel_exit % p->ntab) * ysz;
+  pp.tw_z = static_cast<const char*>(p->tw_rows[1]) + size_t(sel_exit % p->ntab) * zsz;
+  pp.tw_yf = p->tw_rows[0];
+  pp.tw_zf = p->tw_rows[1];
+  pp.nx = p->n[0];
+  pp.planes = long(p->batch) * p->n[0];
+  pp.mom.nb = nb;
+  for (int b = 0; b < nb; ++b)
+    for (int a = 0; a < 3; ++a) {
+      pp.mom.bot[b][a] = boxes[b].bottom[a];
+      pp.mom.num[b][a] = boxes[b].count[a];
+    }
+  for (int a = 0; a < 3; ++a) {
+    pp.mom.n[a] = p->n[a];
+    pp.mom.dx[a] = p->dx[a];
+  }
+  pp.mom.partials = p->obs_lines;
+  {
+    ProfileScope scope(p, WFS_K_SLAB_EDGE, s);
+    WFS_TRY(launch_yz_slab(p->n[1], LINES_OBSERVE, 1, data, pp, s));
+  }
+  ProfileScope scope(p, WFS_K_OBSERVABLES, s);
+  return launch_sum_partials(p->obs_lines, long(p->n[0]) * g.b, nb, p->batch, p->dx[0] * p->dx[1] * p->dx[2] / 27.0,
+                             out, 7L * nb, s);
+}
+
+int observe(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, double* out, hipStream_t s);
+
 int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
   if (V) WFS_TRY(set_potential(p, V, s));
   if (!p->pv_set) { set_last_error("wfs_step: no potential (pass V or call wfs_set_potential first)"); return WFS_ERR_INVALID; }
@@ -439,6 +487,13 @@ int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t
   WFS_TRY(run_pass(p, 2, MODE_INV, psi, nullptr, nullptr, 1.0, s, nsteps));
   return 0;
 }
+
+// nsteps steps with the box observables of the state before steps 0, every, 2 every, ...
+// (< nsteps) written to out[o][packet][box][7]: the same launches and results as alternating
+// wfs_observables and wfs_step(every) calls.  The slab scheme runs the exit of one call, the
+// observation and the entry of the next as one launch (run_slab_observe).
+int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every, const wfs_box* boxes, int nb,
+                   double* out, hipStream_t s);
 
 int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
   if (!V) { set_last_error("wfs_step: the power series needs V"); return WFS_ERR_INVALID; }
@@ -501,6 +556,7 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
   if (p->plane_out) (void)hipFree(p->plane_out);
   if (p->pv) (void)hipFree(p->pv);
   if (p->pv_lines) (void)hipFree(p->pv_lines);
+  if (p->obs_lines) (void)hipFree(p->obs_lines);
   for (int k = 0; k < 2; ++k)
     if (p->tw_rows[k]) (void)hipFree(p->tw_rows[k]);
   if (p->pending) {
@@ -627,10 +683,9 @@ extern "C" int wfs_fft3d(wfs_plan* p, void* data, int direction, void* stream) {
   return 0;
 }
 
-extern "C" int wfs_observables(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, double* out,
-                               void* stream) {
-  if (!p || !psi || (nboxes > 0 && (!boxes || !out)) || nboxes < 0) { set_last_error("wfs_observables: bad argument"); return WFS_ERR_INVALID; }
-  hipStream_t s = static_cast<hipStream_t>(stream);
+namespace wfs {
+namespace {
+int observe(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, double* out, hipStream_t s) {
   for (int b = 0; b < nboxes; ++b) {
     const wfs_box& bx = boxes[b];
     for (int a = 0; a < 3; ++a) {
@@ -650,6 +705,62 @@ extern "C" int wfs_observables(wfs_plan* p, const void* psi, const wfs_box* boxe
     if (rc) return rc;
   }
   return 0;
+}
+
+int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every, const wfs_box* boxes, int nb,
+                   double* out, hipStream_t s) {
+  const long per_obs = long(p->batch) * nb * 7;
+  const bool fused = p->method == WFS_METHOD_FOURIER && p->variant[WFS_K_YZ_SLAB] != 0 && nb <= kBoxesPerPass &&
+                     nb > 0;
+  if (!fused) {
+    for (int k = 0; k < nsteps; k += every) {
+      if (nb > 0) WFS_TRY(observe(p, psi, boxes, nb, out + (k / every) * per_obs, s));
+      const int m = std::min(every, nsteps - k);
+      WFS_TRY(p->method == WFS_METHOD_FOURIER ? fourier_steps(p, psi, k == 0 ? V : nullptr, m, s)
+                                              : power_series_steps(p, psi, V, m, s));
+    }
+    return 0;
+  }
+  if (V) WFS_TRY(set_potential(p, V, s));
+  if (!p->pv_set) { set_last_error("wfs_step_observe: no potential (pass V or call wfs_set_potential first)"); return WFS_ERR_INVALID; }
+  WFS_TRY(observe(p, psi, boxes, nb, out, s));
+  const void* pv = nullptr;
+  WFS_TRY(current_table(p, s, &pv));
+  WFS_TRY(run_slab_lines(p, LINES_ENTRY, psi, nullptr, s, 0));
+  WFS_TRY(run_x_slab(p, psi, p->khalf, s, 0));
+  for (int k = 0; k < nsteps; k += every) {
+    const int m = std::min(every, nsteps - k);
+    for (int step = 1; step <= m; ++step) {
+      WFS_TRY(run_slab_lines(p, LINES_FULL, psi, pv, s, step));
+      WFS_TRY(run_x_slab(p, psi, step == m ? p->khalf : p->kfull, s, step));
+    }
+    if (k + m < nsteps) {
+      WFS_TRY(run_slab_observe(p, psi, m, boxes, nb, out + ((k + m) / every) * per_obs, s));
+      WFS_TRY(run_x_slab(p, psi, p->khalf, s, 0));
+    } else {
+      WFS_TRY(run_slab_lines(p, LINES_EXIT, psi, nullptr, s, m));
+    }
+  }
+  return 0;
+}
+}  // namespace
+}  // namespace wfs
+
+extern "C" int wfs_observables(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, double* out,
+                               void* stream) {
+  if (!p || !psi || (nboxes > 0 && (!boxes || !out)) || nboxes < 0) { set_last_error("wfs_observables: bad argument"); return WFS_ERR_INVALID; }
+  return observe(p, psi, boxes, nboxes, out, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int wfs_step_observe(wfs_plan* p, void* psi, const void* V, int nsteps, int observe_every,
+                                const wfs_box* boxes, int nboxes, double* out, void* stream) {
+  if (!p || !psi || nsteps < 0 || observe_every < 1 || nboxes < 0 || (nboxes > 0 && (!boxes || !out))) {
+    set_last_error("wfs_step_observe: bad argument");
+    return WFS_ERR_INVALID;
+  }
+  if (nsteps == 0) return 0;
+  if (p->method == WFS_METHOD_POWER_SERIES && !V) { set_last_error("wfs_step_observe: the power series needs V"); return WFS_ERR_INVALID; }
+  return steps_observed(p, psi, V, nsteps, observe_every, boxes, nboxes, out, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int wfs_plan_set_profiling(wfs_plan* p, int enable) {

@@ -47,6 +47,73 @@ __device__ __forceinline__ int opaque_v(int v) {
   return v;
 }
 
+__device__ __forceinline__ double lines_simpson(int i, int n) {
+  if (i == 0 || i == n - 1) return 1.0;
+  return (i & 1) ? 4.0 : 2.0;
+}
+
+__device__ __forceinline__ double lines_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Box moments of the position-space row a thread holds in the z-line mapping of a
+// yz_lines_kernel round (row c0 = tid mod LY of the group, z = u + N2Z i): per box
+//   sum w |psi|^2 [1, x, y, z, x^2, y^2, z^2],  w = s(ix) s(iy) s(iz)  (Simpson 1-4-2-...-4-1)
+// accumulated in FP64, reduced over the block and written as the group's record
+// partials[group][box][7].  Same sums as box_moments_kernel (wfs_observables.hip).
+template <typename R, int NZ, int LY, int B, int N2Z, int N1, int T>
+__device__ void lines_moments(const cx<R> (&a)[N1], const LinesMoments& m, long plane, int bres, long group, int tid,
+                              int u) {
+  constexpr int NB = kBoxesPerPass;
+  constexpr int kWaves = T / 64;
+  __shared__ double red[kWaves][NB * 7];
+  const int c0 = tid % LY;
+  const int gx = int(plane % m.n[0]);
+  const int gy = B * c0 + bres;
+  const double rx = m.dx[0] * double(gx - m.n[0] / 2);
+  const double ry = m.dx[1] * double(gy - m.n[1] / 2);
+  const int lane = tid & 63, wave = tid >> 6;
+  // one box at a time (few registers live besides the row's points)
+#pragma unroll 1
+  for (int b = 0; b < m.nb; ++b) {
+    const int ix = gx - m.bot[b][0], iy = gy - m.bot[b][1];
+    const bool row_in = ix >= 0 && ix < m.num[b][0] && iy >= 0 && iy < m.num[b][1];
+    const double wxy = row_in ? lines_simpson(ix, m.num[b][0]) * lines_simpson(iy, m.num[b][1]) : 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (row_in) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        // an opaque z index per point: the point's position and weight are formed where they
+        // are used, not all sixteen ahead (FP64 registers)
+        const int gz = opaque_v(u) + N2Z * i;
+        const int iz = gz - m.bot[b][2];
+        const double rho = double(a[i].x) * double(a[i].x) + double(a[i].y) * double(a[i].y);
+        const double rz = m.dx[2] * double(gz - m.n[2] / 2);
+        const double wr = (iz >= 0 && iz < m.num[b][2]) ? lines_simpson(iz, m.num[b][2]) * rho : 0.0;
+        s0 += wr;
+        s1 += wr * rz;
+        s2 += wr * rz * rz;
+      }
+    }
+    const double v[7] = {wxy * s0, wxy * rx * s0, wxy * ry * s0, wxy * s1, wxy * rx * rx * s0,
+                         wxy * ry * ry * s0, wxy * s2};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const double t = lines_wave_sum(v[q]);
+      if (lane == 0) red[wave][b * 7 + q] = t;
+    }
+  }
+  __syncthreads();
+  if (tid < m.nb * 7) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += red[w][tid];
+    m.partials[group * m.nb * 7 + tid] = t;
+  }
+}
+
 template <int NZ, int LY, int H>
 constexpr int yz_lines_threads() {
   return NZ * (LY / (NZ / split_of<NZ>::n2)) / H;
@@ -155,12 +222,12 @@ __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_lines_kernel(cx<R>*
     }
   };
   // one transform of every z line (no potential), round by round
-  auto transform_z = [&](auto sign_c) {
+  auto transform_z = [&](const void* twp, auto sign_c) {
     const int u = thread_u_z();
 #pragma unroll
     for (int r = 0; r < H; ++r) {
       if (r) __syncthreads();
-      load_wz(p.tw_z, u);
+      load_wz(twp, u);
       column_dft_first<NZ, N2Z, decltype(sign_c)::value>(a[r], w);
       exchange_z(a[r], u, sign_c);
     }
@@ -297,19 +364,36 @@ __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_lines_kernel(cx<R>*
   } else if constexpr (MODE == LINES_ENTRY) {
     cols_to_rows();
     __syncthreads();
-    transform_z(std::integral_constant<int, -1>{});
+    transform_z(p.tw_z, std::integral_constant<int, -1>{});
     __syncthreads();
     rows_to_cols();
     __syncthreads();
     transform_y(p.tw_yf, std::integral_constant<int, -1>{});
-  } else {
+  } else if constexpr (MODE == LINES_EXIT) {
     transform_y(p.tw_yi, std::integral_constant<int, +1>{});
     __syncthreads();
     cols_to_rows();
     __syncthreads();
-    transform_z(std::integral_constant<int, +1>{});
+    transform_z(p.tw_z, std::integral_constant<int, +1>{});
     __syncthreads();
     rows_to_cols();
+  } else {
+    // LINES_OBSERVE: the exit launch, the box moments of the position-space rows it would
+    // store, and the entry launch that would read them back, in one pass (same arithmetic)
+    static_assert(H == 1 && !PERSIST, "the observing launch holds one round, one group per block");
+    transform_y(p.tw_yi, std::integral_constant<int, +1>{});
+    __syncthreads();
+    cols_to_rows();
+    __syncthreads();
+    transform_z(p.tw_z, std::integral_constant<int, +1>{});
+    lines_moments<R, NZ, LY, B, N2Z, N1, yz_lines_threads<NZ, LY, H>()>(a[0], p.mom, plane, bres, group, tid,
+                                                                        thread_u_z());
+    __syncthreads();
+    transform_z(p.tw_zf, std::integral_constant<int, -1>{});
+    __syncthreads();
+    rows_to_cols();
+    __syncthreads();
+    transform_y(p.tw_yf, std::integral_constant<int, -1>{});
   }
 
   {
