@@ -250,9 +250,10 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * kernel on whole x planes held in LDS; variant 0 runs the three passes.  The
  * results are identical; the autotuner keeps the faster.
  * WFS_K_YZ_SLAB: variants 1 and 2 (the default where supported: complex64
- * grids with ny = nz = 256 or 192 and nx <= 256; 2 at 256, 1 at 192) run a
- * step as two launches, the y transform split in a length-ny/4 stage fused
- * with the z work
+ * grids with ny = nz = 256 or 192 and nx <= 256, 2 at 256 and 1 at 192; at
+ * ny = nz = 128 variant 2 below 256 x planes) run a step as two launches, the
+ * y transform split in a length-ny/B stage (B = 4; 2 at 128) fused with the z
+ * work
  * (kernel class WFS_K_YZ_SLAB; the entry and exit launches of a call count
  * as WFS_K_SLAB_EDGE) and a length-4 stage fused with the x pass
  * (WFS_K_X_SLAB); variant 2 runs the y-z launch as one persistent block per

@@ -1,7 +1,8 @@
-"""Two-launch split step for complex64 grids with ny = nz = 256 or 192 (the slab scheme).
+"""Two-launch split step for complex64 grids with ny = nz = 256, 192 or 128 (the slab scheme).
 
-The length-n y transform is split in a length-n/4 stage on the rows y = 4 a + b (fused with
-the z inverse / potential / z forward work) and a length-4 stage fused with the x pass.  Its
+The length-n y transform is split in a length-n/B stage on the rows y = B a + b (fused with
+the z inverse / potential / z forward work) and a length-B stage fused with the x pass
+(B = 4; B = 2 at 128).  Its
 arithmetic differs from the four axis passes only in how the y transform is factored, so the
 two paths agree to float rounding; both are pinned to the FP64 oracle at the complex64 bar.
 """
@@ -58,7 +59,8 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("form", FORMS)
-@pytest.mark.parametrize("nx,n", [(8, 256), (16, 256), (32, 256), (8, 192), (16, 192), (96, 192)])
+@pytest.mark.parametrize("nx,n", [(8, 256), (16, 256), (32, 256), (8, 192), (16, 192), (96, 192), (8, 128),
+                                  (128, 128)])
 def test_slab_matches_oracle_table_potential(dev, nx, n, form):
     shape = (nx, n, n)
     psi0, V = _inputs(shape, 100 + nx)
@@ -75,7 +77,7 @@ def test_slab_matches_oracle_table_potential(dev, nx, n, form):
 
 
 @pytest.mark.parametrize("form", FORMS)
-@pytest.mark.parametrize("n", [192, 256])
+@pytest.mark.parametrize("n", [128, 192, 256])
 def test_slab_matches_oracle_separable_potential(dev, n, form):
     shape = (16, n, n)
     psi0, _ = _inputs(shape, 7)
@@ -103,7 +105,7 @@ def test_slab_single_step_and_entry_exit(dev, form):
 
 
 @pytest.mark.parametrize("form", FORMS)
-@pytest.mark.parametrize("n", [192, 256])
+@pytest.mark.parametrize("n", [128, 192, 256])
 def test_slab_batched_bit_identical_to_single_runs(dev, n, form):
     batch = 3
     shape = (8, n, n)
@@ -118,7 +120,7 @@ def test_slab_batched_bit_identical_to_single_runs(dev, n, form):
         assert torch.equal(both[i], one), f"packet {i}"
 
 
-@pytest.mark.parametrize("n", [192, 256])
+@pytest.mark.parametrize("n", [128, 192, 256])
 def test_slab_full_grid_against_passes_and_norm(dev, n):
     """n^3 (256^3 is the benchmark grid): no absorber, so the norm is kept; slab vs four passes"""
     shape = (n, n, n)
@@ -156,13 +158,17 @@ def test_slab_defaults_and_limits(dev):
     assert variant((16, 256, 256), batch=4) == 2
     assert variant((192, 192, 192)) == 1
     assert variant((64, 192, 192), batch=3) == 1
+    assert variant((128, 128, 128)) == 2                 # 128 x planes: the plane pass leaves CUs idle
+    assert variant((64, 128, 128), batch=8) == 0         # 512 planes: the plane pass
+    assert variant((128, 128, 128), C128) == 0
     assert variant((256, 256, 256), C128) == 0
     assert variant((512, 256, 256)) == 0
-    assert variant((256, 128, 128)) == 0
+    assert variant((256, 128, 128)) == 0                 # 256 x 128 planes of 128^2
     assert variant((256, 256, 128)) == 0
-    p = _abi.Plan((64, 128, 128), DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    p = _abi.Plan((64, 64, 64), DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
     with pytest.raises(_abi.WfsError):
         p.set_variant("yz_slab", 1)
+    _abi.Plan((64, 128, 128), DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER).set_variant("yz_slab", 1)
     q = _abi.Plan((16, 256, 256), DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
     with pytest.raises(_abi.WfsError):
         q.set_variant("yz_slab", 3)

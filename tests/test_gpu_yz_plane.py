@@ -38,6 +38,7 @@ def _axes(rng, batch, n):
 
 def _run(shape, precision, variant, psi0, V=None, axes=None, batch=1, steps=5):
     p = _abi.Plan(shape, DX, 0.01, 1.0, precision, _abi.WFS_METHOD_FOURIER, batch=batch)
+    p.set_variant("yz_slab", 0)      # the plane pass itself, not the two-launch form at 128
     p.set_variant("yz_plane", variant)
     psi = psi0.clone()
     if axes is not None:

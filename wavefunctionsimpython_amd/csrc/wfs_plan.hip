@@ -609,12 +609,15 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->variant[WFS_K_YZ_PLANE] =
       (yz_plane_supported(precision, n[1], n[2], 2) && n[1] == 128 && long(batch) * n[0] >= 512) ? 2
       : yz_plane_supported(precision, n[1], n[2], 1) ? 1 : 0;
-  // slab scheme at ny = nz = 256 or 192 (complex64).  At 256 the persistent y-z launch and
-  // 8-z x blocks (measured fastest at 256^3: 90 and 66 us against 115 and 79 for the other
+  // slab scheme at ny = nz = 256, 192 or 128 (complex64).  At 256 the persistent y-z launch
+  // and 8-z x blocks (measured fastest at 256^3: 90 and 66 us against 115 and 79 for the other
   // forms); at 192 one block per row group (two 72 KiB blocks per CU) and 16-z x blocks
-  // (192^3 x 15: 662 and 460 us; 8-z x blocks 514 us)
-  p->variant[WFS_K_YZ_SLAB] = slab_supported(precision, n) ? (n[1] == 256 ? 2 : 1) : 0;
-  if (slab_supported(precision, n)) p->variant[WFS_K_X_SLAB] = n[1] == 256 ? 2 : 1;
+  // (192^3 x 15: 662 and 460 us; 8-z x blocks 514 us); at 128 only while the x planes are too
+  // few to fill the CUs with plane-pass blocks, fewer than 256 (128^3: 29 against 34 us per
+  // step; from 8 packets on the plane pass is as fast)
+  const bool slab = slab_supported(precision, n) && (n[1] != 128 || long(batch) * n[0] < 256);
+  p->variant[WFS_K_YZ_SLAB] = slab ? (n[1] == 192 ? 1 : 2) : 0;
+  if (slab) p->variant[WFS_K_X_SLAB] = n[1] == 192 ? 1 : 2;
   p->pending = new std::vector<std::pair<hipEvent_t, hipEvent_t>>[WFS_K_COUNT];
   p->event_pool = new std::vector<hipEvent_t>();
   int rc = 0;

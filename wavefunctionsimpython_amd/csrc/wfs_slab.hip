@@ -27,7 +27,12 @@ namespace wfs {
 
 namespace {
 
-constexpr int kSlabB = 4;   // y rows y = 4 a + b: the length-4 stage rides on the x launch
+// rows y = B a + b of a row group: the length-B stage rides on the x launch (B = 4 at 256 and
+// 192, LY = 64 and 48 rows per group; B = 2 at 128, LY = 64)
+template <int N> struct SlabShape {
+  static constexpr int B = N == 128 ? 2 : 4;
+  static constexpr int LY = N / B;
+};
 
 // Lane pairs: the partner lanes tid and tid ^ ZB share the (x, z) points of a block; lane
 // q = (tid / ZB) % 2 holds rows b = q and q + 2 (and, after the forward y stage, ky
@@ -54,7 +59,8 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   constexpr int N1 = NX / N2;
   constexpr int A = NY / B;
   constexpr int TZ = NZ / ZB;
-  static_assert(B == 4, "the lane-pair y stage is written for B = 4");
+  static_assert(B == 4 || B == 2, "the lane-pair y stage is written for B = 4 and B = 2");
+  constexpr int RPL = B / 2;               // rows (columns after the y stage) per lane
   static_assert(N1 * N2 == NX && N1 % N2 == 0 && NZ % ZB == 0 && NY % B == 0 && 2 * ZB <= 64, "bad slab split");
   using C = cx<R>;
   using CC = const __attribute__((address_space(4))) C;
@@ -76,28 +82,37 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   // element i of row q + 2 j
   auto off = [&](int i, int j) { return unsigned(zl) + unsigned(q + 2 * j) * NZ + unsigned(t + N2 * i) * XS; };
 
-  // y-stage twiddles W_NY^{b ka} of this lane's rows b = q, q + 2
-  C wq[2];
+  // y-stage twiddles W_NY^{b ka} of this lane's rows b = q (+ 2)
+  C wq[RPL];
   {
     CC* twy = (CC*)(p.tw_y);
     C wb[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) wb[b] = mk<R>(twy[(b * ka) % NY].x, twy[(b * ka) % NY].y);
     wq[0] = q ? wb[1] : wb[0];
-    wq[1] = q ? wb[3] : wb[2];
+    if constexpr (RPL == 2) wq[RPL - 1] = q ? wb[B - 1] : wb[B - 2];
   }
 
-  C a[2][N1];
+  C a[RPL][N1];
 #pragma unroll
-  for (int j = 0; j < 2; ++j)
+  for (int j = 0; j < RPL; ++j)
 #pragma unroll
     for (int i = 0; i < N1; ++i) a[j][i] = blk[off(i, j)];
 
-  // forward y stage: a[0] <- X[q], a[1] <- X[q + 2]
+  // forward y stage, B = 2: X[kb] = Z[0] + (-1)^kb Z[1], lane q keeps X[q]
+  if constexpr (B == 2) {
 #pragma unroll
-  for (int i = 0; i < N1; ++i) {
+    for (int i = 0; i < N1; ++i) {
+      const C z = cmul(a[0][i], wq[0]);
+      const C recv = shfl_xor_c<R>(z, ZB);
+      a[0][i] = q ? csub(recv, z) : cadd(z, recv);
+    }
+  }
+  // forward y stage, B = 4: a[0] <- X[q], a[1] <- X[q + 2]
+#pragma unroll
+  for (int i = 0; i < (B == 4 ? N1 : 0); ++i) {
     const C z0 = cmul(a[0][i], wq[0]);
-    const C z1 = cmul(a[1][i], wq[1]);
+    const C z1 = cmul(a[RPL - 1][i], wq[RPL - 1]);
     const C e0 = cadd(z0, z1), e1 = csub(z0, z1);
     const C keep = q ? e1 : e0;
     const C recv = shfl_xor_c<R>(q ? e0 : e1, ZB);
@@ -105,7 +120,7 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     const C Q = q ? keep : recv;                         // E_1[q]
     const C Qt = q ? mk<R>(Q.y, -Q.x) : Q;               // W_4^q E_1[q], W_4 = -i
     a[0][i] = cadd(P, Qt);
-    a[1][i] = csub(P, Qt);
+    a[RPL - 1][i] = csub(P, Qt);
   }
 
   // x forward, kinetic phase, x inverse on the columns ky = ka + A kb, kb = q + 2 j, one
@@ -117,7 +132,7 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   const C* __restrict__ ky = reinterpret_cast<const C*>(p.ky);
   const C* __restrict__ kz = reinterpret_cast<const C*>(p.kz);
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < RPL; ++j) {
     C w[N1];
     if (j) __syncthreads();
     {
@@ -141,17 +156,26 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     column_dft<NX, N2, +1>(a[j], w, lds, t, lidx);
   }
 
-  // inverse y stage: a[0] <- Y'[q], a[1] <- Y'[q + 2]
+  // inverse y stage, B = 2: out[b] = X[0] + (-1)^b X[1], Y'[q] = W_NY^{-q ka} out[q]
+  if constexpr (B == 2) {
 #pragma unroll
-  for (int i = 0; i < N1; ++i) {
-    const C f0 = cadd(a[0][i], a[1][i]), f1 = csub(a[0][i], a[1][i]);
+    for (int i = 0; i < N1; ++i) {
+      const C v = a[0][i];
+      const C recv = shfl_xor_c<R>(v, ZB);
+      a[0][i] = cmulc(q ? csub(recv, v) : cadd(v, recv), wq[0]);
+    }
+  }
+  // inverse y stage, B = 4: a[0] <- Y'[q], a[1] <- Y'[q + 2]
+#pragma unroll
+  for (int i = 0; i < (B == 4 ? N1 : 0); ++i) {
+    const C f0 = cadd(a[0][i], a[RPL - 1][i]), f1 = csub(a[0][i], a[RPL - 1][i]);
     const C keep = q ? f1 : f0;
     const C recv = shfl_xor_c<R>(q ? f0 : f1, ZB);
     const C P = q ? recv : keep;                         // F_0[q]
     const C Q = q ? keep : recv;                         // F_1[q]
     const C Qt = q ? mk<R>(-Q.y, Q.x) : Q;               // W_4^{-q} F_1[q], W_4^{-1} = +i
     a[0][i] = cmulc(cadd(P, Qt), wq[0]);
-    a[1][i] = cmulc(csub(P, Qt), wq[1]);
+    a[RPL - 1][i] = cmulc(csub(P, Qt), wq[RPL - 1]);
   }
 
   // store addresses recomputed from an opaque copy of the thread index, so the load
@@ -160,7 +184,7 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     const int tt = opaque_v(tid);
     const unsigned base = unsigned(tt % ZB) + unsigned((tt / ZB) % 2) * NZ + unsigned(tt / (2 * ZB)) * XS;
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < RPL; ++j)
 #pragma unroll
       for (int i = 0; i < N1; ++i) blk[base + unsigned(2 * j) * NZ + unsigned(N2 * i) * XS] = a[j][i];
   }
@@ -181,11 +205,11 @@ int cu_count() {
 // persistent: one block per CU walking the row groups, else one block per group
 template <int N, int MODE, bool PERSIST>
 int launch_lines_m(void* data, const YzPlaneParams& p, hipStream_t s) {
-  constexpr int LY = N / kSlabB;
+  constexpr int LY = SlabShape<N>::LY, B = SlabShape<N>::B;
   static_assert(yz_lines_lds_bytes<float, N, LY, 1>() <= 160 * 1024, "rows do not fit in LDS");
-  const long groups = p.planes * kSlabB;
+  const long groups = p.planes * B;
   const long grid = PERSIST ? std::min<long>(groups, cu_count()) : groups;
-  hipLaunchKernelGGL((yz_lines_kernel<float, N, LY, kSlabB, 1, MODE, PERSIST, true>), dim3(unsigned(grid)),
+  hipLaunchKernelGGL((yz_lines_kernel<float, N, LY, B, 1, MODE, PERSIST, true>), dim3(unsigned(grid)),
                      dim3(yz_lines_threads<N, LY, 1>()), 0, s, reinterpret_cast<float2*>(data), p);
   return int(hipGetLastError());
 }
@@ -206,9 +230,10 @@ int launch_lines_n(int mode, int variant, void* data, const YzPlaneParams& p, hi
 
 template <int NX, int N, int ZB>
 int launch_x_slab_z(void* data, const XSlabParams& p, hipStream_t s) {
-  constexpr long per_packet = long(N / kSlabB) * (N / ZB);
+  constexpr int B = SlabShape<N>::B;
+  constexpr long per_packet = long(N / B) * (N / ZB);
   static_assert(per_packet % 16 == 0, "the XCD pairing needs a multiple of 16 blocks");
-  hipLaunchKernelGGL((x_slab_kernel<float, NX, N, N, kSlabB, ZB>), dim3(unsigned(p.packets * per_packet)),
+  hipLaunchKernelGGL((x_slab_kernel<float, NX, N, N, B, ZB>), dim3(unsigned(p.packets * per_packet)),
                      dim3(2 * ZB * split_of<NX>::n2), 0, s, reinterpret_cast<float2*>(data), p);
   return int(hipGetLastError());
 }
@@ -237,33 +262,35 @@ int launch_x_slab_ny(int nx, int variant, void* data, const XSlabParams& p, hipS
 
 }  // namespace
 
-// complex64, ny = nz in {192, 256}, nx a supported FFT length up to 256 (at 512 the x lines
-// of a lane pair do not fit in registers)
+// complex64, ny = nz in {128, 192, 256}, nx a supported FFT length up to 256 (at 512 the x
+// lines of a lane pair do not fit in registers)
 bool slab_supported(int precision, const int n[3]) {
-  return precision == WFS_C64 && n[1] == n[2] && (n[1] == 192 || n[1] == 256) && n[0] <= 256 &&
+  return precision == WFS_C64 && n[1] == n[2] && (n[1] == 128 || n[1] == 192 || n[1] == 256) && n[0] <= 256 &&
          fft_length_supported(n[0]);
 }
 
 template <int N>
 static SlabRows slab_rows_n() {
   SlabRows g;
-  g.b = kSlabB;
+  g.b = SlabShape<N>::B;
   g.n1 = N / split_of<N>::n2;
-  g.uy = (N / kSlabB) / g.n1;
+  g.uy = SlabShape<N>::LY / g.n1;
   g.uz = split_of<N>::n2;
   return g;
 }
 
-SlabRows slab_rows(int n) { return n == 192 ? slab_rows_n<192>() : slab_rows_n<256>(); }
+SlabRows slab_rows(int n) {
+  return n == 128 ? slab_rows_n<128>() : n == 192 ? slab_rows_n<192>() : slab_rows_n<256>();
+}
 
 template <int N>
 static LinesOrder slab_lines_order_n() {
-  constexpr int LY = N / kSlabB;
+  constexpr int LY = SlabShape<N>::LY;
   LinesOrder m;
   m.ny = N;
   m.nz = N;
   m.ly = LY;
-  m.b = kSlabB;
+  m.b = SlabShape<N>::B;
   m.rh = LY;
   m.n2z = split_of<N>::n2;
   m.n1 = N / split_of<N>::n2;
@@ -271,27 +298,31 @@ static LinesOrder slab_lines_order_n() {
   return m;
 }
 
-LinesOrder slab_lines_order(int n) { return n == 192 ? slab_lines_order_n<192>() : slab_lines_order_n<256>(); }
+LinesOrder slab_lines_order(int n) {
+  return n == 128 ? slab_lines_order_n<128>() : n == 192 ? slab_lines_order_n<192>() : slab_lines_order_n<256>();
+}
 
 int launch_yz_slab(int n, int mode, int variant, void* data, const YzPlaneParams& p, hipStream_t s) {
-  if (p.planes <= 0 || p.planes * kSlabB > 0x7fffffffL) {
+  if (p.planes <= 0 || p.planes * 4 > 0x7fffffffL) {
     set_last_error("slab y-z pass: bad plane count %ld", p.planes);
     return WFS_ERR_INVALID;
   }
   if (n == 256) return launch_lines_n<256>(mode, variant, data, p, s);
   if (n == 192) return launch_lines_n<192>(mode, variant, data, p, s);
+  if (n == 128) return launch_lines_n<128>(mode, variant, data, p, s);
   set_last_error("slab y-z pass: unsupported plane %d", n);
   return WFS_ERR_UNSUPPORTED;
 }
 
 int launch_x_slab(int nx, int n, int variant, void* data, const XSlabParams& p, hipStream_t s) {
-  const long blocks = p.packets * long(n / kSlabB) * (n / 8);
+  const long blocks = p.packets * long(n / 2) * (n / 8);
   if (p.packets <= 0 || blocks > 0x7fffffffL) {
     set_last_error("slab x pass: bad packet count %ld", p.packets);
     return WFS_ERR_INVALID;
   }
   if (n == 256) return launch_x_slab_ny<256>(nx, variant, data, p, s);
   if (n == 192) return launch_x_slab_ny<192>(nx, variant, data, p, s);
+  if (n == 128) return launch_x_slab_ny<128>(nx, variant, data, p, s);
   set_last_error("slab x pass: unsupported plane %d", n);
   return WFS_ERR_UNSUPPORTED;
 }

@@ -285,135 +285,135 @@ __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_lines_kernel(cx<R>*
   }
 
   for (;;) {
-  // persistent form: the next group's points, loaded now and used after this group's store
-  C nxt[PERSIST ? N1 : 1];
-  const long next = group + gridDim.x;
-  const bool more = PERSIST && next < groups;
-  if constexpr (PERSIST) {
-    if (more) {
-      const long np = next / B;
-      const C* __restrict__ nb = data + np * long(NY) * NZ + long(next % B) * NZ;
-      const int c0 = tid % CH, u = tid / CH;
+    // persistent form: the next group's points, loaded now and used after this group's store
+    C nxt[PERSIST ? N1 : 1];
+    const long next = group + gridDim.x;
+    const bool more = PERSIST && next < groups;
+    if constexpr (PERSIST) {
+      if (more) {
+        const long np = next / B;
+        const C* __restrict__ nb = data + np * long(NY) * NZ + long(next % B) * NZ;
+        const int c0 = tid % CH, u = tid / CH;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) nxt[i] = nb[unsigned(c0) + unsigned(u + N2Y * i) * RS];
+        for (int i = 0; i < N1; ++i) nxt[i] = nb[unsigned(c0) + unsigned(u + N2Y * i) * RS];
+      }
     }
-  }
 
-  if constexpr (MODE == LINES_FULL) {
-    // y inverse on columns
-    transform_y(p.tw_yi, std::integral_constant<int, +1>{});
-    __syncthreads();
-    cols_to_rows();
-    __syncthreads();
+    if constexpr (MODE == LINES_FULL) {
+      // y inverse on columns
+      transform_y(p.tw_yi, std::integral_constant<int, +1>{});
+      __syncthreads();
+      cols_to_rows();
+      __syncthreads();
 
-    // z inverse, potential phase and z forward on rows c0 + r LY/H (z = u + N2Z i).  Each
-    // row round is finished (exchange, potential, first forward step) before the next one's
-    // exchange, so that only one round's potential factors are held.
-    {
-      const int u = thread_u_z();
+      // z inverse, potential phase and z forward on rows c0 + r LY/H (z = u + N2Z i).  Each
+      // row round is finished (exchange, potential, first forward step) before the next one's
+      // exchange, so that only one round's potential factors are held.
+      {
+        const int u = thread_u_z();
 #pragma unroll
-      for (int r = 0; r < H; ++r) {
-        if (r) __syncthreads();
-        load_wz(p.tw_z, u);
-        column_dft_first<NZ, N2Z, +1>(a[r], w);
-        exchange_z(a[r], u, std::integral_constant<int, +1>{});
-        const int c0 = opaque_v(tid) % RH;
-        if (p.spx) {
-          const long pk = plane / p.nx;
-          const C* __restrict__ px = reinterpret_cast<const C*>(p.spx);
-          const C* __restrict__ py = reinterpret_cast<const C*>(p.spy) + pk * NY;
-          const C* __restrict__ pzb = reinterpret_cast<const C*>(p.spz) + pk * NZ;
-          const C pxy = cmul(px[plane], py[bres + B * (c0 + r * RH)]);
-          const int ur = opaque_v(u);     // separate loads per round (not held across rounds)
+        for (int r = 0; r < H; ++r) {
+          if (r) __syncthreads();
+          load_wz(p.tw_z, u);
+          column_dft_first<NZ, N2Z, +1>(a[r], w);
+          exchange_z(a[r], u, std::integral_constant<int, +1>{});
+          const int c0 = opaque_v(tid) % RH;
+          if (p.spx) {
+            const long pk = plane / p.nx;
+            const C* __restrict__ px = reinterpret_cast<const C*>(p.spx);
+            const C* __restrict__ py = reinterpret_cast<const C*>(p.spy) + pk * NY;
+            const C* __restrict__ pzb = reinterpret_cast<const C*>(p.spz) + pk * NZ;
+            const C pxy = cmul(px[plane], py[bres + B * (c0 + r * RH)]);
+            const int ur = opaque_v(u);     // separate loads per round (not held across rounds)
 #pragma unroll
-          for (int i = 0; i < N1; ++i) {
-            a[r][i] = cmul(a[r][i], cmul(pzb[ur + N2Z * i], pxy));
-            if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");   // at most 4 factors in flight
+            for (int i = 0; i < N1; ++i) {
+              a[r][i] = cmul(a[r][i], cmul(pzb[ur + N2Z * i], pxy));
+              if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");   // at most 4 factors in flight
+            }
+          } else if constexpr (PV_LINES) {
+            constexpr unsigned T = unsigned(yz_lines_threads<NZ, LY, H>());
+            const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + group * (long(LY) * NZ);
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+              a[r][i] = cmul(a[r][i], pvb[unsigned(r * N1 + i) * T + unsigned(tid)]);
+              if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");
+            }
+          } else {
+            const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + plane * long(NY) * NZ + long(bres) * NZ;
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+              a[r][i] = cmul(a[r][i], pvb[unsigned(c0 + r * RH) * RS + unsigned(u + N2Z * i)]);
+              if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");
+            }
           }
-        } else if constexpr (PV_LINES) {
-          constexpr unsigned T = unsigned(yz_lines_threads<NZ, LY, H>());
-          const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + group * (long(LY) * NZ);
-#pragma unroll
-          for (int i = 0; i < N1; ++i) {
-            a[r][i] = cmul(a[r][i], pvb[unsigned(r * N1 + i) * T + unsigned(tid)]);
-            if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");
-          }
-        } else {
-          const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + plane * long(NY) * NZ + long(bres) * NZ;
-#pragma unroll
-          for (int i = 0; i < N1; ++i) {
-            a[r][i] = cmul(a[r][i], pvb[unsigned(c0 + r * RH) * RS + unsigned(u + N2Z * i)]);
-            if (H > 1 && i % 4 == 3) asm volatile("" ::: "memory");
-          }
+          load_wz(p.tw_z, u);
+          column_dft_first<NZ, N2Z, -1>(a[r], w);
         }
-        load_wz(p.tw_z, u);
-        column_dft_first<NZ, N2Z, -1>(a[r], w);
-      }
 #pragma unroll
-      for (int r = 0; r < H; ++r) {
-        __syncthreads();
-        exchange_z(a[r], u, std::integral_constant<int, -1>{});
+        for (int r = 0; r < H; ++r) {
+          __syncthreads();
+          exchange_z(a[r], u, std::integral_constant<int, -1>{});
+        }
       }
+      __syncthreads();
+      rows_to_cols();
+      __syncthreads();
+
+      // y forward on columns
+      transform_y(p.tw_yf, std::integral_constant<int, -1>{});
+    } else if constexpr (MODE == LINES_ENTRY) {
+      cols_to_rows();
+      __syncthreads();
+      transform_z(p.tw_z, std::integral_constant<int, -1>{});
+      __syncthreads();
+      rows_to_cols();
+      __syncthreads();
+      transform_y(p.tw_yf, std::integral_constant<int, -1>{});
+    } else if constexpr (MODE == LINES_EXIT) {
+      transform_y(p.tw_yi, std::integral_constant<int, +1>{});
+      __syncthreads();
+      cols_to_rows();
+      __syncthreads();
+      transform_z(p.tw_z, std::integral_constant<int, +1>{});
+      __syncthreads();
+      rows_to_cols();
+    } else {
+      // LINES_OBSERVE: the exit launch, the box moments of the position-space rows it would
+      // store, and the entry launch that would read them back, in one pass (same arithmetic)
+      static_assert(H == 1 && !PERSIST, "the observing launch holds one round, one group per block");
+      transform_y(p.tw_yi, std::integral_constant<int, +1>{});
+      __syncthreads();
+      cols_to_rows();
+      __syncthreads();
+      transform_z(p.tw_z, std::integral_constant<int, +1>{});
+      lines_moments<R, NZ, LY, B, N2Z, N1, yz_lines_threads<NZ, LY, H>()>(a[0], p.mom, plane, bres, group, tid,
+                                                                          thread_u_z());
+      __syncthreads();
+      transform_z(p.tw_zf, std::integral_constant<int, -1>{});
+      __syncthreads();
+      rows_to_cols();
+      __syncthreads();
+      transform_y(p.tw_yf, std::integral_constant<int, -1>{});
     }
-    __syncthreads();
-    rows_to_cols();
-    __syncthreads();
 
-    // y forward on columns
-    transform_y(p.tw_yf, std::integral_constant<int, -1>{});
-  } else if constexpr (MODE == LINES_ENTRY) {
-    cols_to_rows();
-    __syncthreads();
-    transform_z(p.tw_z, std::integral_constant<int, -1>{});
-    __syncthreads();
-    rows_to_cols();
-    __syncthreads();
-    transform_y(p.tw_yf, std::integral_constant<int, -1>{});
-  } else if constexpr (MODE == LINES_EXIT) {
-    transform_y(p.tw_yi, std::integral_constant<int, +1>{});
-    __syncthreads();
-    cols_to_rows();
-    __syncthreads();
-    transform_z(p.tw_z, std::integral_constant<int, +1>{});
-    __syncthreads();
-    rows_to_cols();
-  } else {
-    // LINES_OBSERVE: the exit launch, the box moments of the position-space rows it would
-    // store, and the entry launch that would read them back, in one pass (same arithmetic)
-    static_assert(H == 1 && !PERSIST, "the observing launch holds one round, one group per block");
-    transform_y(p.tw_yi, std::integral_constant<int, +1>{});
-    __syncthreads();
-    cols_to_rows();
-    __syncthreads();
-    transform_z(p.tw_z, std::integral_constant<int, +1>{});
-    lines_moments<R, NZ, LY, B, N2Z, N1, yz_lines_threads<NZ, LY, H>()>(a[0], p.mom, plane, bres, group, tid,
-                                                                        thread_u_z());
-    __syncthreads();
-    transform_z(p.tw_zf, std::integral_constant<int, -1>{});
-    __syncthreads();
-    rows_to_cols();
-    __syncthreads();
-    transform_y(p.tw_yf, std::integral_constant<int, -1>{});
-  }
-
-  {
-    const int t = opaque_v(tid);
-    const int c0 = t % CH, u = t / CH;
+    {
+      const int t = opaque_v(tid);
+      const int c0 = t % CH, u = t / CH;
 #pragma unroll
-    for (int h = 0; h < H; ++h)
+      for (int h = 0; h < H; ++h)
 #pragma unroll
-      for (int i = 0; i < N1; ++i) blk[unsigned(c0 + h * CH) + unsigned(u + N2Y * i) * RS] = a[h][i];
-  }
-  if (!more) break;
-  if constexpr (PERSIST) {
+        for (int i = 0; i < N1; ++i) blk[unsigned(c0 + h * CH) + unsigned(u + N2Y * i) * RS] = a[h][i];
+    }
+    if (!more) break;
+    if constexpr (PERSIST) {
 #pragma unroll
-    for (int i = 0; i < N1; ++i) a[0][i] = nxt[i];
-    group = next;
-    plane = group / B;
-    bres = int(group % B);
-    blk = data + plane * long(NY) * NZ + long(bres) * NZ;
-    __syncthreads();   // the LDS of this group's last exchange is written again next
-  }
+      for (int i = 0; i < N1; ++i) a[0][i] = nxt[i];
+      group = next;
+      plane = group / B;
+      bres = int(group % B);
+      blk = data + plane * long(NY) * NZ + long(bres) * NZ;
+      __syncthreads();   // the LDS of this group's last exchange is written again next
+    }
   }
 }
 
