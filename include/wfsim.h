@@ -255,7 +255,7 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * y transform split in a length-ny/B stage (B = 4; 2 at 128) fused with the z
  * work
  * (kernel class WFS_K_YZ_SLAB; the entry and exit launches of a call count
- * as WFS_K_SLAB_EDGE) and a length-4 stage fused with the x pass
+ * as WFS_K_SLAB_EDGE) and a length-B stage fused with the x pass
  * (WFS_K_X_SLAB); variant 2 runs the y-z launch as one persistent block per
  * CU that loads the next row group during the current one's arithmetic;
  * variant 0 runs the four axis passes.  The results agree to float rounding
