@@ -253,13 +253,12 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * grids with ny = nz = 256 or 192 and nx <= 256, 2 at 256 and 1 at 192; at
  * ny = nz = 128 variant 2 below 256 x planes) run a step as two launches, the
  * y transform split in a length-ny/B stage (B = 4; 2 at 128) fused with the z
- * work
- * (kernel class WFS_K_YZ_SLAB; the entry and exit launches of a call count
- * as WFS_K_SLAB_EDGE) and a length-B stage fused with the x pass
+ * work (kernel class WFS_K_YZ_SLAB; the entry and exit launches of a call
+ * count as WFS_K_SLAB_EDGE) and a length-B stage fused with the x pass
  * (WFS_K_X_SLAB); variant 2 runs the y-z launch as one persistent block per
  * CU that loads the next row group during the current one's arithmetic;
  * variant 0 runs the four axis passes.  The results agree to float rounding
- * (the length-256 y transform is factored differently); the slab variants
+ * (the length-ny y transform is factored differently); the slab variants
  * among themselves are bit-identical.  WFS_K_X_SLAB: variant 1 (default at
  * 192) puts 16 z points in an x block, variant 2 (default at 256) 8, pairing
  * the two blocks of a 128-byte line on one XCD.  The autotuner keeps the
