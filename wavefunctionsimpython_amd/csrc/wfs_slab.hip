@@ -42,7 +42,9 @@ template <int N> struct SlabShape {
 //            X[k' + 2 k''] = E_0[k'] + (-1)^k'' W_4^{k'} E_1[k']
 //   inverse: F_q[beta] = X[q] + (-1)^beta X[q + 2],
 //            out[b] = F_0[b mod 2] + W_4^{-b} F_1[b mod 2],  Y'[b] = W_NY^{-b ka} out[b]
-// Lane q keeps the k' = q (beta = q) values and receives the partner's.
+// Lane q keeps the k' = q (beta = q) values and receives the partner's.  For B = 2 (128^2
+// planes) each lane holds the one row b = q and the length-2 stage is one swap:
+//   X[kb] = Z[0] + (-1)^kb Z[1],  out[b] = X[0] + (-1)^b X[1].
 template <typename R>
 __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
   return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
@@ -99,28 +101,29 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
 #pragma unroll
     for (int i = 0; i < N1; ++i) a[j][i] = blk[off(i, j)];
 
-  // forward y stage, B = 2: X[kb] = Z[0] + (-1)^kb Z[1], lane q keeps X[q]
   if constexpr (B == 2) {
+    // forward y stage, B = 2: X[kb] = Z[0] + (-1)^kb Z[1], lane q keeps X[q]
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const C z = cmul(a[0][i], wq[0]);
       const C recv = shfl_xor_c<R>(z, ZB);
       a[0][i] = q ? csub(recv, z) : cadd(z, recv);
     }
-  }
-  // forward y stage, B = 4: a[0] <- X[q], a[1] <- X[q + 2]
+  } else {
+    // forward y stage, B = 4: a[0] <- X[q], a[1] <- X[q + 2]
 #pragma unroll
-  for (int i = 0; i < (B == 4 ? N1 : 0); ++i) {
-    const C z0 = cmul(a[0][i], wq[0]);
-    const C z1 = cmul(a[RPL - 1][i], wq[RPL - 1]);
-    const C e0 = cadd(z0, z1), e1 = csub(z0, z1);
-    const C keep = q ? e1 : e0;
-    const C recv = shfl_xor_c<R>(q ? e0 : e1, ZB);
-    const C P = q ? recv : keep;                         // E_0[q]
-    const C Q = q ? keep : recv;                         // E_1[q]
-    const C Qt = q ? mk<R>(Q.y, -Q.x) : Q;               // W_4^q E_1[q], W_4 = -i
-    a[0][i] = cadd(P, Qt);
-    a[RPL - 1][i] = csub(P, Qt);
+    for (int i = 0; i < N1; ++i) {
+      const C z0 = cmul(a[0][i], wq[0]);
+      const C z1 = cmul(a[RPL - 1][i], wq[RPL - 1]);
+      const C e0 = cadd(z0, z1), e1 = csub(z0, z1);
+      const C keep = q ? e1 : e0;
+      const C recv = shfl_xor_c<R>(q ? e0 : e1, ZB);
+      const C P = q ? recv : keep;                         // E_0[q]
+      const C Q = q ? keep : recv;                         // E_1[q]
+      const C Qt = q ? mk<R>(Q.y, -Q.x) : Q;               // W_4^q E_1[q], W_4 = -i
+      a[0][i] = cadd(P, Qt);
+      a[RPL - 1][i] = csub(P, Qt);
+    }
   }
 
   // x forward, kinetic phase, x inverse on the columns ky = ka + A kb, kb = q + 2 j, one
@@ -156,26 +159,27 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     column_dft<NX, N2, +1>(a[j], w, lds, t, lidx);
   }
 
-  // inverse y stage, B = 2: out[b] = X[0] + (-1)^b X[1], Y'[q] = W_NY^{-q ka} out[q]
   if constexpr (B == 2) {
+    // inverse y stage, B = 2: out[b] = X[0] + (-1)^b X[1], Y'[q] = W_NY^{-q ka} out[q]
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const C v = a[0][i];
       const C recv = shfl_xor_c<R>(v, ZB);
       a[0][i] = cmulc(q ? csub(recv, v) : cadd(v, recv), wq[0]);
     }
-  }
-  // inverse y stage, B = 4: a[0] <- Y'[q], a[1] <- Y'[q + 2]
+  } else {
+    // inverse y stage, B = 4: a[0] <- Y'[q], a[1] <- Y'[q + 2]
 #pragma unroll
-  for (int i = 0; i < (B == 4 ? N1 : 0); ++i) {
-    const C f0 = cadd(a[0][i], a[RPL - 1][i]), f1 = csub(a[0][i], a[RPL - 1][i]);
-    const C keep = q ? f1 : f0;
-    const C recv = shfl_xor_c<R>(q ? f0 : f1, ZB);
-    const C P = q ? recv : keep;                         // F_0[q]
-    const C Q = q ? keep : recv;                         // F_1[q]
-    const C Qt = q ? mk<R>(-Q.y, Q.x) : Q;               // W_4^{-q} F_1[q], W_4^{-1} = +i
-    a[0][i] = cmulc(cadd(P, Qt), wq[0]);
-    a[RPL - 1][i] = cmulc(csub(P, Qt), wq[RPL - 1]);
+    for (int i = 0; i < N1; ++i) {
+      const C f0 = cadd(a[0][i], a[RPL - 1][i]), f1 = csub(a[0][i], a[RPL - 1][i]);
+      const C keep = q ? f1 : f0;
+      const C recv = shfl_xor_c<R>(q ? f0 : f1, ZB);
+      const C P = q ? recv : keep;                         // F_0[q]
+      const C Q = q ? keep : recv;                         // F_1[q]
+      const C Qt = q ? mk<R>(-Q.y, Q.x) : Q;               // W_4^{-q} F_1[q], W_4^{-1} = +i
+      a[0][i] = cmulc(cadd(P, Qt), wq[0]);
+      a[RPL - 1][i] = cmulc(csub(P, Qt), wq[RPL - 1]);
+    }
   }
 
   // store addresses recomputed from an opaque copy of the thread index, so the load
