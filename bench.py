@@ -20,6 +20,7 @@ host cores).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -264,6 +265,11 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del pairs
+    # The training leaves millions of Python objects behind; a full collection of the cyclic
+    # garbage collector inside a timed run then stalls it for ~37 ms (measured in the kernel
+    # trace as one idle gap).  Collect once and move the survivors out of the collector's view.
+    gc.collect()
+    gc.freeze()
     test = pr.random_lattice_problem(np.random.default_rng(seed_test), n=n, dx=dx, k0=0.5, r0=3.0)
     V = test.potential()
     psi0 = test.packet()
@@ -307,8 +313,6 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
         torch.cuda.synchronize()
         gtimes.append(time.perf_counter() - t0)
     t_plain_graph = min(gtimes)
-    # destroy the graph before the parareal runs: while it was alive, the eager launches of the
-    # parareal correction sweep ran about 5x slower (42 against 8 ms per run, measured)
     graph.close()
     base = pr.SplitStepCoarse(test.shape, (dx,) * 3, S * test.dt, 1.0, V, 1, axes=test.axes())
     fno = pr.FNOCoarse(net, V, pscale, vscale, base=base)
