@@ -125,3 +125,17 @@ def test_edge_cases(dev):
     assert torch.equal(a, b)
     with pytest.raises(ValueError):
         plans[0].step_observe(a, None, 4, 0, _boxes(shape))
+
+
+def test_slab_fused_observations_boxes_sharing_z_ranges(dev):
+    """boxes with one z range (the observation box and its x halves, as the benchmark measures),
+    and a run of them broken by a box with another z range"""
+    shape = (16, 256, 256)
+    b, c = 3, 201
+    boxes = [((b, b, b), (13, c, c)), ((b, b, b), (7, c, c)), ((b + 6, b, b), (7, c, c)),
+             ((0, 5, 7), (9, 9, 9)), ((2, 5, b), (5, 9, c))]
+    print(f"case: slab, {shape}, boxes sharing z ranges", flush=True)
+    plans, psi, V = _pair(shape, C64, _abi.WFS_METHOD_FOURIER, 1, None, 8)
+    _check(plans, psi, V, 9, 3, boxes[:3])
+    plans, psi, V = _pair(shape, C64, _abi.WFS_METHOD_FOURIER, 1, None, 8)
+    _check(plans, psi, V, 9, 3, boxes[1:])
