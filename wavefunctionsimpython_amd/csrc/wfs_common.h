@@ -48,6 +48,18 @@ __host__ __device__ __forceinline__ T cmulc(T a, T b) {
 template <typename T, typename S>
 __host__ __device__ __forceinline__ T cscale(T a, S s) { a.x *= s; a.y *= s; return a; }
 
+// element T at a 32-bit byte offset from a base address: with a block-uniform base the
+// address is the scalar base plus one 32-bit vector offset (no 64-bit vector arithmetic per
+// access); the caller keeps the offset below 2^32
+template <typename T>
+__device__ __forceinline__ T& byte_at(T* base, unsigned off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ const T& byte_at(const T* base, unsigned off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
 // a * (hi + lo) and a * conj(hi + lo) for a factor held as an unevaluated sum of two
 // floats (hi = float(w), lo = float(w - hi)).  A float-rounded factor applied every
 // step to the same mode errs the same way every step, so its error grows linearly

@@ -45,6 +45,10 @@ template <int N> struct SlabShape {
 // Lane q keeps the k' = q (beta = q) values and receives the partner's.  For B = 2 (128^2
 // planes) each lane holds the one row b = q and the length-2 stage is one swap:
 //   X[kb] = Z[0] + (-1)^kb Z[1],  out[b] = X[0] + (-1)^b X[1].
+// Lane-dependent choices are signs, not selects: with s = (-1)^q, lane q of a B = 4 pair
+// holds s times its second column through the x transforms (all linear) and uses s W_NY^{b ka}
+// as that row's twiddle.  Negation commutes exactly with every rounded operation, so the
+// results are bit for bit those of the selecting form, with one select per value left.
 template <typename R>
 __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
   return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
@@ -64,6 +68,7 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   static_assert(B == 4 || B == 2, "the lane-pair y stage is written for B = 4 and B = 2");
   constexpr int RPL = B / 2;               // rows (columns after the y stage) per lane
   static_assert(N1 * N2 == NX && N1 % N2 == 0 && NZ % ZB == 0 && NY % B == 0 && 2 * ZB <= 64, "bad slab split");
+  static_assert((N2 & (N2 - 1)) == 0, "the twiddle index masks t by N2 - 1");
   using C = cx<R>;
   using CC = const __attribute__((address_space(4))) C;
   constexpr int LINES = 2 * ZB;            // x lines per round: (q, z)
@@ -81,6 +86,7 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   const long pk = bid / (long(TZ) * A);
   C* __restrict__ blk = data + pk * (long(NX) * NY * NZ) + long(B * ka) * NZ + long(zt) * ZB;
   constexpr unsigned XS = unsigned(NY) * NZ;   // x stride; offsets stay below 2^32 (NX <= 512)
+  constexpr unsigned EB = unsigned(sizeof(C));  // byte offsets too: NX NY NZ 8 <= 2^27 at 256^3
   // element i of row q + 2 j
   auto off = [&](int i, int j) { return unsigned(zl) + unsigned(q + 2 * j) * NZ + unsigned(t + N2 * i) * XS; };
 
@@ -92,37 +98,38 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
 #pragma unroll
     for (int b = 0; b < B; ++b) wb[b] = mk<R>(twy[(b * ka) % NY].x, twy[(b * ka) % NY].y);
     wq[0] = q ? wb[1] : wb[0];
-    if constexpr (RPL == 2) wq[RPL - 1] = q ? wb[B - 1] : wb[B - 2];
+    if constexpr (RPL == 2) wq[RPL - 1] = q ? mk<R>(-wb[B - 1].x, -wb[B - 1].y) : wb[B - 2];   // s W_NY^{b ka}
   }
 
   C a[RPL][N1];
 #pragma unroll
   for (int j = 0; j < RPL; ++j)
 #pragma unroll
-    for (int i = 0; i < N1; ++i) a[j][i] = blk[off(i, j)];
+    for (int i = 0; i < N1; ++i) a[j][i] = byte_at(blk, off(i, j) * EB);
 
+  const R sgn = q ? R(-1) : R(1);          // s = (-1)^q
   if constexpr (B == 2) {
-    // forward y stage, B = 2: X[kb] = Z[0] + (-1)^kb Z[1], lane q keeps X[q]
+    // forward y stage, B = 2: X[kb] = Z[0] + (-1)^kb Z[1], lane q keeps X[q] = recv + s z
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const C z = cmul(a[0][i], wq[0]);
       const C recv = shfl_xor_c<R>(z, ZB);
-      a[0][i] = q ? csub(recv, z) : cadd(z, recv);
+      a[0][i] = mk<R>(fma_r(sgn, z.x, recv.x), fma_r(sgn, z.y, recv.y));
     }
   } else {
-    // forward y stage, B = 4: a[0] <- X[q], a[1] <- X[q + 2]
+    // forward y stage, B = 4: a[0] <- X[q], a[1] <- s X[q + 2].  With z1 = s Z[q + 2] the
+    // kept value z0 + z1 is E_q[q] and the sent one E_q[1 - q]; lane 0 then holds E_0[0]
+    // and receives E_1[0], lane 1 holds E_1[1] and receives E_0[1]:
+    //   X[q] = kk + recv,  X[q + 2] = s (kk - recv),  kk = keep or W_4 keep (lane 1)
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const C z0 = cmul(a[0][i], wq[0]);
       const C z1 = cmul(a[RPL - 1][i], wq[RPL - 1]);
-      const C e0 = cadd(z0, z1), e1 = csub(z0, z1);
-      const C keep = q ? e1 : e0;
-      const C recv = shfl_xor_c<R>(q ? e0 : e1, ZB);
-      const C P = q ? recv : keep;                         // E_0[q]
-      const C Q = q ? keep : recv;                         // E_1[q]
-      const C Qt = q ? mk<R>(Q.y, -Q.x) : Q;               // W_4^q E_1[q], W_4 = -i
-      a[0][i] = cadd(P, Qt);
-      a[RPL - 1][i] = csub(P, Qt);
+      const C keep = cadd(z0, z1);
+      const C recv = shfl_xor_c<R>(csub(z0, z1), ZB);
+      const C kk = q ? mk<R>(keep.y, -keep.x) : keep;     // W_4^q keep, W_4 = -i
+      a[0][i] = cadd(kk, recv);
+      a[RPL - 1][i] = csub(kk, recv);
     }
   }
 
@@ -139,22 +146,26 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     C w[N1];
     if (j) __syncthreads();
     {
-      const int tt = opaque_v(t);
+      // unsigned and masked (t < N2, a power of two): the index mod NX is one mask and the
+      // product a 24-bit multiply
+      const unsigned tt = unsigned(opaque_v(t)) & unsigned(N2 - 1);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(tt * k1) % NX];
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = byte_at(tw, (tt * unsigned(k1)) % unsigned(NX) * EB);
     }
     column_dft<NX, N2, -1>(a[j], w, lds, t, lidx);
     {
-      const int tt = opaque_v(t);
+      const unsigned tt = unsigned(opaque_v(t));
       const C kyz = cmul(ky[ka + A * (q + 2 * j)], kz[zt * ZB + zl]);
 #pragma unroll
-      for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(kx[tt + N2 * i], kyz));
+      for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(byte_at(kx, (tt + unsigned(N2 * i)) * EB), kyz));
     }
     __syncthreads();
     {
-      const int tt = opaque_v(t);
+      // unsigned and masked (t < N2, a power of two): the index mod NX is one mask and the
+      // product a 24-bit multiply
+      const unsigned tt = unsigned(opaque_v(t)) & unsigned(N2 - 1);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tw[(tt * k1) % NX];
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = byte_at(tw, (tt * unsigned(k1)) % unsigned(NX) * EB);
     }
     column_dft<NX, N2, +1>(a[j], w, lds, t, lidx);
   }
@@ -165,20 +176,20 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
     for (int i = 0; i < N1; ++i) {
       const C v = a[0][i];
       const C recv = shfl_xor_c<R>(v, ZB);
-      a[0][i] = cmulc(q ? csub(recv, v) : cadd(v, recv), wq[0]);
+      a[0][i] = cmulc(mk<R>(fma_r(sgn, v.x, recv.x), fma_r(sgn, v.y, recv.y)), wq[0]);
     }
   } else {
-    // inverse y stage, B = 4: a[0] <- Y'[q], a[1] <- Y'[q + 2]
+    // inverse y stage, B = 4, from a[0] = X[q] and a[1] = s X[q + 2]: the kept a[0] + a[1] is
+    // F_q[q] and the sent a[0] - a[1] is F_q[1 - q]; with kk = keep or W_4^{-1} keep (lane 1)
+    //   out[q] = kk + recv,  out[q + 2] = s (kk - recv),  a[0] <- Y'[q], a[1] <- Y'[q + 2]
+    // (s W_NY^{(q+2) ka} is the second row's twiddle, so the sign cancels)
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
-      const C f0 = cadd(a[0][i], a[RPL - 1][i]), f1 = csub(a[0][i], a[RPL - 1][i]);
-      const C keep = q ? f1 : f0;
-      const C recv = shfl_xor_c<R>(q ? f0 : f1, ZB);
-      const C P = q ? recv : keep;                         // F_0[q]
-      const C Q = q ? keep : recv;                         // F_1[q]
-      const C Qt = q ? mk<R>(-Q.y, Q.x) : Q;               // W_4^{-q} F_1[q], W_4^{-1} = +i
-      a[0][i] = cmulc(cadd(P, Qt), wq[0]);
-      a[RPL - 1][i] = cmulc(csub(P, Qt), wq[RPL - 1]);
+      const C keep = cadd(a[0][i], a[RPL - 1][i]);
+      const C recv = shfl_xor_c<R>(csub(a[0][i], a[RPL - 1][i]), ZB);
+      const C kk = q ? mk<R>(-keep.y, keep.x) : keep;     // W_4^{-q} keep, W_4^{-1} = +i
+      a[0][i] = cmulc(cadd(kk, recv), wq[0]);
+      a[RPL - 1][i] = cmulc(csub(kk, recv), wq[RPL - 1]);
     }
   }
 
@@ -190,7 +201,7 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
 #pragma unroll
     for (int j = 0; j < RPL; ++j)
 #pragma unroll
-      for (int i = 0; i < N1; ++i) blk[base + unsigned(2 * j) * NZ + unsigned(N2 * i) * XS] = a[j][i];
+      for (int i = 0; i < N1; ++i) byte_at(blk, (base + unsigned(2 * j) * NZ + unsigned(N2 * i) * XS) * EB) = a[j][i];
   }
 }
 
