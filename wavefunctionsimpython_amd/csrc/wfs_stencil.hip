@@ -138,11 +138,12 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
 
   // Each thread owns the u cells e = tid + NT r and the w cells e = tid + NT r of the two
   // windows.  The next planes' operands are fetched into registers one iteration ahead so
-  // their latency overlaps the current plane's arithmetic; psi and V of a w cell are kept
-  // for the output at the same point one plane later.
+  // their latency overlaps the current plane's arithmetic (psi and V of plane x + 2 while
+  // plane x is computed, before w(x + 1) needs those of x + 1); psi and V of a w cell are
+  // kept for the output at the same point one plane later.
   C ureg[RU];
-  C pnext[RW], pcur[RW];
-  float2 vnext[RW], vcur[RW];
+  C pahead[RW], pnext[RW], pcur[RW];
+  float2 vahead[RW], vnext[RW], vcur[RW];
 
   auto fetch_u = [&](int x) {
     const long xo = long(clampi(x, nx)) * sx;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
     }
   };
   // psi and V of the owned in-grid w cells of plane x (x in the grid)
-  auto fetch_pv = [&](int x) {
+  auto fetch_pv_to = [&](int x, C (&pd)[RW], float2 (&vd)[RW]) {
     const long xo = long(x) * sx;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
@@ -172,11 +173,12 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
       const int gy = y0 - 1 + e / WZ, gz = z0 - 1 + e % WZ;
       if (e < WY * WZ && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
         const long i = xo + long(gy) * sy + gz;
-        pnext[r] = psi[i];
-        vnext[r] = V[i];
+        pd[r] = psi[i];
+        vd[r] = V[i];
       }
     }
   };
+  auto fetch_pv = [&](int x) { fetch_pv_to(x, pnext, vnext); };
   // w plane x from u(x-1), u(x), u(x+1) and pnext/vnext; out-of-grid cells copied after a barrier
   auto make_w = [&](int x) {
     if (x >= 0 && x < nx) {
@@ -241,10 +243,16 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
   if (x0 + 1 < nx) fetch_pv(x0 + 1);
   __syncthreads();
 
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {   // defined values where no plane x + 2 is fetched
+    pahead[r] = pnext[r];
+    vahead[r] = vnext[r];
+  }
   // state at the top of iteration x: su = u(x..x+2), sw = w(x-1), w(x), pnext = psi/V(x+1),
   // pcur = psi/V(x)
   for (int x = x0; x < x1; ++x) {
     fetch_u(x + 3);                           // into registers, stored after this plane
+    if (x + 2 < nx) fetch_pv_to(x + 2, pahead, vahead);   // used by w(x + 2), next iteration
     make_w(x + 1);                            // replaces w(x-2)
     __syncthreads();
     // output plane x at the owned core cells (w cells with 1 <= j <= TY, 1 <= k <= TZ)
@@ -263,8 +271,12 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
     }
     store_u(x + 3);                           // replaces u(x), last read by w(x + 1)
 #pragma unroll
-    for (int r = 0; r < RW; ++r) { pcur[r] = pnext[r]; vcur[r] = vnext[r]; }
-    if (x + 2 < nx) fetch_pv(x + 2);
+    for (int r = 0; r < RW; ++r) {
+      pcur[r] = pnext[r];
+      vcur[r] = vnext[r];
+      pnext[r] = pahead[r];
+      vnext[r] = vahead[r];
+    }
     __syncthreads();
   }
 }
