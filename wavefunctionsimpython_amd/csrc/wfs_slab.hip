@@ -1,24 +1,25 @@
-// Two-launch split step for complex64 grids with ny = nz = 256 or 192 (gfx950).
+// Two-launch split step for complex64 grids with ny = nz in {128, 192, 256} and nx <= 256
+// (gfx950).
 //
 // A 256^2 y-z plane (512 KiB) or a 192^2 one (288 KiB) does not fit in one CU, so the y
-// transform is split in two stages, ny = LY * B with B = 4 (LY = 64 or 48), y = B a + b,
-// ky = ka + LY kb:
+// transform is split in two stages, ny = LY * B (B = 4, LY = 64 or 48; at 128, B = 2 and
+// LY = 64), y = B a + b, ky = ka + LY kb:
 //   X[ka + LY kb] = sum_b W_B^{b kb} W_NY^{b ka} sum_a x[B a + b] W_LY^{a ka}
 // and each stage rides on a pass that is needed anyway:
 //
-//   yz_lines_kernel<NZ, LY, B 4>  (one block per x plane and residue b: the LY rows
-//     y = 4 a + b with all z, 128 KiB at 256, 72 KiB at 192):  length-LY inverse y stage
-//     over a, z inverse, potential phase, z forward, length-LY forward y stage
-//   x_slab_kernel  (one block per packet, ka and 8 or 16 consecutive z: the 4 rows
-//     y = 4 ka + b with all x):  twiddles W_NY^{b ka} and length-4 forward y stage over b, x forward,
-//     kinetic phase K(kx, ky, kz), x inverse, length-4 inverse y stage and twiddles
+//   yz_lines_kernel<NZ, LY, B>  (one block per x plane and residue b: the LY rows
+//     y = B a + b with all z, 128 KiB at 256, 72 KiB at 192, 66 KiB at 128):  length-LY
+//     inverse y stage over a, z inverse, potential phase, z forward, length-LY forward y stage
+//   x_slab_kernel  (one block per packet, ka and 8 or 16 consecutive z: the B rows
+//     y = B ka + b with all x):  twiddles W_NY^{b ka} and length-B forward y stage over b,
+//     x forward, kinetic phase K(kx, ky, kz), x inverse, length-B inverse y stage and twiddles
 //
 // Between the passes the y axis holds the half-transformed Y[b][ka] at row B ka + b.  The
 // y-z launch reads the potential phase table in lines order (LinesOrder), one contiguous run
-// per wavefront load.  A step
-// moves 16 B per point in each pass (+8 for a potential table) against 64 (+8) for the four
-// axis passes.  The arithmetic differs from the axis passes only in how the length-256 y
-// transform is factored, so the results agree with theirs to float rounding, not bit for bit.
+// per wavefront load.  A step moves 16 B per point in each pass (+8 for a potential table)
+// against 64 (+8) for the four axis passes.  The arithmetic differs from the axis passes only
+// in how the y transform is factored, so the results agree with theirs to float rounding, not
+// bit for bit.
 #include "wfs_yz_lines.h"
 
 #include <algorithm>
@@ -85,8 +86,9 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   const int ka = int((bid / TZ) % A);
   const long pk = bid / (long(TZ) * A);
   C* __restrict__ blk = data + pk * (long(NX) * NY * NZ) + long(B * ka) * NZ + long(zt) * ZB;
-  constexpr unsigned XS = unsigned(NY) * NZ;   // x stride; offsets stay below 2^32 (NX <= 512)
-  constexpr unsigned EB = unsigned(sizeof(C));  // byte offsets too: NX NY NZ 8 <= 2^27 at 256^3
+  constexpr unsigned XS = unsigned(NY) * NZ;   // x stride
+  constexpr unsigned EB = unsigned(sizeof(C));  // byte offsets within a packet: NX NY NZ 8 <= 2^27
+  static_assert(long(NX) * NY * NZ * sizeof(C) <= (1L << 32), "32-bit byte offsets");
   // element i of row q + 2 j
   auto off = [&](int i, int j) { return unsigned(zl) + unsigned(q + 2 * j) * NZ + unsigned(t + N2 * i) * XS; };
 
