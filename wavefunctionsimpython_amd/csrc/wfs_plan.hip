@@ -602,7 +602,9 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   p->elem = precision == WFS_C128 ? 16 : 8;
   p->ntab = precision == WFS_C128 ? 1 : kDither;
   for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
-  p->variant[WFS_K_HORNER] = 1;   // fused stage pairs on 16 x 64 tiles (measured fastest at 256^3)
+  // fused stage pairs: 8 x 128 tiles where the z axis fills them (measured fastest at 256^3),
+  // else 16 x 64
+  p->variant[WFS_K_HORNER] = p->n[2] >= 128 ? 3 : 1;
   // fused plane pass where the plane fits in LDS; at 128^2 two rounds (two blocks per CU) once
   // there are enough planes to fill the CUs twice over (measured: 1024 planes 13% faster, 128
   // planes 11% slower)

@@ -130,31 +130,51 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
   u_out += poff;
   psi += poff;
   V += poff;
-  const long sx = long(ny) * nz, sy = nz;
+  const long sx = long(ny) * nz;
   auto clampi = [](int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); };
   auto slot = [](int x) { return ((x % 3) + 3) % 3; };
-  // does the w window reach outside the grid in y or z (then out-of-grid w cells are copied)
-  const bool face = (y0 == 0) || (z0 == 0) || (y0 + TY >= ny) || (z0 + TZ >= nz);
 
   // Each thread owns the u cells e = tid + NT r and the w cells e = tid + NT r of the two
-  // windows.  The next planes' operands are fetched into registers one iteration ahead so
-  // their latency overlaps the current plane's arithmetic (psi and V of plane x + 2 while
-  // plane x is computed, before w(x + 1) needs those of x + 1); psi and V of a w cell are
-  // kept for the output at the same point one plane later.
+  // windows; what depends only on the cell (its voxel's offset within an x plane, its LDS
+  // positions, whether it is a core cell) is computed once.  An out-of-grid w cell is
+  // computed at its clamped voxel from the same u cells, psi and V as that voxel's own w
+  // cell, so it equals the copy the clamped Laplacian implies, with no extra pass.  Plane
+  // bases are block-uniform and in-plane offsets 32-bit (ny nz <= 2^18).
+  unsigned uoff[RU];                        // clamped voxel of u cell r, offset in its plane
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    const int e = tid + NT * r;
+    const int gy = clampi(y0 - 2 + e / UZ, ny), gz = clampi(z0 - 2 + e % UZ, nz);
+    uoff[r] = unsigned(gy) * unsigned(nz) + unsigned(gz);
+  }
+  unsigned woff[RW];                        // clamped voxel of w cell r, offset in its plane
+  int wsu[RW];                              // that voxel's position in an su plane
+  bool core[RW];                            // w cell r is an output cell of this block
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int e = tid + NT * r;
+    const int j = e / WZ, k = e % WZ;
+    const int gy = y0 - 1 + j, gz = z0 - 1 + k;
+    const int cy = clampi(gy, ny), cz = clampi(gz, nz);
+    woff[r] = unsigned(cy) * unsigned(nz) + unsigned(cz);
+    wsu[r] = (cy - (y0 - 2)) * UZ + (cz - (z0 - 2));
+    core[r] = e < WY * WZ && j >= 1 && j <= TY && k >= 1 && k <= TZ && gy < ny && gz < nz;
+  }
+
+  // The next planes' operands are fetched into registers ahead so their latency overlaps the
+  // current plane's arithmetic (u one plane ahead; psi and V of plane x + 2 while plane x is
+  // computed, before w(x + 1) needs those of x + 1); psi and V of a w cell are kept for the
+  // output at the same point one plane later.
   C ureg[RU];
   C pahead[RW], pnext[RW], pcur[RW];
   float2 vahead[RW], vnext[RW], vcur[RW];
+  constexpr unsigned EB = unsigned(sizeof(C));
 
   auto fetch_u = [&](int x) {
-    const long xo = long(clampi(x, nx)) * sx;
+    const C* __restrict__ base = u_in + long(clampi(x, nx)) * sx;
 #pragma unroll
-    for (int r = 0; r < RU; ++r) {
-      const int e = tid + NT * r;
-      if (e < UY * UZ) {
-        const int gy = clampi(y0 - 2 + e / UZ, ny), gz = clampi(z0 - 2 + e % UZ, nz);
-        ureg[r] = u_in[xo + long(gy) * sy + gz];
-      }
-    }
+    for (int r = 0; r < RU; ++r)
+      if (tid + NT * r < UY * UZ) ureg[r] = byte_at(base, uoff[r] * EB);
   };
   auto store_u = [&](int x) {
     C* dst = &su[slot(x)][0][0];
@@ -164,44 +184,34 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
       if (e < UY * UZ) dst[e] = ureg[r];
     }
   };
-  // psi and V of the owned in-grid w cells of plane x (x in the grid)
+  // psi and V at the (clamped) voxels of the owned w cells of plane x (x in the grid)
   auto fetch_pv_to = [&](int x, C (&pd)[RW], float2 (&vd)[RW]) {
-    const long xo = long(x) * sx;
+    const C* __restrict__ pb = psi + long(x) * sx;
+    const float2* __restrict__ vb = V + long(x) * sx;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
-      const int e = tid + NT * r;
-      const int gy = y0 - 1 + e / WZ, gz = z0 - 1 + e % WZ;
-      if (e < WY * WZ && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
-        const long i = xo + long(gy) * sy + gz;
-        pd[r] = psi[i];
-        vd[r] = V[i];
+      if (tid + NT * r < WY * WZ) {
+        pd[r] = byte_at(pb, woff[r] * EB);
+        vd[r] = byte_at(vb, woff[r] * unsigned(sizeof(float2)));
       }
     }
   };
   auto fetch_pv = [&](int x) { fetch_pv_to(x, pnext, vnext); };
-  // w plane x from u(x-1), u(x), u(x+1) and pnext/vnext; out-of-grid cells copied after a barrier
+  // w plane x from u(x-1), u(x), u(x+1) and pnext/vnext; a plane outside the grid is a copy
+  // of the nearest grid plane
   auto make_w = [&](int x) {
     if (x >= 0 && x < nx) {
-      const int ws = slot(x), um = slot(x - 1), uc = slot(x), up = slot(x + 1);
+      C* dst = &sw[slot(x)][0][0];
+      const C* um = &su[slot(x - 1)][0][0];
+      const C* uc = &su[slot(x)][0][0];
+      const C* up = &su[slot(x + 1)][0][0];
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
         const int e = tid + NT * r;
-        const int j = e / WZ, k = e % WZ;
-        const int gy = y0 - 1 + j, gz = z0 - 1 + k;
-        if (e < WY * WZ && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
-          const int a = j + 1, b = k + 1;
-          sw[ws][j][k] = horner_point<R>(su[uc][a][b], su[um][a][b], su[up][a][b], su[uc][a - 1][b],
-                                         su[uc][a + 1][b], su[uc][a][b - 1], su[uc][a][b + 1], pnext[r], vnext[r],
-                                         c1, inv_2m, inv_dx2);
-        }
-      }
-      if (face) {
-        __syncthreads();
-        for (int e = tid; e < WY * WZ; e += NT) {
-          const int j = e / WZ, k = e % WZ;
-          const int gy = y0 - 1 + j, gz = z0 - 1 + k;
-          if (gy >= 0 && gy < ny && gz >= 0 && gz < nz) continue;
-          sw[ws][j][k] = sw[ws][clampi(gy, ny) - (y0 - 1)][clampi(gz, nz) - (z0 - 1)];
+        if (e < WY * WZ) {
+          const int c = wsu[r];
+          dst[e] = horner_point<R>(uc[c], um[c], up[c], uc[c - UZ], uc[c + UZ], uc[c - 1], uc[c + 1], pnext[r],
+                                   vnext[r], c1, inv_2m, inv_dx2);
         }
       }
     } else {
@@ -255,18 +265,18 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
     if (x + 2 < nx) fetch_pv_to(x + 2, pahead, vahead);   // used by w(x + 2), next iteration
     make_w(x + 1);                            // replaces w(x-2)
     __syncthreads();
-    // output plane x at the owned core cells (w cells with 1 <= j <= TY, 1 <= k <= TZ)
-    const int wm = slot(x - 1), wc = slot(x), wp = slot(x + 1);
-    const long xo = long(x) * sx;
+    // output plane x at the owned core cells
+    {
+      const C* wm = &sw[slot(x - 1)][0][0];
+      const C* wc = &sw[slot(x)][0][0];
+      const C* wp = &sw[slot(x + 1)][0][0];
+      C* __restrict__ ob = u_out + long(x) * sx;
 #pragma unroll
-    for (int r = 0; r < RW; ++r) {
-      const int e = tid + NT * r;
-      const int a = e / WZ, b = e % WZ;
-      const int gy = y0 - 1 + a, gz = z0 - 1 + b;
-      if (e < WY * WZ && a >= 1 && a <= TY && b >= 1 && b <= TZ && gy < ny && gz < nz) {
-        const long i = xo + long(gy) * sy + gz;
-        u_out[i] = horner_point<R>(sw[wc][a][b], sw[wm][a][b], sw[wp][a][b], sw[wc][a - 1][b], sw[wc][a + 1][b],
-                                   sw[wc][a][b - 1], sw[wc][a][b + 1], pcur[r], vcur[r], c2, inv_2m, inv_dx2);
+      for (int r = 0; r < RW; ++r) {
+        const int e = tid + NT * r;
+        if (core[r])
+          byte_at(ob, woff[r] * EB) = horner_point<R>(wc[e], wm[e], wp[e], wc[e - WZ], wc[e + WZ], wc[e - 1],
+                                                      wc[e + 1], pcur[r], vcur[r], c2, inv_2m, inv_dx2);
       }
     }
     store_u(x + 3);                           // replaces u(x), last read by w(x + 1)
@@ -294,8 +304,8 @@ static void launch_pair_f32(int batch, int nx, int ny, int nz, const void* u_in,
 
 // variant (complex64): 1 = 16 x 64 (y, z) tile, 2 = 8 x 64, 3 = 8 x 128.  A wider z tile wastes
 // less of the partly used memory lines at its z halo, a deeper y tile less y halo; both cost LDS
-// and so blocks per CU.  Measured at 256^3 (psi and V fetched two planes ahead): 133, 141 and
-// 132 us per launch; 1 is the default.
+// and so blocks per CU.  Measured at 256^3: 127, 128 and 123 us per launch; the plan takes 3
+// where nz >= 128, else 1.
 int launch_horner_pair(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
                        const void* psi, const float2* V, double c1, double c2, double inv_2m, double inv_dx2,
                        int variant, hipStream_t s) {
