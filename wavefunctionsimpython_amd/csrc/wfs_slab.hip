@@ -109,6 +109,18 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
 #pragma unroll
     for (int i = 0; i < N1; ++i) a[j][i] = byte_at(blk, off(i, j) * EB);
 
+  // the x transforms' table twiddles W_NX^{t k1} (t < N2, k1 < N1) and the kinetic factors
+  // kx(x), staged in LDS once per block (their loads in flight with the points'); every
+  // transform of both columns reads them from there
+  __shared__ C tws[N2 * N1];
+  __shared__ C kxs[NX];
+  {
+    const C* __restrict__ tw = reinterpret_cast<const C*>(p.tw_x);
+    const C* __restrict__ kx = reinterpret_cast<const C*>(p.kx);
+    for (int e = tid; e < N2 * N1; e += 2 * ZB * N2) tws[e] = tw[((e / N1) * (e % N1)) % NX];
+    for (int e = tid; e < NX; e += 2 * ZB * N2) kxs[e] = kx[e];
+  }
+
   const R sgn = q ? R(-1) : R(1);          // s = (-1)^q
   if constexpr (B == 2) {
     // forward y stage, B = 2: X[kb] = Z[0] + (-1)^kb Z[1], lane q keeps X[q] = recv + s z
@@ -136,38 +148,33 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   }
 
   // x forward, kinetic phase, x inverse on the columns ky = ka + A kb, kb = q + 2 j, one
-  // column at a time: the table twiddles and kinetic factors are loaded where they are used
+  // column at a time: the table twiddles and kinetic factors are read where they are used
   // (opaque indices), so that only the other column's points stay live besides the transform
-  const C* __restrict__ tw = reinterpret_cast<const C*>(p.tw_x);
   auto lidx = [&](int k1, int n2) { return (k1 * N2 + n2) * LINES + line; };
-  const C* __restrict__ kx = reinterpret_cast<const C*>(p.kx);
   const C* __restrict__ ky = reinterpret_cast<const C*>(p.ky);
   const C* __restrict__ kz = reinterpret_cast<const C*>(p.kz);
+  __syncthreads();                         // tws, kxs staged
 #pragma unroll
   for (int j = 0; j < RPL; ++j) {
     C w[N1];
     if (j) __syncthreads();
     {
-      // unsigned and masked (t < N2, a power of two): the index mod NX is one mask and the
-      // product a 24-bit multiply
-      const unsigned tt = unsigned(opaque_v(t)) & unsigned(N2 - 1);
+      const C* tr = tws + opaque_v(t) * N1;
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = byte_at(tw, (tt * unsigned(k1)) % unsigned(NX) * EB);
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tr[k1];
     }
     column_dft<NX, N2, -1>(a[j], w, lds, t, lidx);
     {
-      const unsigned tt = unsigned(opaque_v(t));
+      const C* kr = kxs + opaque_v(t);
       const C kyz = cmul(ky[ka + A * (q + 2 * j)], kz[zt * ZB + zl]);
 #pragma unroll
-      for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(byte_at(kx, (tt + unsigned(N2 * i)) * EB), kyz));
+      for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(kr[N2 * i], kyz));
     }
     __syncthreads();
     {
-      // unsigned and masked (t < N2, a power of two): the index mod NX is one mask and the
-      // product a 24-bit multiply
-      const unsigned tt = unsigned(opaque_v(t)) & unsigned(N2 - 1);
+      const C* tr = tws + opaque_v(t) * N1;
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = byte_at(tw, (tt * unsigned(k1)) % unsigned(NX) * EB);
+      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tr[k1];
     }
     column_dft<NX, N2, +1>(a[j], w, lds, t, lidx);
   }
