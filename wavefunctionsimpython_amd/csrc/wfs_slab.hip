@@ -69,7 +69,6 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   static_assert(B == 4 || B == 2, "the lane-pair y stage is written for B = 4 and B = 2");
   constexpr int RPL = B / 2;               // rows (columns after the y stage) per lane
   static_assert(N1 * N2 == NX && N1 % N2 == 0 && NZ % ZB == 0 && NY % B == 0 && 2 * ZB <= 64, "bad slab split");
-  static_assert((N2 & (N2 - 1)) == 0, "the twiddle index masks t by N2 - 1");
   using C = cx<R>;
   using CC = const __attribute__((address_space(4))) C;
   constexpr int LINES = 2 * ZB;            // x lines per round: (q, z)
