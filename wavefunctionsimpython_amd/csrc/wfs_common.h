@@ -163,6 +163,26 @@ __device__ __forceinline__ T twiddle_ct(T a) {
   }
 }
 
+// Sums of seven values over the 64 lanes of a wavefront by recursive halving: at the xor-32,
+// -16 and -8 steps each lane keeps half of its remaining values and sends the other half to
+// its partner, then xor 4, 2, 1 finish one value; 10 exchanges instead of 7 x 6.  Lane l
+// returns the sum of value 4 b5 + 2 b4 + b3 (b the bits of l; 7 means none).
+__device__ __forceinline__ double wave_sum7(const double (&v)[7], int lane) {
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+  double y[4], z[2];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double lo = v[j], hi = j + 4 < 7 ? v[j + 4] : 0.0;
+    y[j] = (h5 ? hi : lo) + __shfl_xor(h5 ? lo : hi, 32, 64);
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) z[j] = (h4 ? y[j + 2] : y[j]) + __shfl_xor(h4 ? y[j] : y[j + 2], 16, 64);
+  double s = (h3 ? z[1] : z[0]) + __shfl_xor(h3 ? z[0] : z[1], 8, 64);
+#pragma unroll
+  for (int off = 4; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  return s;
+}
+
 // compile-time loop
 template <typename F, int... Is>
 __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {

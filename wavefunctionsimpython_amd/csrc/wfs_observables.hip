@@ -127,14 +127,13 @@ __global__ __launch_bounds__(kObsThreads) void box_moments_kernel(const cx<R>* _
   }
 
   __shared__ double red[kWaves][NB * kObsQ];
+  static_assert(kObsQ == 7, "the wavefront sums take seven values");
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     if (b >= set.n) break;
-#pragma unroll
-    for (int q = 0; q < kObsQ; ++q) {
-      const double v = wave_sum(acc[b][q]);
-      if (lane == 0) red[wave][b * kObsQ + q] = v;
-    }
+    const double v = wave_sum7(acc[b], lane);
+    const int q = (lane >> 3) & 7;
+    if ((lane & 7) == 0 && q < kObsQ) red[wave][b * kObsQ + q] = v;
   }
   __syncthreads();
   if (threadIdx.x < set.n * kObsQ) {

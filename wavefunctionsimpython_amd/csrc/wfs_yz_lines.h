@@ -52,25 +52,6 @@ __device__ __forceinline__ double lines_simpson(int i, int n) {
   return (i & 1) ? 4.0 : 2.0;
 }
 
-// Sums of seven values over the 64 lanes of a wavefront by recursive halving: at the xor-32,
-// -16 and -8 steps each lane keeps half of its remaining values and sends the other half to
-// its partner, then xor 4, 2, 1 finish one value; 10 exchanges instead of 7 x 6.  Lane l
-// returns the sum of value 4 b5 + 2 b4 + b3 (b the bits of l; 7 means none).
-__device__ __forceinline__ double lines_wave_sum7(const double (&v)[7], int lane) {
-  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
-  double y[4], z[2];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const double lo = v[j], hi = j + 4 < 7 ? v[j + 4] : 0.0;
-    y[j] = (h5 ? hi : lo) + __shfl_xor(h5 ? lo : hi, 32, 64);
-  }
-#pragma unroll
-  for (int j = 0; j < 2; ++j) z[j] = (h4 ? y[j + 2] : y[j]) + __shfl_xor(h4 ? y[j] : y[j + 2], 16, 64);
-  double s = (h3 ? z[1] : z[0]) + __shfl_xor(h3 ? z[0] : z[1], 8, 64);
-#pragma unroll
-  for (int off = 4; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  return s;
-}
 
 // Box moments of the position-space row a thread holds in the z-line mapping of a
 // yz_lines_kernel round (row c0 = tid mod LY of the group, z = u + N2Z i): per box
@@ -114,7 +95,7 @@ __device__ void lines_moments(const cx<R> (&a)[N1], const LinesMoments& m, long 
     const double v[7] = {wxy * s0, wxy * rx * s0, wxy * ry * s0, wxy * s1, wxy * rx * rx * s0,
                          wxy * ry * ry * s0, wxy * s2};
     {
-      const double t = lines_wave_sum7(v, lane);
+      const double t = wave_sum7(v, lane);
       const int q = (lane >> 3) & 7;
       if ((lane & 7) == 0 && q < 7) red[wave][b * 7 + q] = t;
     }
