@@ -26,8 +26,10 @@ below; the pins live in ``tests/test_oracle_*.py``.
   observation_box_voxels / volume_probability_box / expected_location_box
                                  hand-worked voxel examples
   absorbing_potential            faces: 1-D polynomial ramp closed form; corners
-                                 and edges: restatement-pinned (parity unpinned
-                                 beyond the faces: no reference vectors exist)
+                                 and edges: closed form for symmetric and for
+                                 per-side parameters (every mix() collapses),
+                                 edge->face and corner->edge limits at box planes
+                                 eps away, planted per-branch side/mix slips caught
   potential_wall                 plateau/slope closed form, slit geometry at
                                  points away from cell-boundary ties
   harmonic_potential / lattice_potential   closed form (no reference counterpart)
@@ -437,15 +439,61 @@ def _len3(a, b, c):
     return np.sqrt(a * a + b * b + c * c)
 
 
+# Corner and edge branches of absorbing_potential.cu in the kernel's order (the first
+# branch whose region holds wins).  Per entry:
+#   region  per axis -1 (below the box bottom), +1 (above its top), 0 (strictly inside)
+#   sides   per axis the side (-1 bottom / +1 top) whose corner, offset, exponent and
+#           outer potential the branch reads; the kernel reads the region's own side
+#   angle   "u": u = atan(|d_y| / |d_x|) * 2/pi blends x (u=0) with y (u=1), v = 0;
+#           "v": v = atan(|d_z| / |d_other|) * 2/pi blends the other outside axis
+#           (v=0) with z (v=1); "uv": corners, u as above and
+#           v = atan(|d_z| / sqrt(d_x^2 + d_y^2)) * 2/pi
+#   umix    for "v" branches the fixed u the kernel passes to mix() for the max
+#           distance, the exponent and the potential (an inside axis enters the max
+#           distance as the box half-size and the exponent and potential as 0)
+_CAP_BRANCHES = [
+    # corners (absorbing_potential.cu:58-155)
+    ((-1, -1, -1), (-1, -1, -1), "uv", None),       # :59-70
+    ((-1, -1, 1), (-1, -1, 1), "uv", None),         # :71-82
+    ((1, -1, 1), (1, -1, 1), "uv", None),           # :83-94
+    ((1, -1, -1), (1, -1, -1), "uv", None),         # :95-106
+    ((1, 1, 1), (1, 1, 1), "uv", None),             # :108-119
+    ((1, 1, -1), (1, 1, -1), "uv", None),           # :120-131
+    ((-1, 1, -1), (-1, 1, -1), "uv", None),         # :132-143
+    ((-1, 1, 1), (-1, 1, 1), "uv", None),           # :144-155
+    # bottom edges (:156-196)
+    ((-1, -1, 0), (-1, -1, 0), "u", None),          # :157-166
+    ((0, -1, 1), (0, -1, 1), "v", (0.0, 1.0, 1.0)),    # :167-176
+    ((1, -1, 0), (1, -1, 0), "u", None),            # :177-186
+    ((0, -1, -1), (0, -1, -1), "v", (0.0, 1.0, 1.0)),  # :187-196
+    # pillar edges (:197-237)
+    ((-1, 0, -1), (-1, 0, -1), "v", (0.0, 0.0, 0.0)),  # :198-207
+    ((-1, 0, 1), (-1, 0, 1), "v", (0.0, 0.0, 0.0)),    # :208-217
+    ((1, 0, 1), (1, 0, 1), "v", (0.0, 0.0, 0.0)),      # :218-227
+    ((1, 0, -1), (1, 0, -1), "v", (0.0, 0.0, 0.0)),    # :228-237
+    # top plane edges (:238-278)
+    ((-1, 1, 0), (-1, 1, 0), "u", None),            # :239-248
+    ((0, 1, 1), (0, 1, 1), "v", (1.0, 1.0, 1.0)),      # :249-258
+    ((1, 1, 0), (1, 1, 0), "u", None),              # :259-268
+    ((0, 1, -1), (0, 1, -1), "v", (0.0, 0.0, 0.0)),    # :269-278
+]
+
+# faces (:279-303) in the kernel's order: (axis, side)
+_CAP_FACES = [(1, -1), (2, 1), (0, 1), (2, -1), (0, -1), (1, 1)]
+
+
 def absorbing_potential(shape, dx, bottom, top, offset, pot_pos, pot_neg, exp_pos, exp_neg):
     """Imaginary part added by the complex absorbing potential (float32).
 
     sources/cuda_kernels/absorbing_potential.cu:38-306.  r = dr*v - 0.5*N*dr
-    (:52); faces :279-303 (1-D ramps), corners :58-155 and edges :156-278 blend
-    exponent, outer potential and max distance by atan angles, including the
-    reference's own mix() arguments (e.g. the u=0 maxD blends of :172, :192,
-    :274 and the u=0 exponent/potential blends of :275-276).  Returns a
-    float32 array of the per-voxel potential (to be added as +i*pot).
+    (:52); d = r - bottom below the box, r - top above it.  Faces (:279-303) are
+    1-D ramps p*(max(|d| - offset, 0) / maxD)^e with maxD = box_half - (|corner| +
+    offset); corners (:58-155) and edges (:156-278) blend maxD, e and p of the
+    axes involved with mix() (common.cu:394-397) at atan angles, including the
+    reference's own fixed mix() arguments on the "v" edges (e.g. u=0 for maxD at
+    :172, :192, :274, and u=0 for the exponent and potential at :275-276), and
+    use the length of d over the outside axes.  Returns a float32 array of the
+    per-voxel potential (to be added as +i*pot).
     """
     pi_f = _f(np.pi)
     two = _f(2.0)
@@ -465,140 +513,59 @@ def absorbing_potential(shape, dx, bottom, top, offset, pot_pos, pot_neg, exp_po
     for d in range(3):
         v = np.arange(shape[d], dtype=_f)
         r.append(dr[d] * v - half * Nf[d] * dr[d])
-    X = r[0][:, None, None]
-    Y = r[1][None, :, None]
-    Z = r[2][None, None, :]
-    X, Y, Z = np.broadcast_arrays(X, Y, Z)
-    dBx, dBy, dBz = X - b[0], Y - b[1], Z - b[2]
-    dTx, dTy, dTz = X - t[0], Y - t[1], Z - t[2]
+    R = np.broadcast_arrays(r[0][:, None, None], r[1][None, :, None], r[2][None, None, :])
+    dB = [R[d] - b[d] for d in range(3)]
+    dT = [R[d] - t[d] for d in range(3)]
+    # region tests per axis and side, all strict (a voxel on a box plane matches no branch)
+    cond = {(d, -1): dB[d] < 0 for d in range(3)}
+    cond.update({(d, 1): dT[d] > 0 for d in range(3)})
+    cond.update({(d, 0): (dB[d] > 0) & (dT[d] < 0) for d in range(3)})
     out = np.zeros(shape, dtype=_f)
     done = np.zeros(shape, dtype=bool)
-
-    def md(c):  # box_half - (|corner| + offset) for the axes in use
-        return bh - (np.abs(c) + np.array([off, off, off], dtype=_f))
 
     def ang(num, den):
         return np.arctan(num / den) / pi_f * two
 
     with np.errstate(all="ignore"):
-        # ---- corners (absorbing_potential.cu:58-155) ----
-        for sx, sy, sz in [(-1, -1, -1), (-1, -1, 1), (1, -1, 1), (1, -1, -1),
-                           (1, 1, 1), (1, 1, -1), (-1, 1, -1), (-1, 1, 1)]:
-            cx = dBx < 0 if sx < 0 else dTx > 0
-            cy = dBy < 0 if sy < 0 else dTy > 0
-            cz = dBz < 0 if sz < 0 else dTz > 0
-            m = cx & cy & cz & ~done
+        for region, sides, angle, umix in _CAP_BRANCHES:
+            m = cond[(0, region[0])] & cond[(1, region[1])] & cond[(2, region[2])] & ~done
             if not m.any():
                 continue
-            ddx = (dBx if sx < 0 else dTx)[m]
-            ddy = (dBy if sy < 0 else dTy)[m]
-            ddz = (dBz if sz < 0 else dTz)[m]
-            u = ang(np.abs(ddy), np.abs(ddx))
-            v = ang(np.abs(ddz), np.sqrt(ddx * ddx + ddy * ddy))
-            corner = np.array([b[0] if sx < 0 else t[0], b[1] if sy < 0 else t[1],
-                               b[2] if sz < 0 else t[2]], dtype=_f)
-            mdv = md(corner)
-            e3 = np.array([en[0] if sx < 0 else ep[0], en[1] if sy < 0 else ep[1],
-                           en[2] if sz < 0 else ep[2]], dtype=_f)
-            p3 = np.array([pn[0] if sx < 0 else pp[0], pn[1] if sy < 0 else pp[1],
-                           pn[2] if sz < 0 else pp[2]], dtype=_f)
-            maxD = _mix(mdv, u, v)
-            ex = _mix(e3, u, v)
-            mp = _mix(p3, u, v)
-            out[m] = np.power(np.maximum(_len3(ddx, ddy, ddz) - off, zero) / maxD, ex) * mp
-            done |= m
-
-        def edge(cond, num, den, uv, maxd3, exp3, pot3, lx, ly, lz, umix):
-            """One edge branch: ``uv`` is 'u' (x-y angle) or 'v' (z angle);
-            ``umix`` holds the fixed u used by the reference's mix() calls
-            when the angle is v (per quantity: maxD, exponent, potential)."""
-            m = cond & ~done
-            if not m.any():
-                return
-            a = ang(num[m], den[m])
-            if uv == "u":
-                q = [(a, zero)] * 3
+            # d on the outside axes (the side of the region), 0 on the inside axis
+            dd = [(dB[a] if region[a] < 0 else dT[a])[m] if region[a] else np.zeros(int(m.sum()), dtype=_f)
+                  for a in range(3)]
+            md3 = np.array([bh[a] - (np.abs(b[a] if sides[a] < 0 else t[a]) + off) if sides[a] else bh[a]
+                            for a in range(3)], dtype=_f)
+            e3 = np.array([(en[a] if sides[a] < 0 else ep[a]) if sides[a] else 0.0 for a in range(3)], dtype=_f)
+            p3 = np.array([(pn[a] if sides[a] < 0 else pp[a]) if sides[a] else 0.0 for a in range(3)], dtype=_f)
+            if angle == "uv":
+                u = ang(np.abs(dd[1]), np.abs(dd[0]))
+                v = ang(np.abs(dd[2]), np.sqrt(dd[0] * dd[0] + dd[1] * dd[1]))
+                q = [(u, v)] * 3
+            elif angle == "u":
+                u = ang(np.abs(dd[1]), np.abs(dd[0]))
+                q = [(u, zero)] * 3
             else:
-                q = [(_f(umix[0]), a), (_f(umix[1]), a), (_f(umix[2]), a)]
-            maxD = _mix(maxd3, *q[0])
-            ex = _mix(exp3, *q[1])
-            mp = _mix(pot3, *q[2])
-            ln = _len3(lx[m], ly[m], lz[m])
-            out[m] = np.power(np.maximum(ln - off, zero) / maxD, ex) * mp
-            done[m] = True
-
-        Z0 = np.zeros(shape, dtype=_f)
-        o = off
-        # ---- bottom edges (absorbing_potential.cu:156-196) ----
-        edge((dBx < 0) & (dBy < 0) & (dBz > 0) & (dTz < 0), dBy, dBx, "u",
-             bh - (np.abs(np.array([b[0], b[1], 0], dtype=_f)) + np.array([o, o, 0], dtype=_f)),
-             np.array([en[0], en[1], 0], dtype=_f), np.array([pn[0], pn[1], 0], dtype=_f),
-             dBx, dBy, Z0, None)
-        edge((dBx > 0) & (dBy < 0) & (dTx < 0) & (dTz > 0), dTz, -dBy, "v",
-             bh - (np.abs(np.array([0, b[1], t[2]], dtype=_f)) + np.array([0, o, o], dtype=_f)),
-             np.array([0, en[1], ep[2]], dtype=_f), np.array([0, pn[1], pp[2]], dtype=_f),
-             Z0, dBy, dTz, (0.0, 1.0, 1.0))
-        edge((dBy < 0) & (dBz > 0) & (dTx > 0) & (dTz < 0), -dBy, dTx, "u",
-             bh - (np.abs(np.array([t[0], b[1], 0], dtype=_f)) + np.array([o, o, 0], dtype=_f)),
-             np.array([ep[0], en[1], 0], dtype=_f), np.array([pp[0], pn[1], 0], dtype=_f),
-             dTx, dBy, Z0, None)
-        edge((dBx > 0) & (dBy < 0) & (dBz < 0) & (dTx < 0), dBz, dBy, "v",
-             bh - (np.abs(np.array([0, b[1], b[2]], dtype=_f)) + np.array([0, o, o], dtype=_f)),
-             np.array([0, en[1], en[2]], dtype=_f), np.array([0, pn[1], pn[2]], dtype=_f),
-             Z0, dBy, dBz, (0.0, 1.0, 1.0))
-        # ---- pillar edges (absorbing_potential.cu:197-237) ----
-        edge((dBx < 0) & (dBy > 0) & (dBz < 0) & (dTy < 0), dBz, dBx, "v",
-             bh - (np.abs(np.array([b[0], 0, b[2]], dtype=_f)) + np.array([o, 0, o], dtype=_f)),
-             np.array([en[0], 0, en[2]], dtype=_f), np.array([pn[0], 0, pn[2]], dtype=_f),
-             dBx, Z0, dBz, (0.0, 0.0, 0.0))
-        edge((dBx < 0) & (dBy > 0) & (dTy < 0) & (dTz > 0), dTz, -dBx, "v",
-             bh - (np.abs(np.array([b[0], 0, t[2]], dtype=_f)) + np.array([o, 0, o], dtype=_f)),
-             np.array([en[0], 0, ep[2]], dtype=_f), np.array([pn[0], 0, pp[2]], dtype=_f),
-             dBx, Z0, dTz, (0.0, 0.0, 0.0))
-        edge((dBy > 0) & (dTx > 0) & (dTy < 0) & (dTz > 0), dTz, dTx, "v",
-             bh - (np.abs(np.array([t[0], 0, t[2]], dtype=_f)) + np.array([o, 0, o], dtype=_f)),
-             np.array([ep[0], 0, ep[2]], dtype=_f), np.array([pp[0], 0, pp[2]], dtype=_f),
-             dTx, Z0, dTz, (0.0, 0.0, 0.0))
-        edge((dBy > 0) & (dBz < 0) & (dTx > 0) & (dTy < 0), -dBz, dTx, "v",
-             bh - (np.abs(np.array([t[0], 0, b[2]], dtype=_f)) + np.array([o, 0, o], dtype=_f)),
-             np.array([ep[0], 0, en[2]], dtype=_f), np.array([pp[0], 0, pn[2]], dtype=_f),
-             dTx, Z0, dBz, (0.0, 0.0, 0.0))
-        # ---- top plane edges (absorbing_potential.cu:238-278) ----
-        edge((dBx < 0) & (dBz > 0) & (dTy > 0) & (dTz < 0), dTy, -dBx, "u",
-             bh - (np.abs(np.array([b[0], t[1], 0], dtype=_f)) + np.array([o, o, 0], dtype=_f)),
-             np.array([en[0], ep[1], 0], dtype=_f), np.array([pn[0], pp[1], 0], dtype=_f),
-             dBx, dTy, Z0, None)
-        edge((dBx > 0) & (dTx < 0) & (dTy > 0) & (dTz > 0), dTz, dTy, "v",
-             bh - (np.abs(np.array([0, t[1], t[2]], dtype=_f)) + np.array([0, o, o], dtype=_f)),
-             np.array([0, ep[1], ep[2]], dtype=_f), np.array([0, pp[1], pp[2]], dtype=_f),
-             Z0, dTy, dTz, (1.0, 1.0, 1.0))
-        edge((dBz > 0) & (dTx > 0) & (dTy > 0) & (dTz < 0), dTy, dTx, "u",
-             bh - (np.abs(np.array([t[0], t[1], 0], dtype=_f)) + np.array([o, o, 0], dtype=_f)),
-             np.array([ep[0], ep[1], 0], dtype=_f), np.array([pp[0], pp[1], 0], dtype=_f),
-             dTx, dTy, Z0, None)
-        edge((dBx > 0) & (dBz < 0) & (dTx < 0) & (dTy > 0), -dBz, dTy, "v",
-             bh - (np.abs(np.array([0, t[1], b[2]], dtype=_f)) + np.array([0, o, o], dtype=_f)),
-             np.array([0, ep[1], en[2]], dtype=_f), np.array([0, pp[1], pn[2]], dtype=_f),
-             Z0, dTy, dBz, (0.0, 0.0, 0.0))
-        # ---- faces (absorbing_potential.cu:279-303) ----
-
-        def face(cond, dist, axis, side):
-            m = cond & ~done
+                other = 0 if region[0] else 1
+                v = ang(np.abs(dd[2]), np.abs(dd[other]))
+                q = [(_f(umix[k]), v) for k in range(3)]
+            maxD = _mix(md3, *q[0])
+            ex = _mix(e3, *q[1])
+            mp = _mix(p3, *q[2])
+            out[m] = np.power(np.maximum(_len3(dd[0], dd[1], dd[2]) - off, zero) / maxD, ex) * mp
+            done |= m
+        for axis, side in _CAP_FACES:
+            m = ~done
+            for a in range(3):
+                m &= cond[(a, side if a == axis else 0)]
             if not m.any():
-                return
-            corner = b[axis] if side < 0 else t[axis]
-            maxD = bh[axis] - (np.abs(corner) + off)
+                continue
+            dist = (-dB[axis] if side < 0 else dT[axis])[m]
+            maxD = bh[axis] - (np.abs(b[axis] if side < 0 else t[axis]) + off)
             e = en[axis] if side < 0 else ep[axis]
             p = pn[axis] if side < 0 else pp[axis]
-            out[m] = np.power(np.maximum(dist[m] - off, zero) / maxD, e) * p
-            done[m] = True
-
-        face((dBx > 0) & (dBy < 0) & (dBz > 0) & (dTx < 0) & (dTz < 0), -dBy, 1, -1)
-        face((dBx > 0) & (dBy > 0) & (dTx < 0) & (dTy < 0) & (dTz > 0), dTz, 2, 1)
-        face((dBy > 0) & (dBz > 0) & (dTx > 0) & (dTy < 0) & (dTz < 0), dTx, 0, 1)
-        face((dBx > 0) & (dBy > 0) & (dBz < 0) & (dTx < 0) & (dTy < 0), -dBz, 2, -1)
-        face((dBx < 0) & (dBy > 0) & (dBz > 0) & (dTy < 0) & (dTz < 0), -dBx, 0, -1)
-        face((dBx > 0) & (dBz > 0) & (dTx < 0) & (dTy > 0) & (dTz < 0), dTy, 1, 1)
+            out[m] = np.power(np.maximum(dist - off, zero) / maxD, e) * p
+            done |= m
     return out
 
 

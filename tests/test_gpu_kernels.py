@@ -12,6 +12,8 @@ import torch
 import oracle
 from wavefunctionsimpython_amd import _abi
 
+import cap_closed_forms
+
 pytestmark = pytest.mark.gpu
 
 
@@ -195,6 +197,43 @@ def test_cap_generator(dev, shape, bottom, top, offset):
     assert np.all(got.real == 0)
     scale = np.max(np.abs(ref))
     assert np.max(np.abs(got.imag - ref)) <= 4e-6 * scale
+
+
+def test_cap_kernel_corner_and_edge_pins(dev):
+    """The GPU CAP at the oracle's pinning geometries: symmetric-parameter closed form for every
+    voxel, per-side closed form, and the edge/face and corner/edge limits at eps = 2e-3 Bohr
+    (the same bounds as the oracle pins), plus agreement with the oracle itself."""
+    g = dict(shape=(24, 20, 28), dx=(0.5, 0.6, 0.45), offset=0.4)
+    bh = cap_closed_forms.box_half(g["shape"], g["dx"])
+
+    def gpu_cap(shape, dx, bottom, top, offset, pp, pn, ep, en):
+        V = torch.zeros(shape, dtype=torch.complex64, device=dev)
+        _abi.add_cap(shape, dx, bottom, top, offset, pp, pn, ep, en, V)
+        out = V.cpu().numpy()
+        assert np.all(out.real == 0)
+        ref = oracle.absorbing_potential(shape, dx, bottom, top, offset, pp, pn, ep, en)
+        assert np.max(np.abs(out.imag - ref)) <= 4e-6 * np.max(np.abs(ref))
+        return out.imag
+
+    bottom, top = tuple(-(bh - 3.1)), tuple(bh - 3.1)
+    cap = gpu_cap(g["shape"], g["dx"], bottom, top, g["offset"], (-37.5,) * 3, (-37.5,) * 3, (2.5,) * 3, (2.5,) * 3)
+    want = cap_closed_forms.symmetric_closed_form(g["shape"], g["dx"], bottom, top, g["offset"], -37.5, 2.5)
+    assert np.max(np.abs(cap - want)) / 37.5 < 1e-5
+
+    bottom, top = tuple(-(bh - 3.1)), tuple(bh - 4.3)
+    cap = gpu_cap(g["shape"], g["dx"], bottom, top, g["offset"], (-70.0,) * 3, (-20.0,) * 3, (3.0,) * 3, (1.5,) * 3)
+    want, mask = cap_closed_forms.per_side_closed_form(g["shape"], g["dx"], bottom, top, g["offset"], -20.0, -70.0,
+                                                       1.5, 3.0)
+    assert np.max(np.abs(cap - want)[mask]) / 70.0 < 1e-5
+
+    shape, dx, k, off = (26, 22, 30), (0.5, 0.55, 0.45), 6, 0.3
+    pp, pn, ep, en = (-60.0, -25.0, -90.0), (-35.0, -80.0, -15.0), (2.0, 3.0, 1.5), (2.5, 1.5, 3.5)
+    bottom, top = cap_closed_forms.eps_planes(shape, dx, k, 2e-3)
+    cap = gpu_cap(shape, dx, bottom, top, off, pp, pn, ep, en)
+    lim = cap_closed_forms.edge_face_limits(cap, shape, dx, bottom, top, off, pp, pn, ep, en, k)
+    cont = cap_closed_forms.corner_edge_continuity(cap, shape, k)
+    bad = {key: v for key, v in {**lim, **cont}.items() if v > 1e-2}
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("slits,normal,rot", [(0, (1, 0, 0), 0.0), (1, (1, 0, 0), 0.0), (2, (1, 0, 0), 0.0),

@@ -13,6 +13,8 @@ import scipy.linalg
 
 import oracle
 
+import cap_closed_forms
+
 RNG = np.random.default_rng(1234)
 
 
@@ -318,6 +320,100 @@ def test_cap_corner_is_symmetric_for_symmetric_parameters():
         a = cap[16 + 8 + i, 16 + 8 + j, 16 + 8 + k]
         b = cap[16 - 8 - i, 16 - 8 - j, 16 - 8 - k]
         assert a < 0 and np.isclose(a, b, rtol=1e-6)
+
+
+# The corner and edge blends (absorbing_potential.cu:58-278): closed forms under symmetric
+# parameters, one value per side, and the limits at the box planes (tests/cap_closed_forms.py).
+_CAP_SYM = dict(shape=(24, 20, 28), dx=(0.5, 0.6, 0.45), depth=3.1, offset=0.4, p=-37.5, e=2.5)
+_CAP_SIDE = dict(depth_neg=3.1, depth_pos=4.3, p_neg=-20.0, p_pos=-70.0, e_neg=1.5, e_pos=3.0)
+_CAP_EPS = dict(shape=(26, 22, 30), dx=(0.5, 0.55, 0.45), k=6, eps=2e-3, offset=0.3,
+                pp=(-60.0, -25.0, -90.0), pn=(-35.0, -80.0, -15.0), ep=(2.0, 3.0, 1.5), en=(2.5, 1.5, 3.5))
+
+
+def _cap_symmetric_deviation():
+    g = _CAP_SYM
+    bh = cap_closed_forms.box_half(g["shape"], g["dx"])
+    bottom, top = tuple(-(bh - g["depth"])), tuple(bh - g["depth"])
+    cap = oracle.absorbing_potential(g["shape"], g["dx"], bottom, top, g["offset"], (g["p"],) * 3, (g["p"],) * 3,
+                                     (g["e"],) * 3, (g["e"],) * 3)
+    want = cap_closed_forms.symmetric_closed_form(g["shape"], g["dx"], bottom, top, g["offset"], g["p"], g["e"])
+    return float(np.max(np.abs(cap - want)) / abs(g["p"])), cap, (bottom, top)
+
+
+def _cap_per_side_deviation():
+    g, q = _CAP_SYM, _CAP_SIDE
+    bh = cap_closed_forms.box_half(g["shape"], g["dx"])
+    bottom, top = tuple(-(bh - q["depth_neg"])), tuple(bh - q["depth_pos"])
+    cap = oracle.absorbing_potential(g["shape"], g["dx"], bottom, top, g["offset"], (q["p_pos"],) * 3,
+                                     (q["p_neg"],) * 3, (q["e_pos"],) * 3, (q["e_neg"],) * 3)
+    want, mask = cap_closed_forms.per_side_closed_form(g["shape"], g["dx"], bottom, top, g["offset"], q["p_neg"],
+                                                       q["p_pos"], q["e_neg"], q["e_pos"])
+    return float(np.max(np.abs(cap - want)[mask]) / abs(q["p_pos"]))
+
+
+def _cap_limit_deviations():
+    g = _CAP_EPS
+    bottom, top = cap_closed_forms.eps_planes(g["shape"], g["dx"], g["k"], g["eps"])
+    cap = oracle.absorbing_potential(g["shape"], g["dx"], bottom, top, g["offset"], g["pp"], g["pn"], g["ep"], g["en"])
+    lim = cap_closed_forms.edge_face_limits(cap, g["shape"], g["dx"], bottom, top, g["offset"], g["pp"], g["pn"],
+                                            g["ep"], g["en"], g["k"])
+    cont = cap_closed_forms.corner_edge_continuity(cap, g["shape"], g["k"])
+    return lim, cont
+
+
+def test_cap_symmetric_parameters_closed_form():
+    """Every voxel, including all 8 corners and 12 edges, equals p (max(|d| - off, 0) / maxD)^e with
+    the collapsed blends (and the box-half-size / zero entries of the (0,-1,+-1), (0,+1,-1) edges)."""
+    dev, cap, (bottom, top) = _cap_symmetric_deviation()
+    assert dev < 1e-5
+    sides, _ = cap_closed_forms.side_distances(_CAP_SYM["shape"], _CAP_SYM["dx"], bottom, top)
+    S = np.stack(np.meshgrid(*sides, indexing="ij")).reshape(3, -1)
+    regions = {tuple(c) for c in S.T if 0 < np.count_nonzero(c) and 9 not in c}
+    assert len(regions) == 26                       # 6 faces, 12 edges, 8 corners all sampled
+    assert np.all(cap[np.all(S == 0, axis=0).reshape(cap.shape)] == 0)
+
+
+def test_cap_per_side_parameters_closed_form():
+    """One potential / exponent / layer depth per side: voxels whose outside axes share a side
+    read that side's parameters (a pos/neg swap in a same-side corner or edge fails)."""
+    assert _cap_per_side_deviation() < 1e-5
+
+
+def test_cap_edges_meet_faces_and_corners_meet_edges():
+    """Box planes eps = 2e-3 Bohr beyond a voxel plane, distinct parameters per axis and side:
+    each edge voxel at eps meets the face ramp its mix() arguments select (or the kernel's own
+    limit on the (0,-1,+-1) / (0,+1,-1) edges), each corner voxel at eps meets its edge."""
+    lim, cont = _cap_limit_deviations()
+    assert len(lim) == 24 and len(cont) == 8
+    bad = {k: v for k, v in {**lim, **cont}.items() if v > 1e-2}
+    assert not bad, bad
+
+
+def _plant(branch_region, sides=None, umix=None):
+    br = list(oracle.wfs_oracle._CAP_BRANCHES)
+    i = [b[0] for b in br].index(branch_region)
+    r, s, a, u = br[i]
+    br[i] = (r, sides or s, a, umix or u)
+    return br
+
+
+@pytest.mark.parametrize("region,sides,umix", [
+    ((1, -1, 1), (1, 1, 1), None),          # corner reads the top plate's y parameters
+    ((-1, -1, -1), (-1, -1, 1), None),      # same-side corner, z swapped
+    ((-1, 1, 1), (1, 1, 1), None),          # corner, x swapped
+    ((1, -1, 0), (-1, -1, 0), None),        # "u" edge, x swapped
+    ((-1, 0, 1), (-1, 0, -1), None),        # pillar edge, z swapped
+    ((0, -1, 1), (0, 1, 1), None),          # (0,-1,+1) edge, y swapped
+    ((0, 1, 1), None, (0.0, 1.0, 1.0)),     # wrong fixed u for the max distance
+    ((0, -1, -1), None, (1.0, 1.0, 1.0)),   # the reference's u = 0 "fixed"
+    ((0, 1, -1), None, (1.0, 1.0, 1.0)),    # the reference's u = 0 "fixed"
+])
+def test_planted_cap_branch_slips_are_caught(monkeypatch, region, sides, umix):
+    monkeypatch.setattr(oracle.wfs_oracle, "_CAP_BRANCHES", _plant(region, sides, umix))
+    lim, cont = _cap_limit_deviations()
+    caught = (_cap_symmetric_deviation()[0] > 1e-5 or _cap_per_side_deviation() > 1e-5
+              or max(lim.values()) > 1e-2 or max(cont.values()) > 1e-2)
+    assert caught
 
 
 def test_wall_plateau_slope_and_slits():
