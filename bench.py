@@ -459,8 +459,9 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU decomposition is "
-              "described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path "
+              "(an x-slab decomposition with one all-to-all per transform is designed, not built)",
+              file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py is a one-GPU benchmark; run it without torch.distributed", file=sys.stderr)

@@ -94,6 +94,23 @@ int wfs_plan_destroy(wfs_plan* plan);
 int wfs_step(wfs_plan* plan, void* psi, const void* V, int nsteps, void* stream);
 
 /*
+ * Table-copy schedule of the split-step (no reference counterpart; part of the
+ * precision policy).  Complex64 plans hold 8 dithered copies of every per-axis
+ * twiddle and kinetic table whose mean is within 1/16 ulp of the exact value,
+ * and step k of the run uses copy k mod 8, so that float rounding does not
+ * repeat from step to step (complex128 plans hold one copy).  The plan keeps a
+ * running index: every wfs_step / wfs_step_observe call advances it by its step
+ * count, so the reference's loop of one evolve_state() per step
+ * (sources/core_sim.py:42-55) cycles the copies as one long call does.  A
+ * captured graph replays the copies of its capture (capture a multiple of 8
+ * steps to keep the cycle).  A caller that needs a call to be the same map every
+ * time (parareal's fine and coarse propagators) sets the index before the call.
+ * wfs_plan_get_step_index returns the index in [0, 8) (-1 for a null plan).
+ */
+int wfs_plan_set_step_index(wfs_plan* plan, long index);
+long wfs_plan_get_step_index(wfs_plan* plan);
+
+/*
  * Store the potential of subsequent wfs_step calls made with V == NULL.
  * For the split-step the plan keeps the phase table cexp_i(-dt V) (the intent
  * of operators.init_potential_operator, sources/operators.py:41-55); callers
@@ -137,9 +154,8 @@ int wfs_observables(wfs_plan* plan, const void* psi, const wfs_box* boxes, int n
  * out[o][packet][box][7] (device; o < ceil(nsteps / observe_every)): the
  * results of alternating wfs_observables and wfs_step(observe_every) calls,
  * which is how core_sim.run_iteration measures before each step
- * (sources/core_sim.py:19-65, sources/measurement.py:644-704) with the step
- * count between measurements of the SURVEY's wfs_step(..., observe_every, ...)
- * boundary.  With the slab scheme (WFS_K_YZ_SLAB) the exit launch of one
+ * (sources/core_sim.py:19-65, sources/measurement.py:644-704), here with
+ * observe_every steps between measurements.  With the slab scheme (WFS_K_YZ_SLAB) the exit launch of one
  * stretch, the observation and the entry launch of the next run as one launch;
  * the state is bit-identical to the alternating calls and the observables
  * agree to FP64 rounding (summation order).  V as for wfs_step.
@@ -347,8 +363,11 @@ int wfs_narrow_c128(const void* U, void* out, long n, void* stream);
  * wfs_* call ordered on it (and on streams joined to it by events) is
  * recorded instead of executed; wfs_graph_end instantiates the graph;
  * wfs_graph_launch replays it.  Plans used inside must not allocate during
- * capture (power-series scratch is allocated by a first uncaptured wfs_step;
- * wfs_set_potential must have been called once before capture).
+ * capture: power-series scratch is allocated by a first uncaptured wfs_step,
+ * the potential table by a first uncaptured wfs_set_potential, a plane-current
+ * workspace by a first uncaptured wfs_plane_current; a call that would allocate
+ * on a capturing stream returns WFS_ERR_INVALID instead.  A graph replays the
+ * table copies of its capture (wfs_plan_set_step_index).
  */
 typedef struct wfs_graph wfs_graph;
 int wfs_graph_begin(void* stream, wfs_graph** out);

@@ -53,6 +53,7 @@ def test_graph_replays_match_eager(dev):
     g = graphed.capture(10)
     assert g.node_count() > 0
     for replay in range(3):
+        eager.set_step_index(0)       # a replay uses the table copies of its capture
         eager.run(10)
         torch.cuda.synchronize()
         g.launch()

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def config1(tmp):
-    """64^3 complex128 free packet."""
+    """64^3 complex128 free packet (the free-packet workload)."""
     return {
         "simulation": {"method": "fourier", "delta_time_h_bar_per_hartree": 0.01, "total_iteration_count": 200,
                        "double_precision_calculation": True, "is_dynamic_potential_mode": False},

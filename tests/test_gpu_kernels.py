@@ -72,7 +72,7 @@ def _config1_state(prec):
 
 
 def test_split_step_free_packet_c128(dev):
-    """64^3 complex128 free packet, 200 steps of dt = 0.01."""
+    """64^3 complex128 free packet, 200 steps of dt = 0.01 (the free-packet workload)."""
     shape, dx, psi0 = _config1_state(_abi.WFS_C128)
     V = np.zeros(shape, np.complex64)
     p = _plan(shape, dx, 0.01, 1.0, _abi.WFS_C128)

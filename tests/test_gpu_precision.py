@@ -7,7 +7,9 @@ step, so the complex64 error would grow linearly with the step count (1e-7 per
 step at 128^3 with plainly rounded tables).  The in-register DFT twiddles are
 applied as hi + lo pairs and the per-axis tables are cycled through dithered
 copies; measured at 128^3 over 1600 steps: 1.3e-5 (lattice), 3.8e-5 (free),
-5.4e-6 after 250 steps.
+5.4e-6 after 250 steps.  The plan's running copy index makes the reference's
+calling pattern -- one evolve_state() per step, sources/core_sim.py:42-55 --
+cycle the copies as one long call does.
 """
 
 import numpy as np
@@ -67,3 +69,29 @@ def test_complex64_within_1e5_for_250_steps_at_128(dev):
     err = _rel(a, b)
     print(f"250 steps complex64 vs complex128: rel L2 {err:.2e}")
     assert err < 1e-5
+
+
+def test_complex64_one_step_per_call_250_steps_at_128(dev):
+    """250 calls of one step each (the reference loop's pattern) stay within 1e-5 of the
+    complex128 run; restarting the table copies at copy 0 on every call (what a per-call
+    schedule would do) rounds every step the same way and drifts further."""
+    prob = pr.random_lattice_problem(np.random.default_rng(2), n=128, dx=0.25, k0=0.5, r0=3.0)
+    V = prob.potential()
+    psi0 = prob.packet()
+    p32 = _abi.Plan(prob.shape, (0.25,) * 3, prob.dt, 1.0, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER)
+    p64 = _abi.Plan(prob.shape, (0.25,) * 3, prob.dt, 1.0, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+    p32.set_potential(V)
+    b = psi0.to(torch.complex128)
+    p64.step(b, V, 250)
+    a = psi0.clone()
+    for k in range(250):
+        assert p32.step_index() == k % 8
+        p32.step(a, None, 1)
+    fixed = psi0.clone()
+    for _ in range(250):
+        p32.set_step_index(0)
+        p32.step(fixed, None, 1)
+    err, err_fixed = _rel(a, b), _rel(fixed, b)
+    print(f"250 x step(1) complex64 vs complex128: rel L2 {err:.2e} (copy 0 on every call: {err_fixed:.2e})")
+    assert err < 1e-5
+    assert err < err_fixed

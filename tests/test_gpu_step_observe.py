@@ -91,6 +91,16 @@ def test_slab_fused_observations_batched_and_192(dev):
     _check(plans, psi, V, 12, 4, _boxes(shape[1:]))
 
 
+@pytest.mark.parametrize("shape", [(16, 128, 128), (128, 128, 128)])
+def test_slab_fused_observations_at_128(dev, shape):
+    """the 128^2 form of the observing launch (64-row groups of rows y = 2a + b, 512 threads):
+    a single 128^3 grid runs the slab scheme, so step_observe takes this path there"""
+    print(f"case: slab at 128^2, shape {shape}, 12 steps every 5, 3 boxes", flush=True)
+    plans, psi, V = _pair(shape, C64, _abi.WFS_METHOD_FOURIER, 1, None, 9)
+    assert plans[0].variants()["yz_slab"] != 0
+    _check(plans, psi, V, 12, 5, _boxes(shape))
+
+
 def test_slab_fused_observations_four_and_five_boxes(dev):
     shape = (8, 256, 256)
     boxes = _boxes(shape) + [((2, 2, 2), (3, 5, 7)), ((4, 10, 20), (1, 1, 1))]

@@ -83,6 +83,8 @@ SIGNATURES = {
                                             _vp, _vp]),
     "wfs_potential_operator": (ctypes.c_int, [ctypes.c_long, _vp, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "wfs_plan_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "wfs_plan_set_step_index": (ctypes.c_int, [_vp, ctypes.c_long]),
+    "wfs_plan_get_step_index": (ctypes.c_long, [_vp]),
     "wfs_set_potential": (ctypes.c_int, [_vp, _vp, _vp]),
     "wfs_set_potential_axes": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "wfs_plan_autotune": (ctypes.c_int, [_vp, _vp]),
@@ -351,6 +353,13 @@ class Plan:
                                       (ctypes.c_int * 2)(cu, cv), ptr(density) if density is not None else None,
                                       ptr(out), stream_handle(stream)), "wfs_plane_current")
         return out
+
+    def set_step_index(self, index: int) -> None:
+        """Table copy of the next step (the plan advances it by every call's steps)."""
+        check(lib().wfs_plan_set_step_index(self._h, int(index)), "wfs_plan_set_step_index")
+
+    def step_index(self) -> int:
+        return int(lib().wfs_plan_get_step_index(self._h))
 
     def set_profiling(self, enable: bool) -> None:
         check(lib().wfs_plan_set_profiling(self._h, 1 if enable else 0), "wfs_plan_set_profiling")

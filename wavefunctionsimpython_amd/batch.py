@@ -101,6 +101,11 @@ class PacketBatch:
     def total_points(self) -> int:
         return sum(int(np.prod(s.n)) for s in self.specs)
 
+    def set_step_index(self, index: int) -> None:
+        """Table copy of every class's next step (a captured graph replays its capture's copies)."""
+        for cls in self.classes:
+            cls.plan.set_step_index(index)
+
     # ------------------------------------------------------------------ stepping
     def _enqueue(self, cls: _Class, nsteps: int, observe_every: int, obs_offset: int) -> int:
         """Steps (and observables) of one class on its stream; returns observations written."""

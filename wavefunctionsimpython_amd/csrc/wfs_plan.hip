@@ -17,6 +17,7 @@ struct wfs_plan {
   size_t elem;            // bytes per state element
   void* tw[3];            // forward twiddles per axis, state precision; ntab dithered copies
   int ntab;               // copies of each per-axis table (complex64: kDither, complex128: 1)
+  int step_index;         // table copy of the next step, in [0, ntab); advanced by every call's steps
   void* kfull[3];         // exp(-i k_d^2 dt / 2m) / N_d, state precision, ntab copies
   void* khalf[3];         // exp(-i k_d^2 dt / 4m) / N_d, state precision, ntab copies
   void* scratch[2];       // power-series ping-pong buffers (lazy)
@@ -164,8 +165,21 @@ void kinetic_table(long n, double dx, double dt, double mass, double f, std::vec
   }
 }
 
-int ensure_scratch(wfs_plan* p) {
+// Lazy allocations cannot be recorded into a graph: a call that would allocate on a stream under
+// capture fails with a message instead (the caller makes one uncaptured call first).
+int refuse_if_capturing(hipStream_t s, const char* what) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone) {
+    set_last_error("%s: its first call allocates a workspace and cannot be captured; make one uncaptured call first",
+                   what);
+    return WFS_ERR_INVALID;
+  }
+  return 0;
+}
+
+int ensure_scratch(wfs_plan* p, hipStream_t s) {
   if (p->scratch[0]) return 0;
+  WFS_TRY(refuse_if_capturing(s, "power-series wfs_step"));
   const size_t bytes = size_t(p->batch) * p->n[0] * p->n[1] * p->n[2] * p->elem;
   for (int i = 0; i < 2; ++i) {
     hipError_t e = hipMalloc(&p->scratch[i], bytes);
@@ -269,8 +283,9 @@ size_t grid_points(const wfs_plan* p) { return size_t(p->batch) * p->n[0] * p->n
 
 bool slab_on(const wfs_plan* p) { return p->variant[WFS_K_YZ_SLAB] != 0; }
 
-int alloc_table(wfs_plan* p, void** t) {
+int alloc_table(wfs_plan* p, void** t, hipStream_t s) {
   if (*t) return 0;
+  WFS_TRY(refuse_if_capturing(s, "the potential phase table"));
   hipError_t e = hipMalloc(t, grid_points(p) * p->elem);
   if (e != hipSuccess) { *t = nullptr; set_last_error("hipMalloc for the potential phase table failed"); return WFS_ERR_ALLOC; }
   return 0;
@@ -281,13 +296,13 @@ int alloc_table(wfs_plan* p, void** t) {
 int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
   const long count = long(grid_points(p));
   if (slab_on(p)) {
-    WFS_TRY(alloc_table(p, &p->pv_lines));
+    WFS_TRY(alloc_table(p, &p->pv_lines, s));
     WFS_TRY(launch_potential_phase_lines(p->precision, count, static_cast<const float2*>(V), p->dt, p->pv_lines,
                                          slab_lines_order(p->n[1]), s));
     p->pv_lines_ok = true;
     p->pv_nat_ok = false;
   } else {
-    WFS_TRY(alloc_table(p, &p->pv));
+    WFS_TRY(alloc_table(p, &p->pv, s));
     WFS_TRY(launch_potential_phase(p->precision, count, static_cast<const float2*>(V), p->dt, p->pv, s));
     p->pv_nat_ok = true;
     p->pv_lines_ok = false;
@@ -305,14 +320,14 @@ int current_table(wfs_plan* p, hipStream_t s, const void** out) {
   const long count = long(grid_points(p));
   if (slab_on(p)) {
     if (!p->pv_lines_ok) {
-      WFS_TRY(alloc_table(p, &p->pv_lines));
+      WFS_TRY(alloc_table(p, &p->pv_lines, s));
       WFS_TRY(launch_table_reorder(p->precision, count, p->pv, p->pv_lines, slab_lines_order(p->n[1]), true, s));
       p->pv_lines_ok = true;
     }
     *out = p->pv_lines;
   } else {
     if (!p->pv_nat_ok) {
-      WFS_TRY(alloc_table(p, &p->pv));
+      WFS_TRY(alloc_table(p, &p->pv, s));
       WFS_TRY(launch_table_reorder(p->precision, count, p->pv_lines, p->pv, slab_lines_order(p->n[1]), false, s));
       p->pv_nat_ok = true;
     }
@@ -413,27 +428,23 @@ int run_x_slab(wfs_plan* p, void* data, const void* const* ktab, hipStream_t s, 
 }
 
 // Slab scheme: the exit launch of one call (table copy sel_exit), the box moments of the
-// position-space state, and the entry launch of the next call (copy 0) as one launch, then
-// the records summed into out[packet][box][7]
-int run_slab_observe(wfs_plan* p, void* data, int sel_exit, const wfs_box* boxes, int nb, double* out,
-                     hipStream_t s) {
+// position-space state, and the entry launch of the next call (copy sel_entry) as one launch,
+// then the records summed into out[packet][box][7].  The records (obs_lines) are allocated
+// with the plan.
+int run_slab_observe(wfs_plan* p, void* data, int sel_exit, int sel_entry, const wfs_box* boxes, int nb,
+                     double* out, hipStream_t s) {
   const SlabRows g = slab_rows(p->n[1]);
   const long groups = long(p->batch) * p->n[0] * g.b;
-  const long need = groups * kBoxesPerPass * 7;
-  if (p->obs_lines_len < need) {
-    if (p->obs_lines) (void)hipFree(p->obs_lines);
-    p->obs_lines = nullptr;
-    p->obs_lines_len = 0;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->obs_lines), size_t(need) * sizeof(double));
-    if (e != hipSuccess) { set_last_error("hipMalloc for observing records failed"); return WFS_ERR_ALLOC; }
-    p->obs_lines_len = need;
+  if (p->obs_lines_len < groups * kBoxesPerPass * 7) {
+    set_last_error("wfs_step_observe: observing records missing");
+    return WFS_ERR_INVALID;
   }
   YzPlaneParams pp;
   const size_t ysz = size_t(g.uy) * g.n1 * p->elem, zsz = size_t(g.uz) * g.n1 * p->elem;
   pp.tw_yi = static_cast<const char*>(p->tw_rows[0]) + size_t(sel_exit % p->ntab) * ysz;
   pp.tw_z = static_cast<const char*>(p->tw_rows[1]) + size_t(sel_exit % p->ntab) * zsz;
-  pp.tw_yf = p->tw_rows[0];
-  pp.tw_zf = p->tw_rows[1];
+  pp.tw_yf = static_cast<const char*>(p->tw_rows[0]) + size_t(sel_entry % p->ntab) * ysz;
+  pp.tw_zf = static_cast<const char*>(p->tw_rows[1]) + size_t(sel_entry % p->ntab) * zsz;
   pp.nx = p->n[0];
   pp.planes = long(p->batch) * p->n[0];
   pp.mom.nb = nb;
@@ -465,26 +476,30 @@ int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t
   WFS_TRY(current_table(p, s, &pv));
   const void* const* kh = p->khalf;
   const void* const* kf = p->kfull;
+  // every pass of the call's step k uses table copy b + k (mod ntab), the entry passes copy b:
+  // the plan's running index b makes a sequence of short calls cycle the dithered copies as
+  // one long call does
+  const int b = p->step_index;
+  p->step_index = int((long(b) + nsteps) % p->ntab);
   if (p->variant[WFS_K_YZ_SLAB] != 0) {
     // two launches per step; between them y holds the half-transformed Y[b][ka]
-    WFS_TRY(run_slab_lines(p, LINES_ENTRY, psi, nullptr, s, 0));
-    WFS_TRY(run_x_slab(p, psi, kh, s, 0));
+    WFS_TRY(run_slab_lines(p, LINES_ENTRY, psi, nullptr, s, b));
+    WFS_TRY(run_x_slab(p, psi, kh, s, b));
     for (int step = 1; step <= nsteps; ++step) {
-      WFS_TRY(run_slab_lines(p, LINES_FULL, psi, pv, s, step));
-      WFS_TRY(run_x_slab(p, psi, step == nsteps ? kh : kf, s, step));
+      WFS_TRY(run_slab_lines(p, LINES_FULL, psi, pv, s, b + step));
+      WFS_TRY(run_x_slab(p, psi, step == nsteps ? kh : kf, s, b + step));
     }
-    return run_slab_lines(p, LINES_EXIT, psi, nullptr, s, nsteps);
+    return run_slab_lines(p, LINES_EXIT, psi, nullptr, s, b + nsteps);
   }
-  // every pass of step k uses table copy k (mod ntab); the entry passes use copy 0
-  WFS_TRY(run_pass(p, 2, MODE_FWD, psi, nullptr, nullptr, 1.0, s, 0));
-  WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s, 0));
-  WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, kh, 1.0, s, 0));
+  WFS_TRY(run_pass(p, 2, MODE_FWD, psi, nullptr, nullptr, 1.0, s, b));
+  WFS_TRY(run_pass(p, 1, MODE_FWD, psi, nullptr, nullptr, 1.0, s, b));
+  WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, kh, 1.0, s, b));
   for (int step = 1; step <= nsteps; ++step) {
-    WFS_TRY(run_yz(p, psi, pv, s, step));
-    WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, step == nsteps ? kh : kf, 1.0, s, step));
+    WFS_TRY(run_yz(p, psi, pv, s, b + step));
+    WFS_TRY(run_pass(p, 0, MODE_FKI, psi, nullptr, step == nsteps ? kh : kf, 1.0, s, b + step));
   }
-  WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s, nsteps));
-  WFS_TRY(run_pass(p, 2, MODE_INV, psi, nullptr, nullptr, 1.0, s, nsteps));
+  WFS_TRY(run_pass(p, 1, MODE_INV, psi, nullptr, nullptr, 1.0, s, b + nsteps));
+  WFS_TRY(run_pass(p, 2, MODE_INV, psi, nullptr, nullptr, 1.0, s, b + nsteps));
   return 0;
 }
 
@@ -497,7 +512,7 @@ int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every,
 
 int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
   if (!V) { set_last_error("wfs_step: the power series needs V"); return WFS_ERR_INVALID; }
-  WFS_TRY(ensure_scratch(p));
+  WFS_TRY(ensure_scratch(p, s));
   const double inv_2m = 1.0 / (2.0 * p->mass);
   const double inv_dx2 = 1.0 / (p->dx[0] * p->dx[0]);   // x spacing on every axis (power_series_operator.cu:58)
   const size_t bytes = grid_points(p) * p->elem;
@@ -650,6 +665,17 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->obs_work), size_t(p->obs_work_len) * sizeof(double));
     if (e != hipSuccess) { set_last_error("hipMalloc for observable workspace failed"); rc = WFS_ERR_ALLOC; }
   }
+  if (rc == 0) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->plane_out), 7 * sizeof(double));
+    if (e != hipSuccess) { set_last_error("hipMalloc for the plane-current result failed"); rc = WFS_ERR_ALLOC; }
+  }
+  if (rc == 0 && slab_supported(precision, n)) {
+    // box-moment records of the slab scheme's observing launch: one per row group and box
+    const long need = long(batch) * n[0] * slab_rows(n[1]).b * kBoxesPerPass * 7;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->obs_lines), size_t(need) * sizeof(double));
+    if (e != hipSuccess) { set_last_error("hipMalloc for observing records failed"); rc = WFS_ERR_ALLOC; }
+    else p->obs_lines_len = need;
+  }
   if (rc) { wfs_plan_destroy(p); return rc; }
   *out = p;
   return 0;
@@ -731,19 +757,22 @@ int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every,
   WFS_TRY(observe(p, psi, boxes, nb, out, s));
   const void* pv = nullptr;
   WFS_TRY(current_table(p, s, &pv));
-  WFS_TRY(run_slab_lines(p, LINES_ENTRY, psi, nullptr, s, 0));
-  WFS_TRY(run_x_slab(p, psi, p->khalf, s, 0));
+  // table copies as in the alternating calls: the stretch that starts at step k uses b + k
+  const int b = p->step_index;
+  p->step_index = int((long(b) + nsteps) % p->ntab);
+  WFS_TRY(run_slab_lines(p, LINES_ENTRY, psi, nullptr, s, b));
+  WFS_TRY(run_x_slab(p, psi, p->khalf, s, b));
   for (int k = 0; k < nsteps; k += every) {
     const int m = std::min(every, nsteps - k);
     for (int step = 1; step <= m; ++step) {
-      WFS_TRY(run_slab_lines(p, LINES_FULL, psi, pv, s, step));
-      WFS_TRY(run_x_slab(p, psi, step == m ? p->khalf : p->kfull, s, step));
+      WFS_TRY(run_slab_lines(p, LINES_FULL, psi, pv, s, b + k + step));
+      WFS_TRY(run_x_slab(p, psi, step == m ? p->khalf : p->kfull, s, b + k + step));
     }
     if (k + m < nsteps) {
-      WFS_TRY(run_slab_observe(p, psi, m, boxes, nb, out + ((k + m) / every) * per_obs, s));
-      WFS_TRY(run_x_slab(p, psi, p->khalf, s, 0));
+      WFS_TRY(run_slab_observe(p, psi, b + k + m, b + k + m, boxes, nb, out + ((k + m) / every) * per_obs, s));
+      WFS_TRY(run_x_slab(p, psi, p->khalf, s, b + k + m));
     } else {
-      WFS_TRY(run_slab_lines(p, LINES_EXIT, psi, nullptr, s, m));
+      WFS_TRY(run_slab_lines(p, LINES_EXIT, psi, nullptr, s, b + k + m));
     }
   }
   return 0;
@@ -766,6 +795,17 @@ extern "C" int wfs_step_observe(wfs_plan* p, void* psi, const void* V, int nstep
   if (nsteps == 0) return 0;
   if (p->method == WFS_METHOD_POWER_SERIES && !V) { set_last_error("wfs_step_observe: the power series needs V"); return WFS_ERR_INVALID; }
   return steps_observed(p, psi, V, nsteps, observe_every, boxes, nboxes, out, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int wfs_plan_set_step_index(wfs_plan* p, long index) {
+  if (!p || index < 0) { set_last_error("wfs_plan_set_step_index: bad argument"); return WFS_ERR_INVALID; }
+  p->step_index = int(index % p->ntab);
+  return 0;
+}
+
+extern "C" long wfs_plan_get_step_index(wfs_plan* p) {
+  if (!p) return -1;
+  return p->step_index;
 }
 
 extern "C" int wfs_plan_set_profiling(wfs_plan* p, int enable) {
@@ -984,11 +1024,12 @@ extern "C" int wfs_plane_current(wfs_plan* p, const void* psi, const double cent
   hipStream_t s = static_cast<hipStream_t>(stream);
   const long need = plane_current_partials_needed(count[0], count[1]);
   if (need > p->plane_work_len) {
+    // grown on demand: the pixel count is known only here
+    WFS_TRY(refuse_if_capturing(s, "wfs_plane_current"));
     if (p->plane_work) { WFS_HIP_CHECK(hipStreamSynchronize(s)); (void)hipFree(p->plane_work); p->plane_work = nullptr; }
     WFS_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p->plane_work), size_t(need) * sizeof(double)));
     p->plane_work_len = need;
   }
-  if (!p->plane_out) WFS_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p->plane_out), 7 * sizeof(double)));
   ProfileScope scope(p, WFS_K_OBSERVABLES, s);
   WFS_TRY(launch_plane_current(p->precision, p->n[0], p->n[1], p->n[2], p->dx, psi, center, normal, size, count,
                                p->mass, density, p->plane_work, p->plane_work_len, p->plane_out, s));

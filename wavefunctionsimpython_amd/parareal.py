@@ -13,7 +13,9 @@ fine and coarse operators see it rounded to complex64.
 
 After k iterations the first k slices equal the serial fine solution exactly:
 U_1^1 = G(U_0) + F(U_0) - G(U_0) is F(U_0) after rounding to complex64, and
-so on by induction, because G is deterministic.
+so on by induction, because G is deterministic.  F and G must be the same maps
+in every call: their plans restart the dithered table-copy schedule at copy 0
+before each call (a plain run instead lets the copies run on from call to call).
 
 Coarse propagators:
   IdentityCoarse    G(psi) = psi
@@ -122,6 +124,7 @@ class SplitStepCoarse:
     def __call__(self, src: torch.Tensor, dst: torch.Tensor) -> None:
         if dst.data_ptr() != src.data_ptr():
             dst.copy_(src)
+        self.plan.set_step_index(0)        # the same map on every call
         self.plan.step(dst, None, self.steps)
 
 
@@ -237,6 +240,7 @@ class Parareal:
         psi = psi0.clone()
         states = [psi.clone()] if keep else None
         for _ in range(self.N):
+            self.serial.set_step_index(0)
             self.serial.step(psi, None, self.steps)
             if keep:
                 states.append(psi.clone())
@@ -286,6 +290,7 @@ class Parareal:
         k = 0
         while k < K:
             t = tick()
+            self.fine.set_step_index(0)                      # F is the same map in every iteration
             self.fine.step(self.Fin, None, self.steps)       # Fin[n] = F(U_n^k), all slices at once
             t2 = tick()
             t_fine += t2 - t
@@ -319,6 +324,7 @@ class Parareal:
             t = tick()
             self._narrow(self.U[k], state)
             for _ in range(k, N):
+                self.serial.set_step_index(0)
                 self.serial.step(state, None, self.steps)
             t_fine += tick() - t
         total = time.perf_counter() - wall0
