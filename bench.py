@@ -66,6 +66,26 @@ def build_workload(dev, n=N):
     return shape, dx, psi, V, boxes
 
 
+def accuracy(psi, ref):
+    """Relative L2 error and phase error arg<ref, psi> (radians) against a complex128 run."""
+    import torch
+    a = psi.to(torch.complex128).reshape(-1)
+    b = ref.to(torch.complex128).reshape(-1)
+    return {"rel_l2_vs_c128": float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)),
+            "phase_err_rad": float(torch.angle(torch.vdot(b, a)))}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_chunks(plan, psi, V, boxes, steps, out=None):
     """steps steps, the boxes measured before every OBS_EVERY-th (core_sim's measure-then-evolve
     order): one wfs_step_observe call.  Returns the observations (float64, device)."""
@@ -82,6 +102,7 @@ def gpu_bench(steps, warmup):
     plan.set_potential(V)          # phase table cexp(-i dt V), kept by the plan
     nobs = max((max(steps, warmup, 40) + OBS_EVERY - 1) // OBS_EVERY, 1)
     out = torch.zeros((nobs, len(boxes), 7), dtype=torch.float64, device=dev)
+    psi0 = psi.clone()
     run_chunks(plan, psi, None, boxes, warmup, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -90,6 +111,20 @@ def gpu_bench(steps, warmup):
     t1 = time.perf_counter()
     elapsed = t1 - t0
     norm = float(out[0, 0, 0].item())     # first observation of the timed run: the state after warm-up
+    # the same two calls in complex128 (the path pinned to the FP64 oracle): error of the result
+    p64 = _abi.Plan(shape, dx, DT, 1.0, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+    ref = psi0.to(torch.complex128)
+    del psi0
+    out64 = torch.zeros_like(out)
+    p64.set_potential(V)
+    run_chunks(p64, ref, None, boxes, warmup, out64)
+    run_chunks(p64, ref, None, boxes, steps, out64)
+    torch.cuda.synchronize()
+    acc = accuracy(psi, ref)
+    nt = (steps + OBS_EVERY - 1) // OBS_EVERY
+    acc["observables_max_abs_err"] = float((out[:nt] - out64[:nt]).abs().max())
+    del p64, ref, out64
+    torch.cuda.empty_cache()
 
     # instrumented window: HIP events around every launch, same stream, same workload
     plan.set_profiling(True)
@@ -134,7 +169,7 @@ def gpu_bench(steps, warmup):
         "horner_launch_GBs": round(32 * npts / (hms / hl * 1e-3) / 1e9, 1),
     }
     return {
-        "elapsed": elapsed, "npts": npts, "norm": norm, "kernels": kernels, "dominant": dominant,
+        "elapsed": elapsed, "npts": npts, "norm": norm, "kernels": kernels, "dominant": dominant, "accuracy": acc,
         "power_series": power_series, "V_host": V.cpu().numpy(),
         "variants": {k: v for k, v in variants.items() if k in prof},
     }
@@ -182,6 +217,7 @@ def cpu_baseline(V, seconds_target=15.0):
     el = time.perf_counter() - t0
     return {
         "value": n ** 3 * steps / el, "unit": "grid-pt-steps/s", "cores": cores, "kind": "port",
+        "cpu_model": cpu_model(),
         "sample": f"{steps} split-steps (4 scipy FFTs each, complex128) of the 256^3 double-slit workload "
                   f"after 1 warm-up step, scipy {scipy.__version__}",
     }
@@ -361,6 +397,100 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
     }
 
 
+def dropin_config(tmp, observe_every):
+    """The 256^3 double-slit workload as a reference-schema config (TOML keys of
+    sources/sim_state.py, potential.py, wave_packet.py, measurement.py): observation box +-32
+    Bohr (expected location), volume probabilities of the two x halves."""
+    import os as _os
+    return {
+        "simulation": {"method": "fourier", "delta_time_h_bar_per_hartree": DT, "total_iteration_count": 0,
+                       "double_precision_calculation": False, "observe_every": observe_every},
+        "wave_packet": {"particle_mass_electron_rest_mass": 1.0, "initial_position_bohr_radii_3": [-16.0, 0.0, 0.0],
+                        "initial_velocity_bohr_radii_hartree_per_h_bar_3": [2.0, 0.0, 0.0],
+                        "initial_standard_deviation_bohr_radii_3": [4.0, 4.0, 4.0]},
+        "volume": {"simulated_volume_dimensions_bohr_radii_3": [L, L, L], "number_of_voxels_3": [N, N, N],
+                   "observation_box_bottom_corner_bohr_radii_3": [-32.0, -32.0, -32.0],
+                   "observation_box_top_corner_bohr_radii_3": [32.0, 32.0, 32.0]},
+        "absorbing_boundary_condition": {"outer_potential_in_positive_xyz_direction_hartree_3": [-100.0] * 3,
+                                         "outer_potential_in_negative_xyz_direction_hartree_3": [-100.0] * 3,
+                                         "start_offset": 0.0, "slope_exponent_in_positive_xyz_direction_3": [3.0] * 3,
+                                         "slope_exponent_in_negative_xyz_direction_3": [3.0] * 3, "enable": True},
+        "potential": {"walls": [{"potential_hartree": 5.0, "center_bohr_radii_3": [0.0, 0.0, 0.0],
+                                 "normal_vector_3": [1.0, 0.0, 0.0], "plateau_thickness_bohr_radii": 2.0,
+                                 "slope_thickness_bohr_radii": 0.0, "slit_count": 2, "slit_spacing_bohr_radii": 6.0,
+                                 "slit_width_bohr_radii": 1.5}]},
+        "measurement": {"expected_location": {"enable_image": True},
+                        "volume_probabilities": [
+                            {"name": "x<0", "bottom_corner_bohr_radii_3": [-32.0, -32.0, -32.0],
+                             "top_corner_bohr_radii_3": [0.0, 32.0, 32.0]},
+                            {"name": "x>0", "bottom_corner_bohr_radii_3": [0.0, -32.0, -32.0],
+                             "top_corner_bohr_radii_3": [32.0, 32.0, 32.0]}]},
+        "paths": {"cache_dir": _os.path.join(tmp, "cache"), "output_dir": _os.path.join(tmp, "output")},
+    }
+
+
+def dropin_bench(steps, warmup):
+    """The drop-in surface at the reference's settings: initialisation.initialize + core_sim.run_iteration
+    (SimState, MeasurementTools; dynamic potential mode, the reference default) on the 256^3 double
+    slit, sampling every step and every 10 steps, against the reference's own loop shape
+    (core_sim.time_step per iteration: measure, evolve one step, update the potential)."""
+    import contextlib
+    import shutil
+    import tempfile
+    import torch
+    from wavefunctionsimpython_amd import core_sim, initialisation
+    tmp = tempfile.mkdtemp(prefix="wfs_dropin_")
+    npts = N ** 3
+    res = {}
+    try:
+        with contextlib.redirect_stdout(sys.stderr):
+            for every in (1, 10):
+                cfg = dropin_config(tmp, every)
+                sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
+                it.total_iteration_count = warmup
+                core_sim.run_iteration(sim, tools, it, observe_every=every, exit_on_quit=False)
+                torch.cuda.synchronize()
+                it.total_iteration_count = warmup + steps
+                t0 = time.perf_counter()
+                core_sim.run_iteration(sim, tools, it, observe_every=every, exit_on_quit=False)
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t0
+                samples = tools.get_expected_location().get_expected_location_evolution().shape[0]
+                res[f"observe_every_{every}"] = {"value": npts * steps / el, "ms_per_step": el / steps * 1e3,
+                                                 "samples": samples}
+                del sim, tools, it
+                torch.cuda.empty_cache()
+            # the reference's loop shape, one time_step per iteration
+            cfg = dropin_config(tmp, 1)
+            sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
+            loop_steps = max(20, steps // 10)
+            for i in range(warmup):
+                it.i = i
+                core_sim.time_step(sim, tools, it)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(warmup, warmup + loop_steps):
+                it.i = i
+                core_sim.time_step(sim, tools, it)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            res["reference_loop_time_step"] = {"value": npts * loop_steps / el, "ms_per_step": el / loop_steps * 1e3,
+                                               "steps": loop_steps}
+            del sim, tools, it
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {
+        "metric": "grid-pt-steps/s at 256^3 through the drop-in surface (core_sim.run_iteration, observables "
+                  "every step)", "value": res["observe_every_1"]["value"], "unit": "grid-pt-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": res["observe_every_1"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "256^3 double slit + CAP from a reference-schema config, SimState + MeasurementTools "
+                               "(expected location + 2 volume probabilities), dynamic potential mode",
+                   "grid": [N, N, N], "dt": DT},
+        "runs": res,
+    }
+
+
 def large_bench(steps):
     """512^3 complex64, V = 3 sum_d cos^2(pi r_d / 4) + a cubic CAP of 16 Bohr (box 128 Bohr,
     observation box +-48), packet sigma 5 at the origin with k0 = (1, 1, 0), dt 0.002.
@@ -441,10 +571,12 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal", "large"], default="splitstep256",
+    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal", "large", "dropin"],
+                    default="splitstep256",
                     help="splitstep256 (default, the headline metric), batch (48 packets, one hipGraph), "
-                         "parareal (time-to-T on the 128^3 lattice, FNO training included) or large (512^3, "
-                         "split-step and power series against complex128; --steps, 2000 in the config)")
+                         "parareal (time-to-T on the 128^3 lattice, FNO training included), large (512^3, "
+                         "split-step and power series against complex128; --steps, 2000 in the config) or dropin "
+                         "(the 256^3 double slit through initialisation + core_sim.run_iteration)")
     ap.add_argument("--train-pairs", type=int, default=2000, help="FNO training pairs (parareal workload)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 5 device copies of a 128 MiB buffer (HBM counter calibration) and exit")
@@ -475,6 +607,9 @@ def main():
     if args.workload == "large":
         print(json.dumps(large_bench(args.steps)), flush=True)
         return
+    if args.workload == "dropin":
+        print(json.dumps(dropin_bench(args.steps, args.warmup)), flush=True)
+        return
     g = gpu_bench(args.steps, args.warmup)
     value = g["npts"] * args.steps / g["elapsed"]
     dom = g["dominant"]
@@ -484,6 +619,19 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom), "kernel": dom,
                 "algorithmic_bytes_per_launch": k["algorithmic_bytes"]}
     cpu = None if args.no_cpu_baseline else cpu_baseline(g["V_host"])
+    ms = g["elapsed"] / args.steps * 1e3
+    # whole-step rates from the timed ms_per_step: the slab scheme's algorithmic 40 B per point and
+    # step plus 32 per sample (the fused observing launch and the split half kick's second x launch),
+    # and the survey's pass model (2 transforms x 3 axis passes x 16 B + 8 B of V = 104 B per step,
+    # + 96 B per sampled step)
+    step_bytes = 40.0 + 32.0 / OBS_EVERY
+    pass_bytes = 104.0 + 96.0 / OBS_EVERY
+    step_gbs = step_bytes * g["npts"] / (ms * 1e-3) / 1e9
+    pass_gbs = pass_bytes * g["npts"] / (ms * 1e-3) / 1e9
+    step_roofline = {"bytes_per_point_step": step_bytes, "achieved": round(step_gbs, 1),
+                     "frac": round(step_gbs / HBM_PEAK_GBS, 4), "pass_model_bytes_per_point_step": pass_bytes,
+                     "pass_model_equivalent": round(pass_gbs, 1), "pass_model_frac": round(pass_gbs / HBM_PEAK_GBS, 4),
+                     "unit": "GB/s"}
     line = {
         "metric": "grid-pt-steps/s at 256^3 (split-step Fourier, complex64)",
         "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -493,6 +641,8 @@ def main():
                    "grid": [N, N, N], "box_bohr": L, "dt": DT, "observe_every": OBS_EVERY,
                    "parallelism": "single GPU"},
         "roofline": roofline,
+        "step_roofline": step_roofline,
+        "accuracy": g["accuracy"],
         "cpu_baseline": cpu,
         "kernels": g["kernels"],
         "power_series": g["power_series"],

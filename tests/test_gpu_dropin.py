@@ -39,7 +39,7 @@ def config1(tmp):
 
 def test_config1_free_packet_matches_analytic_spread(dev, tmp_path):
     cfg = config1(str(tmp_path))
-    sim, tools, it = initialisation.initialize(cfg, use_cache=False)
+    sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
     assert sim.get_simulation_method() == SimulationMethod.FOURIER
     core_sim.run_iteration(sim, tools, it, observe_every=10)
     el = tools.get_expected_location()
@@ -68,7 +68,7 @@ def test_config1_free_packet_matches_analytic_spread(dev, tmp_path):
 def test_config1_state_matches_oracle(dev, tmp_path):
     cfg = config1(str(tmp_path))
     cfg["simulation"]["total_iteration_count"] = 30
-    sim, tools, it = initialisation.initialize(cfg, use_cache=False)
+    sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
     psi0 = sim.get_wave_function().cpu().numpy()
     core_sim.run_iteration(sim, tools, it, observe_every=1)
     ref = oracle.split_step(psi0, np.zeros(psi0.shape, np.complex64), sim.get_delta_x_bohr_radii_3(), 0.01, 1.0,
@@ -109,7 +109,7 @@ def double_slit_config(tmp, n=64, method="fourier"):
 @pytest.mark.parametrize("method", ["fourier", "power_series"])
 def test_double_slit_state_matches_oracle(dev, tmp_path, method):
     cfg = double_slit_config(str(tmp_path), 64, method)
-    sim, tools, it = initialisation.initialize(cfg, use_cache=False)
+    sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
     psi0 = sim.get_wave_function().cpu().numpy()
     V = sim.get_potential().cpu().numpy()
     dx = sim.get_delta_x_bohr_radii_3()
@@ -128,11 +128,37 @@ def test_double_slit_state_matches_oracle(dev, tmp_path, method):
     assert oracle.relative_l2(sim.get_wave_function().cpu().numpy(), ref) < 1e-5
 
 
+@pytest.mark.parametrize("n", [64, 128])
+def test_run_iteration_stretches_match_the_reference_loop(dev, tmp_path, n):
+    """run_iteration hands stretches of iterations to one wfs_step_observe call (at 128^3 the
+    slab scheme fuses every sample into one launch); the reference's loop calls time_step --
+    measure, evolve one step, update the potential -- once per iteration
+    (sources/core_sim.py:19-24, 42-55).  Same final state bit for bit, same series."""
+    a_cfg = double_slit_config(str(tmp_path / "a"), n)
+    b_cfg = double_slit_config(str(tmp_path / "b"), n)
+    sa, ta, ia = initialisation.initialize(use_cache=False, config=a_cfg)
+    sb, tb, ib = initialisation.initialize(use_cache=False, config=b_cfg)
+    if n == 128:
+        assert sa.get_plan().variants()["yz_slab"] != 0
+    core_sim.run_iteration(sa, ta, ia, observe_every=1)
+    for i in range(ib.total_iteration_count):
+        ib.i = i
+        core_sim.time_step(sb, tb, ib)
+    torch.cuda.synchronize()
+    assert torch.equal(sa.get_wave_function(), sb.get_wave_function())
+    ea = ta.get_expected_location().get_expected_location_evolution()
+    eb = tb.get_expected_location().get_expected_location_evolution()
+    sda = ta.get_expected_location().get_standard_deviation_evolution()
+    sdb = tb.get_expected_location().get_standard_deviation_evolution()
+    assert ea.shape == eb.shape == (40, 3)
+    assert np.allclose(ea, eb, rtol=1e-12, atol=1e-13) and np.allclose(sda, sdb, rtol=1e-12, atol=1e-13)
+
+
 def test_observe_every_matches_single_steps(dev, tmp_path):
     a_cfg = double_slit_config(str(tmp_path / "a"))
     b_cfg = double_slit_config(str(tmp_path / "b"))
-    sa, ta, ia = initialisation.initialize(a_cfg, use_cache=False)
-    sb, tb, ib = initialisation.initialize(b_cfg, use_cache=False)
+    sa, ta, ia = initialisation.initialize(use_cache=False, config=a_cfg)
+    sb, tb, ib = initialisation.initialize(use_cache=False, config=b_cfg)
     core_sim.run_iteration(sa, ta, ia, observe_every=1)
     core_sim.run_iteration(sb, tb, ib, observe_every=10)
     ea = ta.get_expected_location().get_expected_location_evolution()
@@ -194,13 +220,13 @@ def test_operators_tables(dev):
 
 def test_snapshot_roundtrip_resumes_psi(dev, tmp_path):
     cfg = double_slit_config(str(tmp_path))
-    sim, tools, it = initialisation.initialize(cfg, use_cache=False)
+    sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
     sim.evolve_state(5)
     it.i = 5
     from wavefunctionsimpython_amd import snapshot_io
     snapshot_io.write_snapshot(sim, it)
     psi5 = sim.get_wave_function().cpu().numpy()
-    sim2, tools2, it2 = initialisation.initialize(copy.deepcopy(cfg), use_cache=True)
+    sim2, tools2, it2 = initialisation.initialize(use_cache=True, config=copy.deepcopy(cfg))
     assert it2.i == 5
     assert np.array_equal(sim2.get_wave_function().cpu().numpy(), psi5)
 
@@ -235,7 +261,7 @@ def test_plane_current_measurement_obeys_continuity(dev, tmp_path):
                                                          "normal_vector_3": [1.0, 0.0, 0.0],
                                                          "size_bohr_radii_2": [19.375, 19.375],
                                                          "resolution_2": [62, 62]}]
-    sim, tools, it = initialisation.initialize(cfg, use_cache=False)
+    sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
     core_sim.run_iteration(sim, tools, it, observe_every=2)
     right, _ = tools.get_volume_probabilities()[0].get_probability_evolution_with_name()
     flux, _ = tools.get_plane_probability_currents()[0].get_probability_current_evolution_with_name()

@@ -10,6 +10,7 @@ import sys
 
 import numpy as np
 import pytest
+import tomli
 
 import oracle
 from wavefunctionsimpython_amd import _abi, math_utils
@@ -141,3 +142,21 @@ name = "First half"
     cfg = initialisation.load_config(str(p))
     assert try_read_param(cfg, "simulation.total_iteration_count", 0) == 541
     assert cfg["potential"]["walls"][0]["slit_count"] == 2
+    # the cache copy written by initialize reads back as the same dict
+    assert tomli.loads(initialisation.toml_dumps(cfg)) == cfg
+
+
+def test_config_file_pick(tmp_path, monkeypatch):
+    """initialize(use_cache) without a config reads the one TOML file under config/, or asks for
+    an index when there are several (sources/initialisation.py:16-40)."""
+    from wavefunctionsimpython_amd import initialisation
+    d = tmp_path / "config"
+    d.mkdir()
+    (d / "only.toml").write_text("[simulation]\ndelta_time_h_bar_per_hartree = 0.1\n")
+    assert initialisation.select_config_file(str(d)).endswith("only.toml")
+    (d / "another.toml").write_text("[simulation]\ndelta_time_h_bar_per_hartree = 0.2\n")
+    answers = iter(["x", "7", "1"])
+    monkeypatch.setattr("builtins.input", lambda: next(answers))
+    assert initialisation.select_config_file(str(d)).endswith("only.toml")     # sorted: another, only
+    with pytest.raises(SystemExit):
+        initialisation.select_config_file(str(tmp_path / "missing"))

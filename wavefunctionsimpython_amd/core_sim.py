@@ -3,9 +3,12 @@
 ``time_step`` keeps the reference order -- measure the current state, advance
 one step, update the potential.  ``run_iteration`` additionally honours
 ``simulation.observe_every`` (an extension key, default 1 = reference
-behaviour): between two measured iterations it advances several steps in one
-``wfs_step`` call, which merges their split-step half kicks, whenever the
-potential cannot change in between.
+behaviour).  Whenever the potential cannot change and no sample needs psi
+itself (no wave-function files, no plane currents), it hands a stretch of up
+to ``STRETCH`` iterations to one ``wfs_step_observe`` call: the box moments of
+every sample come back from the GPU in one copy per stretch, and the split-step
+fuses each sample's exit, moments and re-entry into one launch.  The samples,
+the final state and the series are those of the iteration-by-iteration loop.
 """
 
 from __future__ import annotations
@@ -18,6 +21,9 @@ from . import snapshot_io
 from .iter_data import IterData
 from .measurement import MeasurementTools
 from .sim_state import SimState
+
+# iterations handed to one wfs_step_observe call (SIGINT is answered between stretches)
+STRETCH = 1000
 
 _GREEN = "\033[32m"
 _RED = "\033[31m"
@@ -60,17 +66,25 @@ def run_iteration(sim_state: SimState, measurement_tools: MeasurementTools, iter
         while i < total:
             t0 = time.time()
             iter_data.i = i
-            if i % observe_every == 0:
-                measurement_tools.measure_and_render(sim_state, iter_data)
-            # steps until the next measured iteration
-            span = min(observe_every - i % observe_every, total - i)
-            if span > 1 and sim_state.potential_is_static() and not sim_state.is_wave_function_saving():
-                sim_state.evolve_state(span)
-                sim_state.update_potential()
+            if (i % observe_every == 0 and sim_state.potential_is_static()
+                    and not sim_state.is_wave_function_saving() and not measurement_tools.needs_state_per_sample()):
+                # a stretch of whole sampling intervals in one call
+                span = min(max(STRETCH - STRETCH % observe_every, observe_every), total - i)
+                boxes, owners = measurement_tools.observation_boxes()
+                if boxes:
+                    measurement_tools.record_samples(sim_state.evolve_state_observed(span, observe_every, boxes),
+                                                     owners)
+                else:
+                    sim_state.evolve_state(span)
             else:
-                span = 1
-                sim_state.evolve_state()
-                sim_state.update_potential()
+                if i % observe_every == 0:
+                    measurement_tools.measure_and_render(sim_state, iter_data)
+                # steps until the next measured iteration
+                span = min(observe_every - i % observe_every, total - i)
+                if not (span > 1 and sim_state.potential_is_static() and not sim_state.is_wave_function_saving()):
+                    span = 1
+                sim_state.evolve_state(span)
+            sim_state.update_potential()
             i += span
             iter_data.elapsed_system_time_s += time.time() - t0
             if iter_data.is_quit:

@@ -2,7 +2,9 @@
 
 All box integrals of one measurement go to the GPU in a single
 ``wfs_observables`` call (Simpson weights, FP64 accumulation) and come back in
-one device-to-host copy.  Visual outputs of the reference (vispy volume
+one device-to-host copy; the driver can also hand a whole stretch of
+iterations to one ``wfs_step_observe`` call and append its samples with
+``record_samples`` (one copy per stretch).  Visual outputs of the reference (vispy volume
 rendering, matplotlib plots, mp4) are presentation and are not part of this
 package; their configuration keys are accepted and ignored.
 """
@@ -232,10 +234,9 @@ class MeasurementTools:
         wf.view(dtype=type_str).astype(type_str).tofile(
             os.path.join(out, f"wave_function_raw/wave_function_{iter_data.i:06d}.bin"))
 
-    def measure_and_render(self, sim_state: SimState, iter_data: IterData):
-        """Save, then calculate all observables, then synchronise (sources/measurement.py:644-704)."""
-        if sim_state.is_wave_function_saving():
-            self.write_wave_function_to_file(sim_state, iter_data)
+    def observation_boxes(self):
+        """(boxes, owners): the Simpson boxes of every enabled volume probability and of the
+        expected location, in the order their rows come back from the library."""
         boxes = []
         owners = []
         for v in self.__volume_probabilities:
@@ -245,18 +246,42 @@ class MeasurementTools:
         if self.__expected_location is not None:
             boxes.append(self.__expected_location.get_box())
             owners.append(self.__expected_location)
+        return boxes, owners
+
+    def needs_state_per_sample(self) -> bool:
+        """True when a sample needs psi itself (enabled plane currents), not only box moments."""
+        return any(pc.is_enable_image() for pc in self.__plane_probability_currents)
+
+    @staticmethod
+    def _hand_out(owners, rows):
+        for owner, row in zip(owners, rows):
+            if isinstance(owner, VolumeProbability):
+                owner.set_pending(row[0])
+            else:
+                owner.set_pending(row)
+
+    def record_samples(self, samples: torch.Tensor, owners) -> None:
+        """Append the samples of a wfs_step_observe call -- (n_samples, n_boxes, 7) box moments,
+        sample j taken before step j * observe_every -- to every series, in the order
+        measure_and_render would have appended them one by one (one device-to-host copy)."""
+        host = samples.cpu().numpy()
+        for rows in host:
+            self._hand_out(owners, rows)
+            for owner in owners:
+                owner.synchronize()
+
+    def measure_and_render(self, sim_state: SimState, iter_data: IterData):
+        """Save, then calculate all observables, then synchronise (sources/measurement.py:644-704)."""
+        if sim_state.is_wave_function_saving():
+            self.write_wave_function_to_file(sim_state, iter_data)
+        boxes, owners = self.observation_boxes()
         if boxes:
             plan = sim_state.get_plan()
             if self.__out is None or self.__out.shape[0] != len(boxes):
                 self.__out = torch.empty((len(boxes), 7), dtype=torch.float64,
                                          device=sim_state.get_wave_function().device)
             plan.observables(sim_state.get_wave_function(), boxes, out=self.__out)
-            host = self.__out.cpu().numpy()
-            for owner, row in zip(owners, host):
-                if isinstance(owner, VolumeProbability):
-                    owner.set_pending(row[0])
-                else:
-                    owner.set_pending(row)
+            self._hand_out(owners, self.__out.cpu().numpy())
         for plane in self.__plane_probability_currents:
             if plane.is_enable_image():
                 plane.calculate(sim_state)

@@ -354,14 +354,24 @@ class SimState:
                 return False
         return True
 
-    def evolve_state(self, steps: int = 1):
-        """Advance psi by ``steps`` time steps in place (sources/sim_state.py:754-762)."""
+    def _refresh_potential(self):
         if self.__plan is None:
             raise RuntimeError("initialize_state() must be called before evolve_state()")
         if self.__potential_dirty:
             self.__plan.set_potential(self.__localised_potential_hartree)
             self.__potential_dirty = False
+
+    def evolve_state(self, steps: int = 1):
+        """Advance psi by ``steps`` time steps in place (sources/sim_state.py:754-762)."""
+        self._refresh_potential()
         self.__plan.step(self.__wave_tensor, None, int(steps))
+
+    def evolve_state_observed(self, steps: int, observe_every: int, boxes, out: torch.Tensor | None = None):
+        """``steps`` time steps with the box moments of the state before steps 0, observe_every, ...
+        (< steps) -- the measure-then-evolve order of core_sim, sources/core_sim.py:19-24 -- in one
+        wfs_step_observe call.  Returns the device float64 samples (n_samples, n_boxes, 7)."""
+        self._refresh_potential()
+        return self.__plan.step_observe(self.__wave_tensor, None, int(steps), int(observe_every), boxes, out=out)
 
     # ------------------------------------------------------------------ text
     def get_sim_state_description_text(self, use_colors=False):
