@@ -149,6 +149,21 @@ int wfs_observables(wfs_plan* plan, const void* psi, const wfs_box* boxes, int n
                     void* stream);
 
 /*
+ * wfs_observables without a plan and without allocating (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same results
+ * and out layout as wfs_observables on a plan of this grid (out[packet][box][7],
+ * batch packets of n[0] x n[1] x n[2] stored back to back).  The partial records
+ * go to a static 2 MiB device workspace per GPU (262144 doubles): every pass of
+ * up to 4 boxes needs ceil(rows / 32) * 7 * boxes * batch records, rows = the x-y
+ * extent of the boxes' bounding box; larger requests return WFS_ERR_UNSUPPORTED.
+ * The first call on a device looks the workspace up (hipGetSymbolAddress); no
+ * call allocates.  Calls that overlap on one GPU share the workspace and race.
+ */
+int wfs_observables_noalloc(const int n[3], const double dx[3], int precision, int batch, const void* psi,
+                            const wfs_box* boxes, int nboxes, double* out, void* stream);
+
+/*
  * nsteps steps with the box observables of the state before steps 0,
  * observe_every, 2 observe_every, ... (< nsteps) written to
  * out[o][packet][box][7] (device; o < ceil(nsteps / observe_every)): the

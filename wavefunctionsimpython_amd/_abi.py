@@ -67,6 +67,8 @@ SIGNATURES = {
     "wfs_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "wfs_fft3d": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "wfs_observables": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(WfsBox), ctypes.c_int, _vp, _vp]),
+    "wfs_observables_noalloc": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_int, ctypes.c_int, _vp,
+                                               ctypes.POINTER(WfsBox), ctypes.c_int, _vp, _vp]),
     "wfs_step_observe": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(WfsBox),
                                         ctypes.c_int, _vp, _vp]),
     "wfs_plane_current": (ctypes.c_int, [_vp, _vp, _c_dbl3, _c_dbl3, ctypes.c_double * 2, ctypes.c_int * 2, _vp,
@@ -528,6 +530,27 @@ def init_gaussian(n, dx, precision: int, sigma, r0, k0, psi: torch.Tensor, strea
     check_state(psi, n, precision)
     check(lib().wfs_init_gaussian(i3(n), d3(dx), int(precision), d3(sigma), d3(r0), d3(k0), ptr(psi),
                                   stream_handle(stream)), "wfs_init_gaussian")
+
+
+def observables_noalloc(n, dx, precision: int, psi: torch.Tensor, boxes, out: torch.Tensor, batch: int = 1,
+                        stream=None) -> torch.Tensor:
+    """Box moments without a plan and without allocation (``wfs_observables_noalloc``): out is a
+    caller-owned contiguous float64 device tensor of batch * len(boxes) * 7 entries."""
+    n = tuple(int(v) for v in n)
+    shape = n if batch == 1 else (int(batch),) + n
+    check_state(psi, shape, precision)
+    nb = len(boxes)
+    arr = (WfsBox * max(nb, 1))()
+    for i, (bottom, count) in enumerate(boxes):
+        for a in range(3):
+            arr[i].bottom[a] = int(bottom[a])
+            arr[i].count[a] = int(count[a])
+    if (out.dtype != torch.float64 or not out.is_contiguous() or out.device != psi.device
+            or out.numel() < int(batch) * nb * 7):
+        raise ValueError("out must be a contiguous float64 device tensor with 7 entries per box and packet")
+    check(lib().wfs_observables_noalloc(i3(n), d3(dx), int(precision), int(batch), ptr(psi), arr, nb, ptr(out),
+                                        stream_handle(stream)), "wfs_observables_noalloc")
+    return out
 
 
 def add_cap(n, dx, bottom, top, offset, pot_pos, pot_neg, exp_pos, exp_neg, V: torch.Tensor, stream=None) -> None:

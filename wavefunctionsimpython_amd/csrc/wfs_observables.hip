@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace wfs {
 
@@ -387,3 +388,101 @@ int launch_plane_current(int precision, int nx, int ny, int nz, const double dx[
 }
 
 }  // namespace wfs
+
+// ---------------------------------------------------------------------------
+// Box moments without a plan and without any allocation (wfs_observables_noalloc)
+// ---------------------------------------------------------------------------
+// The partial records live in this file-scope device array: each GPU the library runs on holds
+// its own copy until the process ends.  Host code never hands the array's name to a kernel (in
+// host code it is only a stand-in, not a device address); it passes the address that
+// hipGetSymbolAddress returns on the current device, looked up once per device.
+// Every record the summing kernel reads is written by the moments kernel of the same call, so
+// nothing in the array has to start at zero.  Calls that overlap on one GPU share the array and
+// race.
+namespace wfs {
+namespace {
+
+constexpr long kScratchDoubles = 262144;   // 2 MiB of partial records
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(double** out) {
+  int dev = 0;
+  WFS_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_last_error("wfs_observables_noalloc: device %d beyond the %d the library tracks", dev, kMaxDevices);
+    return WFS_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_last_error("wfs_observables_noalloc: hipGetSymbolAddress of the static workspace failed: %s",
+                     hipGetErrorString(e));
+      return e != hipSuccess ? int(e) : WFS_ERR_INVALID;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+}  // namespace wfs
+
+using namespace wfs;
+
+extern "C" int wfs_observables_noalloc(const int n[3], const double dx[3], int precision, int batch, const void* psi,
+                                       const wfs_box* boxes, int nboxes, double* out, void* stream) {
+  if (!n || !dx || !psi || nboxes < 0 || batch < 1 || (nboxes > 0 && (!boxes || !out)) ||
+      (precision != WFS_C64 && precision != WFS_C128)) {
+    set_last_error("wfs_observables_noalloc: bad argument");
+    return WFS_ERR_INVALID;
+  }
+  for (int a = 0; a < 3; ++a)
+    if (n[a] < 1 || !(dx[a] > 0.0)) { set_last_error("wfs_observables_noalloc: bad grid"); return WFS_ERR_INVALID; }
+  for (int b = 0; b < nboxes; ++b)
+    for (int a = 0; a < 3; ++a)
+      if (boxes[b].bottom[a] < 0 || boxes[b].count[a] < 1 || long(boxes[b].bottom[a]) + boxes[b].count[a] > n[a]) {
+        set_last_error("wfs_observables_noalloc: box %d axis %d [%d, %d) outside the grid of %d voxels", b, a,
+                       boxes[b].bottom[a], boxes[b].bottom[a] + boxes[b].count[a], n[a]);
+        return WFS_ERR_INVALID;
+      }
+  // every pass of up to kBoxesPerPass boxes must fit its records into the static workspace
+  for (int b0 = 0; b0 < nboxes; b0 += kBoxesPerPass) {
+    const int nb = std::min(kBoxesPerPass, nboxes - b0);
+    long lo[2], hi[2];
+    for (int a = 0; a < 2; ++a) {
+      lo[a] = boxes[b0].bottom[a];
+      hi[a] = long(boxes[b0].bottom[a]) + boxes[b0].count[a];
+      for (int b = b0 + 1; b < b0 + nb; ++b) {
+        lo[a] = std::min(lo[a], long(boxes[b].bottom[a]));
+        hi[a] = std::max(hi[a], long(boxes[b].bottom[a]) + boxes[b].count[a]);
+      }
+    }
+    const long rows = (hi[0] - lo[0]) * (hi[1] - lo[1]);
+    const long need = (rows + kRowsPerBlock - 1) / kRowsPerBlock * kObsQ * nb * long(batch);
+    if (need > kScratchDoubles) {
+      set_last_error("wfs_observables_noalloc: boxes %d..%d of %d packets need %ld partial records; the static "
+                     "workspace holds %ld (2 MiB): use wfs_observables with a plan", b0, b0 + nb - 1, batch, need,
+                     kScratchDoubles);
+      return WFS_ERR_UNSUPPORTED;
+    }
+  }
+  if (nboxes == 0) return 0;
+  double* work = nullptr;
+  const int rc = scratch_address(&work);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int b0 = 0; b0 < nboxes; b0 += kBoxesPerPass) {
+    const int nb = std::min(kBoxesPerPass, nboxes - b0);
+    const int r = launch_box_moments(precision, batch, n[0], n[1], n[2], dx, psi, boxes + b0, nb, work,
+                                     kScratchDoubles, out + 7 * b0, 7L * nboxes, s);
+    if (r) return r;
+  }
+  return 0;
+}
