@@ -122,7 +122,9 @@ def gpu_bench(steps, warmup):
     torch.cuda.synchronize()
     acc = accuracy(psi, ref)
     nt = (steps + OBS_EVERY - 1) // OBS_EVERY
-    acc["observables_max_abs_err"] = float((out[:nt] - out64[:nt]).abs().max())
+    # per moment (P, E[r], E[r^2]) the worst sample difference relative to the moment's largest value
+    scale = out64[:nt].abs().amax(dim=(0, 1)).clamp_min(1e-300)
+    acc["observables_max_rel_err"] = float(((out[:nt] - out64[:nt]).abs().amax(dim=(0, 1)) / scale).max())
     del p64, ref, out64
     torch.cuda.empty_cache()
 
