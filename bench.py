@@ -38,8 +38,9 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_POINT = {      # algorithmic HBM bytes per grid point per launch, complex64 state
     "z_fwd": 16, "z_inv": 16, "z_inv_v_fwd": 24, "y_fwd": 16, "y_inv": 16,
     "x_fwd_k_inv": 16, "x_fwd": 16, "x_inv": 16, "horner_stage": 32,
-    # slab scheme (ny = nz = 256): y-z launch with the potential table, x launch, entry/exit
-    "yz_slab": 24, "x_slab": 16, "slab_edge": 16,
+    # slab scheme (ny = nz = 256): y-z launch with the potential table, x launch, entry/exit,
+    # observing launches (entry or exit + entry with the box moments of the position-space rows)
+    "yz_slab": 24, "x_slab": 16, "slab_edge": 16, "slab_observe": 16,
 }
 
 

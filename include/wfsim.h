@@ -264,7 +264,8 @@ int wfs_potential_operator(long count, const void* V, double dt, int precision, 
 #define WFS_K_YZ_SLAB 11
 #define WFS_K_X_SLAB 12
 #define WFS_K_SLAB_EDGE 13
-#define WFS_K_COUNT 14
+#define WFS_K_SLAB_OBSERVE 14
+#define WFS_K_COUNT 15
 int wfs_plan_set_profiling(wfs_plan* plan, int enable);
 
 /*
@@ -285,7 +286,8 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * ny = nz = 128 variant 2 below 256 x planes) run a step as two launches, the
  * y transform split in a length-ny/B stage (B = 4; 2 at 128) fused with the z
  * work (kernel class WFS_K_YZ_SLAB; the entry and exit launches of a call
- * count as WFS_K_SLAB_EDGE) and a length-B stage fused with the x pass
+ * count as WFS_K_SLAB_EDGE, the launches of wfs_step_observe that also take
+ * the box moments as WFS_K_SLAB_OBSERVE) and a length-B stage fused with the x pass
  * (WFS_K_X_SLAB); variant 2 runs the y-z launch as one persistent block per
  * CU that loads the next row group during the current one's arithmetic;
  * variant 0 runs the four axis passes.  The results agree to float rounding
@@ -359,6 +361,16 @@ int wfs_fno_layer(const float* W, const float* v, const float* Z, const float* b
                   const float* Fz, float* u, float* vn, float* T, int cin, int C, int n, int tm, int act,
                   void* stream);
 int wfs_fno_gelu_backward(const float* u, const float* dv, float* du, long n, void* stream);
+/*
+ * wfs_spectral_resample (no reference counterpart): the spectrum of an ndst grid from that of
+ * an nsrc grid, complex64, batch grids stacked: dst[k] = scale * src[k] for every signed
+ * frequency k (numpy fftfreq order) both grids hold, 0 for those only dst holds.  Between
+ * ortho wfs_fft3d transforms, scale = (|ndst| / |nsrc|)^(1/2) (point counts) keeps the values
+ * of a band-limited function: the coarse-grid coarse propagator of parareal runs its network
+ * on such a resampled state.  src and dst must not overlap.
+ */
+int wfs_spectral_resample(const void* src, const int nsrc[3], void* dst, const int ndst[3], long batch, double scale,
+                          void* stream);
 int wfs_fno_axpby(float* y, const float* x, float alpha, float beta, long n, void* stream);
 int wfs_fno_pack(const void* psi, int precision, const void* V, float pscale, float vscale, float* a, long P,
                  void* stream);

@@ -116,3 +116,83 @@ def test_trained_fno_gains_over_coarse_without_it(dev):
     assert runs["fno"].converged and runs["fno"].iterations < runs["identity"].iterations
     assert runs["fno"].iterations <= runs["base"].iterations + 1
     assert _rel(runs["fno"].state, serial[-1]) < 1e-4
+
+
+def _band_limited(n, L, modes, seed):
+    """sum of a few plane waves with frequencies below 4 per axis, sampled on an n^3 grid of side L"""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * (L / n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.zeros((n, n, n), np.complex128)
+    for _ in range(modes):
+        k = rng.integers(-3, 4, 3) * (2 * np.pi / L)
+        f += (rng.standard_normal() + 1j * rng.standard_normal()) * np.exp(1j * (k[0] * X + k[1] * Y + k[2] * Z))
+    return f
+
+
+def test_grid_resampler_is_exact_for_band_limited_states(dev):
+    """down (64^3 -> 16^3) and up (16^3 -> 64^3) keep the values of a function band-limited to
+    the coarse grid: the coarse samples are every fourth fine sample, up(down(f)) = f."""
+    L = 10.0
+    f = _band_limited(64, L, 6, 5)
+    rs = pr.GridResampler(64, 16)
+    fine = torch.from_numpy(f.astype(np.complex64)).to(dev)
+    coarse = torch.empty((16, 16, 16), dtype=torch.complex64, device=dev)
+    rs.down(fine, coarse)
+    want = f[::4, ::4, ::4]
+    assert np.max(np.abs(coarse.cpu().numpy() - want)) < 1e-5 * np.max(np.abs(want))
+    back = torch.zeros_like(fine)
+    rs.up_add(coarse.clone(), back)
+    assert np.max(np.abs(back.cpu().numpy() - f)) < 1e-5 * np.max(np.abs(f))
+    # a mode above the coarse band is dropped going down
+    x = np.arange(64) * (L / 64)
+    high = np.exp(1j * (2 * np.pi / L) * 12 * x)[:, None, None] * np.ones((1, 64, 64))
+    rs.down(torch.from_numpy(high.astype(np.complex64)).to(dev), coarse)
+    assert float(coarse.abs().max()) < 1e-5
+
+
+def test_coarse_grid_fno_with_zero_correction_is_its_base(dev):
+    """a network whose output layer is zero predicts no correction: G = B bit for bit."""
+    prob, V, psi0, _ = _setup(2)
+    base = pr.SplitStepCoarse(prob.shape, (DX,) * 3, STEPS * prob.dt, 1.0, V, 1)
+    net = FNO3d(16, width=8, modes=4, layers=2, in_channels=5, output_scale=0.0, residual_scale=1e-2)
+    g = pr.CoarseGridFNOCoarse(net, V, math.sqrt((N_GRID * DX) ** 3), 0.2, base)
+    a = torch.empty_like(psi0)
+    b = torch.empty_like(psi0)
+    g(psi0, a)
+    base(psi0, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def test_trained_coarse_grid_fno_lowers_its_base_error(dev):
+    """The network on a 16^3 grid (the fine grid is 32^3), trained on solver pairs, lowers the
+    slice error of its one-step split-step base on a held-out problem."""
+    S, N = 200, 6
+    rng = np.random.default_rng(1)
+    probs = [pr.random_lattice_problem(rng, n=N_GRID, dx=DX, k0=0.5, r0=2.0) for _ in range(16)]
+    pairs = pr.TrajectoryPairs(probs, S, base_steps=1)
+    pscale, vscale = math.sqrt((N_GRID * DX) ** 3), 0.2
+    net = FNO3d(16, width=16, modes=8, layers=4, in_channels=5, output_scale=0.0, residual_scale=1e-2)
+    V0 = pairs.V[0]
+    feat = pr.CoarseGridFNOCoarse(net, V0, pscale, vscale, pr.SplitStepCoarse(
+        probs[0].shape, (DX,) * 3, S * probs[0].dt, 1.0, V0, 1))
+    losses = pr.train_coarse(net, pairs, 8, pscale, vscale, lr=1e-3, lr_final=1e-4, epochs=24, mix=0.5,
+                             featurizer=feat)
+    k = len(losses) // 10
+    first, last = float(np.mean(losses[:k])), float(np.mean(losses[-k:]))
+    prob, V, psi0, par = _setup(N, steps=S)
+    _, serial = par.serial_fine(psi0, keep=True)
+    base = pr.SplitStepCoarse(prob.shape, (DX,) * 3, S * prob.dt, 1.0, V, 1)
+    g = pr.CoarseGridFNOCoarse(net, V, pscale, vscale, base)
+    out = torch.empty_like(psi0)
+    e_base, e_net = [], []
+    for j in range(N):
+        base(serial[j], out)
+        e_base.append(_rel(out, serial[j + 1]))
+        g(serial[j], out)
+        e_net.append(_rel(out, serial[j + 1]))
+    print(f"coarse-grid FNO: train loss {first:.3f} -> {last:.3f}; slice error base {np.mean(e_base):.2e} "
+          f"corrected {np.mean(e_net):.2e}")
+    assert last < 0.8 * first
+    assert np.mean(e_net) < 0.9 * np.mean(e_base)

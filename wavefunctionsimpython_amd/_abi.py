@@ -12,6 +12,7 @@ crosses the boundary.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import threading
 
@@ -59,6 +60,7 @@ SIGNATURES = {
                                     ctypes.c_float, ctypes.c_float, ctypes.c_int, _vp]),
     "wfs_parareal_update": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_long, _vp, ctypes.c_long, _vp, _vp]),
     "wfs_narrow_c128": (ctypes.c_int, [_vp, _vp, ctypes.c_long, _vp]),
+    "wfs_spectral_resample": (ctypes.c_int, [_vp, _c_int3, _vp, _c_int3, ctypes.c_long, ctypes.c_double, _vp]),
     "wfs_graph_begin": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "wfs_graph_end": (ctypes.c_int, [_vp, _vp]),
     "wfs_graph_launch": (ctypes.c_int, [_vp, _vp]),
@@ -98,7 +100,7 @@ SIGNATURES = {
 
 # kernel classes of wfs_plan_profile
 KERNEL_CLASSES = ("z_fwd", "z_inv", "z_inv_v_fwd", "y_fwd", "y_inv", "x_fwd_k_inv", "x_fwd", "x_inv",
-                  "horner_stage", "observables", "yz_plane", "yz_slab", "x_slab", "slab_edge")
+                  "horner_stage", "observables", "yz_plane", "yz_slab", "x_slab", "slab_edge", "slab_observe")
 
 
 class WfsError(RuntimeError):
@@ -476,6 +478,26 @@ def fno_modes(Rr: torch.Tensor, Ri: torch.Tensor, X: torch.Tensor, O: torch.Tens
             raise ValueError(f"{name} must be a contiguous float32 tensor of at least {size} elements")
     check(lib().wfs_fno_modes(ptr(Rr), ptr(Ri), ptr(X), ptr(O), int(m3), int(C), stream_handle(stream)),
           "wfs_fno_modes")
+
+
+def spectral_resample(src: torch.Tensor, dst: torch.Tensor, scale: float | None = None, stream=None) -> None:
+    """dst spectrum from src spectrum (``wfs_spectral_resample``): complex64 device tensors of shape
+    (..., n0, n1, n2) with the same leading (batch) size; frequencies both hold are copied times
+    scale (default (|dst| / |src|)^(1/2), the ortho-transform value-preserving factor), the rest 0."""
+    for t, name in ((src, "src"), (dst, "dst")):
+        _require_device(t, name)
+        if t.dtype != torch.complex64 or t.dim() < 3:
+            raise ValueError(f"{name} must be a complex64 tensor of at least 3 dimensions")
+    ns, nd = tuple(src.shape[-3:]), tuple(dst.shape[-3:])
+    batch = src.numel() // (ns[0] * ns[1] * ns[2])
+    if dst.numel() // (nd[0] * nd[1] * nd[2]) != batch:
+        raise ValueError("src and dst hold different numbers of grids")
+    if src.data_ptr() == dst.data_ptr():
+        raise ValueError("src and dst must differ")
+    if scale is None:
+        scale = math.sqrt((nd[0] * nd[1] * nd[2]) / (ns[0] * ns[1] * ns[2]))
+    check(lib().wfs_spectral_resample(ptr(src), i3(ns), ptr(dst), i3(nd), int(batch), float(scale),
+                                      stream_handle(stream)), "wfs_spectral_resample")
 
 
 def _require_device_any(t: torch.Tensor, name: str) -> None:

@@ -147,6 +147,39 @@ __global__ __launch_bounds__(kT) void narrow_kernel(const double2* __restrict__ 
     out[i] = make_float2(float(U[i].x), float(U[i].y));
 }
 
+// Spectrum of an n_dst^3 grid from that of an n_src^3 grid (complex64, batch stacked grids):
+// dst[k] = scale * src[k] for every signed frequency k (numpy fftfreq order) both grids hold,
+// 0 for the frequencies only dst holds.  Downsampling keeps the lowest n_dst per axis,
+// upsampling pads with zeros; with ortho transforms scale = (n_dst / n_src)^(3/2) keeps the
+// sampled function's values.
+__device__ __forceinline__ int signed_freq(int j, int n) { return j <= (n - 1) / 2 ? j : j - n; }
+
+__global__ __launch_bounds__(kT) void resample_kernel(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                      int s0, int s1, int s2, int d0, int d1, int d2, long batch,
+                                                      float scale) {
+  const long per = long(d0) * d1 * d2;
+  const long n = per * batch;
+  for (long i = long(blockIdx.x) * kT + threadIdx.x; i < n; i += long(gridDim.x) * kT) {
+    const long b = i / per;
+    long r = i - b * per;
+    const int j2 = int(r % d2);
+    r /= d2;
+    const int j1 = int(r % d1);
+    const int j0 = int(r / d1);
+    const int f0 = signed_freq(j0, d0), f1 = signed_freq(j1, d1), f2 = signed_freq(j2, d2);
+    // the source holds the signed frequencies -s/2 .. (s-1)/2 of each axis
+    const bool in = f0 >= -(s0 / 2) && f0 <= (s0 - 1) / 2 && f1 >= -(s1 / 2) && f1 <= (s1 - 1) / 2 &&
+                    f2 >= -(s2 / 2) && f2 <= (s2 - 1) / 2;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (in) {
+      const int k0 = f0 < 0 ? f0 + s0 : f0, k1 = f1 < 0 ? f1 + s1 : f1, k2 = f2 < 0 ? f2 + s2 : f2;
+      const float2 x = src[b * (long(s0) * s1 * s2) + (long(k0) * s1 + k1) * s2 + k2];
+      v = make_float2(x.x * scale, x.y * scale);
+    }
+    dst[i] = v;
+  }
+}
+
 }  // namespace
 }  // namespace wfs
 
@@ -246,6 +279,20 @@ extern "C" int wfs_narrow_c128(const void* U, void* out, long n, void* stream) {
   if (!U || !out || n < 1) { set_last_error("wfs_narrow_c128: bad argument"); return WFS_ERR_INVALID; }
   hipLaunchKernelGGL(narrow_kernel, dim3(blocks_of(n)), dim3(kT), 0, static_cast<hipStream_t>(stream),
                      static_cast<const double2*>(U), static_cast<float2*>(out), n);
+  WFS_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int wfs_spectral_resample(const void* src, const int nsrc[3], void* dst, const int ndst[3], long batch,
+                                     double scale, void* stream) {
+  if (!src || !dst || !nsrc || !ndst || batch < 1) { set_last_error("wfs_spectral_resample: bad argument"); return WFS_ERR_INVALID; }
+  for (int a = 0; a < 3; ++a)
+    if (nsrc[a] < 1 || ndst[a] < 1) { set_last_error("wfs_spectral_resample: bad grid"); return WFS_ERR_INVALID; }
+  if (src == dst) { set_last_error("wfs_spectral_resample: source and destination must differ"); return WFS_ERR_INVALID; }
+  const long n = long(ndst[0]) * ndst[1] * ndst[2] * batch;
+  hipLaunchKernelGGL(resample_kernel, dim3(blocks_of(n)), dim3(kT), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const float2*>(src), static_cast<float2*>(dst), nsrc[0], nsrc[1], nsrc[2], ndst[0],
+                     ndst[1], ndst[2], batch, float(scale));
   WFS_LAUNCH_CHECK();
   return 0;
 }

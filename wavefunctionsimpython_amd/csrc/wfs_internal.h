@@ -75,7 +75,7 @@ struct YzPlaneParams {
   const void* tw_yf = nullptr;
   const void* tw_z = nullptr;
   const void* tw_zf = nullptr;  // LINES_OBSERVE: z twiddles of the forward z transform
-  LinesMoments mom;             // LINES_OBSERVE: boxes and partial records
+  LinesMoments mom;             // LINES_OBSERVE, LINES_ENTRY_OBSERVE: boxes and partial records
   const void* pv = nullptr;     // potential phase table, state layout (null when spx is set)
   const void* spx = nullptr;    // separable potential: per-axis phase factors [packet][n_d]
   const void* spy = nullptr;
@@ -94,7 +94,8 @@ enum {
   LINES_FULL = 0,    // y inverse, z inverse, potential phase, z forward, y forward
   LINES_ENTRY = 1,   // z forward, y forward (position space in)
   LINES_EXIT = 2,    // y inverse, z inverse (position space out)
-  LINES_OBSERVE = 3  // y inverse, z inverse, box moments of the position-space rows, z forward, y forward
+  LINES_OBSERVE = 3,  // y inverse, z inverse, box moments of the position-space rows, z forward, y forward
+  LINES_ENTRY_OBSERVE = 4  // box moments of the position-space rows, z forward, y forward
 };
 struct XSlabParams {
   const void* tw_x = nullptr;   // forward twiddles exp(-2 pi i j / nx)

@@ -427,11 +427,12 @@ int run_x_slab(wfs_plan* p, void* data, const void* const* ktab, hipStream_t s, 
   return launch_x_slab(p->n[0], p->n[1], p->variant[WFS_K_X_SLAB], data, xp, s);
 }
 
-// Slab scheme: the exit launch of one call (table copy sel_exit), the box moments of the
-// position-space state, and the entry launch of the next call (copy sel_entry) as one launch,
-// then the records summed into out[packet][box][7].  The records (obs_lines) are allocated
-// with the plan.
-int run_slab_observe(wfs_plan* p, void* data, int sel_exit, int sel_entry, const wfs_box* boxes, int nb,
+// Slab scheme, observing launches: LINES_OBSERVE runs the exit launch of one stretch (table copy
+// sel_exit), the box moments of the position-space state and the entry launch of the next
+// stretch (copy sel_entry) as one launch; LINES_ENTRY_OBSERVE the moments of the position-space
+// input and the entry launch of a call.  The records are then summed into out[packet][box][7].
+// The records (obs_lines) are allocated with the plan.
+int run_slab_observe(wfs_plan* p, int mode, void* data, int sel_exit, int sel_entry, const wfs_box* boxes, int nb,
                      double* out, hipStream_t s) {
   const SlabRows g = slab_rows(p->n[1]);
   const long groups = long(p->batch) * p->n[0] * g.b;
@@ -441,10 +442,17 @@ int run_slab_observe(wfs_plan* p, void* data, int sel_exit, int sel_entry, const
   }
   YzPlaneParams pp;
   const size_t ysz = size_t(g.uy) * g.n1 * p->elem, zsz = size_t(g.uz) * g.n1 * p->elem;
-  pp.tw_yi = static_cast<const char*>(p->tw_rows[0]) + size_t(sel_exit % p->ntab) * ysz;
-  pp.tw_z = static_cast<const char*>(p->tw_rows[1]) + size_t(sel_exit % p->ntab) * zsz;
-  pp.tw_yf = static_cast<const char*>(p->tw_rows[0]) + size_t(sel_entry % p->ntab) * ysz;
-  pp.tw_zf = static_cast<const char*>(p->tw_rows[1]) + size_t(sel_entry % p->ntab) * zsz;
+  const char* ty = static_cast<const char*>(p->tw_rows[0]);
+  const char* tz = static_cast<const char*>(p->tw_rows[1]);
+  if (mode == LINES_OBSERVE) {
+    pp.tw_yi = ty + size_t(sel_exit % p->ntab) * ysz;
+    pp.tw_z = tz + size_t(sel_exit % p->ntab) * zsz;
+    pp.tw_yf = ty + size_t(sel_entry % p->ntab) * ysz;
+    pp.tw_zf = tz + size_t(sel_entry % p->ntab) * zsz;
+  } else {
+    pp.tw_yi = pp.tw_yf = ty + size_t(sel_entry % p->ntab) * ysz;
+    pp.tw_z = tz + size_t(sel_entry % p->ntab) * zsz;
+  }
   pp.nx = p->n[0];
   pp.planes = long(p->batch) * p->n[0];
   pp.mom.nb = nb;
@@ -459,8 +467,8 @@ int run_slab_observe(wfs_plan* p, void* data, int sel_exit, int sel_entry, const
   }
   pp.mom.partials = p->obs_lines;
   {
-    ProfileScope scope(p, WFS_K_SLAB_EDGE, s);
-    WFS_TRY(launch_yz_slab(p->n[1], LINES_OBSERVE, 1, data, pp, s));
+    ProfileScope scope(p, WFS_K_SLAB_OBSERVE, s);
+    WFS_TRY(launch_yz_slab(p->n[1], mode, 1, data, pp, s));
   }
   ProfileScope scope(p, WFS_K_OBSERVABLES, s);
   return launch_sum_partials(p->obs_lines, long(p->n[0]) * g.b, nb, p->batch, p->dx[0] * p->dx[1] * p->dx[2] / 27.0,
@@ -716,7 +724,7 @@ extern "C" int wfs_fft3d(wfs_plan* p, void* data, int direction, void* stream) {
 
 namespace wfs {
 namespace {
-int observe(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, double* out, hipStream_t s) {
+int check_boxes(const wfs_plan* p, const wfs_box* boxes, int nboxes) {
   for (int b = 0; b < nboxes; ++b) {
     const wfs_box& bx = boxes[b];
     for (int a = 0; a < 3; ++a) {
@@ -727,6 +735,11 @@ int observe(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, doub
       }
     }
   }
+  return 0;
+}
+
+int observe(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, double* out, hipStream_t s) {
+  WFS_TRY(check_boxes(p, boxes, nboxes));
   // up to kBoxesPerPass boxes per pass over their bounding box
   for (int b0 = 0; b0 < nboxes; b0 += kBoxesPerPass) {
     const int nb = std::min(kBoxesPerPass, nboxes - b0);
@@ -754,13 +767,14 @@ int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every,
   }
   if (V) WFS_TRY(set_potential(p, V, s));
   if (!p->pv_set) { set_last_error("wfs_step_observe: no potential (pass V or call wfs_set_potential first)"); return WFS_ERR_INVALID; }
-  WFS_TRY(observe(p, psi, boxes, nb, out, s));
+  WFS_TRY(check_boxes(p, boxes, nb));
   const void* pv = nullptr;
   WFS_TRY(current_table(p, s, &pv));
   // table copies as in the alternating calls: the stretch that starts at step k uses b + k
   const int b = p->step_index;
   p->step_index = int((long(b) + nsteps) % p->ntab);
-  WFS_TRY(run_slab_lines(p, LINES_ENTRY, psi, nullptr, s, b));
+  // the first observation rides on the entry launch
+  WFS_TRY(run_slab_observe(p, LINES_ENTRY_OBSERVE, psi, b, b, boxes, nb, out, s));
   WFS_TRY(run_x_slab(p, psi, p->khalf, s, b));
   for (int k = 0; k < nsteps; k += every) {
     const int m = std::min(every, nsteps - k);
@@ -769,7 +783,8 @@ int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every,
       WFS_TRY(run_x_slab(p, psi, step == m ? p->khalf : p->kfull, s, b + k + step));
     }
     if (k + m < nsteps) {
-      WFS_TRY(run_slab_observe(p, psi, b + k + m, b + k + m, boxes, nb, out + ((k + m) / every) * per_obs, s));
+      WFS_TRY(run_slab_observe(p, LINES_OBSERVE, psi, b + k + m, b + k + m, boxes, nb,
+                               out + ((k + m) / every) * per_obs, s));
       WFS_TRY(run_x_slab(p, psi, p->khalf, s, b + k + m));
     } else {
       WFS_TRY(run_slab_lines(p, LINES_EXIT, psi, nullptr, s, b + k + m));
@@ -841,7 +856,7 @@ extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
     return WFS_ERR_UNSUPPORTED;
   }
   if (kc == WFS_K_YZ_SLAB && variant > 0 && !slab_supported(p->precision, p->n)) {
-    set_last_error("wfs_plan_set_variant: the slab scheme needs complex64, ny = nz in {192, 256} and nx <= 256");
+    set_last_error("wfs_plan_set_variant: the slab scheme needs complex64, ny = nz in {128, 192, 256} and nx <= 256");
     return WFS_ERR_UNSUPPORTED;
   }
   p->variant[kc] = variant;

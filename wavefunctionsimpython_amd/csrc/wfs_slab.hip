@@ -247,6 +247,7 @@ int launch_lines_n(int mode, int variant, void* data, const YzPlaneParams& p, hi
   if (mode == LINES_EXIT)
     return persist ? launch_lines_m<N, LINES_EXIT, true>(data, p, s) : launch_lines_m<N, LINES_EXIT, false>(data, p, s);
   if (mode == LINES_OBSERVE) return launch_lines_m<N, LINES_OBSERVE, false>(data, p, s);
+  if (mode == LINES_ENTRY_OBSERVE) return launch_lines_m<N, LINES_ENTRY_OBSERVE, false>(data, p, s);
   set_last_error("slab y-z pass: bad mode %d", mode);
   return WFS_ERR_INVALID;
 }

@@ -364,6 +364,18 @@ __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_lines_kernel(cx<R>*
       rows_to_cols();
       __syncthreads();
       transform_y(p.tw_yf, std::integral_constant<int, -1>{});
+    } else if constexpr (MODE == LINES_ENTRY_OBSERVE) {
+      // the entry launch with the box moments of the position-space rows it reads
+      static_assert(H == 1 && !PERSIST, "the observing launch holds one round, one group per block");
+      cols_to_rows();
+      lines_moments<R, NZ, LY, B, N2Z, N1, yz_lines_threads<NZ, LY, H>()>(a[0], p.mom, plane, bres, group, tid,
+                                                                          thread_u_z());
+      __syncthreads();
+      transform_z(p.tw_z, std::integral_constant<int, -1>{});
+      __syncthreads();
+      rows_to_cols();
+      __syncthreads();
+      transform_y(p.tw_yf, std::integral_constant<int, -1>{});
     } else if constexpr (MODE == LINES_EXIT) {
       transform_y(p.tw_yi, std::integral_constant<int, +1>{});
       __syncthreads();

@@ -182,6 +182,96 @@ class FNOCoarse:
                                              _abi.stream_handle()), "wfs_fno_unpack")
 
 
+class GridResampler:
+    """Band-limited resampling between an n^3 and an nc^3 grid (complex64): ortho transform on
+    one grid, the shared low frequencies copied (``wfs_spectral_resample``, zeros above the
+    coarse band when going up), ortho inverse on the other.  Values of a function band-limited
+    to the coarse grid are kept exactly (up to float rounding)."""
+
+    def __init__(self, n: int, nc: int, device=None):
+        self.n, self.nc = int(n), int(nc)
+        dev = device or _cuda()
+        fourier = _abi.WFS_METHOD_FOURIER
+        self.fine_plan = _abi.Plan((self.n,) * 3, (1.0,) * 3, 1.0, 1.0, _abi.WFS_C64, fourier)
+        self.coarse_plan = _abi.Plan((self.nc,) * 3, (1.0,) * 3, 1.0, 1.0, _abi.WFS_C64, fourier)
+        self.fine_buf = torch.empty((self.n,) * 3, dtype=C64, device=dev)
+        self.coarse_buf = torch.empty((self.nc,) * 3, dtype=C64, device=dev)
+
+    def down(self, x: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """out (nc^3) = x (n^3) resampled; x is not changed."""
+        self.fine_buf.copy_(x)
+        self.fine_plan.fft3d(self.fine_buf, -1)
+        _abi.spectral_resample(self.fine_buf, out)
+        self.coarse_plan.fft3d(out, +1)
+        return out
+
+    def up_add(self, xc: torch.Tensor, dst: torch.Tensor) -> None:
+        """dst (n^3, contiguous complex64) += xc (nc^3) resampled (xc is overwritten by its spectrum)."""
+        if dst.dtype != C64 or not dst.is_contiguous() or dst.numel() != self.fine_buf.numel():
+            raise ValueError("dst must be a contiguous complex64 tensor of the fine grid")
+        self.coarse_plan.fft3d(xc, -1)
+        _abi.spectral_resample(xc, self.fine_buf)
+        self.fine_plan.fft3d(self.fine_buf, +1)
+        n2 = 2 * self.fine_buf.numel()
+        _abi.check(_abi.lib().wfs_fno_axpby(_abi.ptr(dst), _abi.ptr(self.fine_buf), 1.0, 1.0, n2, _abi.stream_handle()),
+                   "wfs_fno_axpby")
+
+
+class CoarseGridFNOCoarse:
+    """G(psi) = B(psi) + the network's correction computed on a coarser grid.
+
+    The features [Re B(psi), Im B(psi), Re psi, Im psi, V] are resampled to the network's nc^3
+    grid (``GridResampler``), the network predicts B(psi) + c there, and c, the correction, is
+    resampled back and added to the fine B(psi).  The network is the same Fourier neural
+    operator (4 layers, 12 modes, width 32 in the parareal workload) at nc^3 instead of n^3
+    points: a call costs the base step, four transforms on each grid and one network pass at
+    (nc/n)^3 of the points.  The correction is band-limited to the coarse grid.
+    """
+
+    name = "fno_coarse_grid"
+
+    def __init__(self, net: FNO3d, V: torch.Tensor, pscale: float, vscale: float, base: SplitStepCoarse):
+        if net.cin != 5 or net.cout != 2 or not net.residual:
+            raise ValueError("network must map 5 feature planes to 2 with a residual")
+        self.net, self.base = net, base
+        self.pscale, self.vscale = float(pscale), float(vscale)
+        n = int(round(V.numel() ** (1.0 / 3.0)))
+        if n ** 3 != V.numel() or net.n > n:
+            raise ValueError("the potential's grid must be cubic and at least the network's")
+        self.rs = GridResampler(n, net.n, V.device)
+        cshape = (net.n,) * 3
+        self.Vc = self.rs.down(V, torch.empty(cshape, dtype=C64, device=V.device))
+        self.Pc = net.P
+        self.a = torch.empty(5 * self.Pc, dtype=torch.float32, device=V.device)
+        self.src_c = torch.empty(cshape, dtype=C64, device=V.device)
+        self.base_c = torch.empty(cshape, dtype=C64, device=V.device)
+        self.corr = torch.empty(2 * self.Pc, dtype=torch.float32, device=V.device)
+        self.corr_c = torch.empty(cshape, dtype=C64, device=V.device)
+
+    def features(self, src: torch.Tensor, base_out: torch.Tensor, Vc: torch.Tensor, a: torch.Tensor) -> None:
+        """Network input at the coarse grid for state src with base output base_out (fine)."""
+        self.rs.down(base_out, self.base_c)
+        self.rs.down(src, self.src_c)
+        pack_features(self.src_c, Vc, self.pscale, self.vscale, a, self.base_c)
+
+    def target(self, psi: torch.Tensor, out: torch.Tensor) -> None:
+        """Training target: psi (fine) resampled to the coarse grid, scaled planes."""
+        self.rs.down(psi, self.src_c)
+        pack_target(self.src_c, self.pscale, out)
+
+    def __call__(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        self.base(src, dst)                                  # dst = B(src), fine grid
+        self.features(src, dst, self.Vc, self.a)
+        out = self.net.forward(self.a)                       # B(src) + c, coarse, scaled planes
+        lib, s = _abi.lib(), _abi.stream_handle()
+        P2 = 2 * self.Pc
+        self.corr.copy_(out[:P2])
+        _abi.check(lib.wfs_fno_axpby(_abi.ptr(self.corr), _abi.ptr(self.a), -1.0, 1.0, P2, s), "wfs_fno_axpby")
+        _abi.check(lib.wfs_fno_unpack(_abi.ptr(self.corr), 1.0 / self.pscale, _abi.ptr(self.corr_c), self.Pc, s),
+                   "wfs_fno_unpack")
+        self.rs.up_add(self.corr_c, dst)
+
+
 # ---------------------------------------------------------------------------- parareal driver
 @dataclass
 class PararealResult:
@@ -402,17 +492,25 @@ def _mix_planes(dst: torch.Tensor, x: torch.Tensor, y: torch.Tensor, a: complex,
 
 def train_coarse(net: FNO3d, pairs: TrajectoryPairs, slices: int, pscale: float, vscale: float,
                  lr: float = 1e-3, lr_final: float = 1e-4, epochs: int = 1, seed: int = 0, mix: float = 0.0,
-                 log=None) -> List[float]:
+                 log=None, featurizer: Optional["CoarseGridFNOCoarse"] = None) -> List[float]:
     """Train ``net`` on ``slices`` advances of every trajectory (slices * B pairs).
 
     epochs == 1: online, each pair is used once right after the solver made it
     (no pair storage, for large grids); epochs > 1: the pair set is stored as
     feature planes and visited ``epochs`` times in shuffled order.  Adam with
     cosine learning-rate decay from lr to lr_final.  Returns every step's loss.
+    With a ``featurizer`` (a CoarseGridFNOCoarse of this network) the features and
+    targets are the pairs resampled to the network's coarser grid.
     """
     if (pairs.base is None) != (net.cin == 3):
         raise ValueError("network input width does not match the pair features")
+    if featurizer is not None and pairs.base is None:
+        raise ValueError("the coarse-grid features need the base propagation")
     dev = pairs.psi.device
+    coarse_V = None
+    if featurizer is not None:
+        rs = featurizer.rs
+        coarse_V = [rs.down(pairs.V[b], torch.empty((net.n,) * 3, dtype=C64, device=dev)) for b in range(pairs.B)]
     B, P = pairs.B, net.P
     total = slices * B * epochs
     rng = np.random.default_rng(seed)
@@ -432,6 +530,10 @@ def train_coarse(net: FNO3d, pairs: TrajectoryPairs, slices: int, pscale: float,
 
     def make(b, feat, targ):
         base_out = pairs.base_out[b] if pairs.base is not None else None
+        if featurizer is not None:
+            featurizer.features(pairs.prev[b], base_out, coarse_V[b], feat)
+            featurizer.target(pairs.psi[b], targ)
+            return
         pack_features(pairs.prev[b], pairs.V[b], pscale, vscale, feat, base_out)
         pack_target(pairs.psi[b], pscale, targ)
 
