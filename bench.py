@@ -274,14 +274,17 @@ def batch_bench(steps, warmup):
     }
 
 
-def parareal_bench(train_pairs, seed_train=1, seed_test=2):
+def parareal_bench(train_pairs, seed_train=1, seed_test=2, coarse_n=32, full_fno=False):
     """Time-to-T on the 128^3 harmonic + cos^2 lattice problem: 32 slices x 50 steps (dt 0.005).
 
     (i) plain serial split-step; (ii) parareal with a one-step split-step coarse propagator;
-    (iii) parareal with the trained FNO correction of that coarse step.  The FNO (4 layers,
-    12 modes, width 32) is trained first on solver pairs (40 trajectories from default_rng(1),
-    online, one pass); training time is reported separately.  Errors are relative L2 against
-    the GPU complex128 run of the same problem.
+    (iii) parareal with the FNO correction of that coarse step, the network (4 layers, 12
+    modes, width 32) run on a coarse_n^3 grid with the features resampled spectrally
+    (CoarseGridFNOCoarse); with full_fno also (iv) the same network on the full 128^3 grid.
+    Each network is trained first on solver pairs (40 trajectories from default_rng(1), online,
+    one pass); training time is reported separately.  Errors are relative L2 and phase against
+    the GPU complex128 run of the same problem.  The cost of one coarse call is reported beside
+    the 50-step fine slice it stands in for.
     """
     import math
     import numpy as np
@@ -292,18 +295,36 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
     n, dx, S, NS = 128, 0.25, 50, 32
     ntraj = 40
     slices_train = max(1, train_pairs // ntraj)
-    rng = np.random.default_rng(seed_train)
-    probs = [pr.random_lattice_problem(rng, n=n, dx=dx, k0=0.5, r0=3.0) for _ in range(ntraj)]
     pscale, vscale = math.sqrt((n * dx) ** 3), 0.2
-    net = FNO3d(n, width=32, modes=12, layers=4, in_channels=5, output_scale=0.0, residual_scale=1e-3)
-    pairs = pr.TrajectoryPairs(probs, S, base_steps=1, autotune=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    losses = pr.train_coarse(net, pairs, slices_train, pscale, vscale, lr=1e-3, lr_final=1e-4, epochs=1,
-                             log=lambda it, l: print(f"train step {it}: loss {l:.4f}", file=sys.stderr, flush=True))
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    del pairs
+
+    def log(it, l):
+        print(f"train step {it}: loss {l:.4f}", file=sys.stderr, flush=True)
+
+    def train(net_n, featurize):
+        rng = np.random.default_rng(seed_train)
+        probs = [pr.random_lattice_problem(rng, n=n, dx=dx, k0=0.5, r0=3.0) for _ in range(ntraj)]
+        net = FNO3d(net_n, width=32, modes=12, layers=4, in_channels=5, output_scale=0.0, residual_scale=1e-3)
+        pairs = pr.TrajectoryPairs(probs, S, base_steps=1, autotune=True)
+        feat = None
+        if featurize:
+            V0 = pairs.V[0]
+            feat = pr.CoarseGridFNOCoarse(net, V0, pscale, vscale,
+                                          pr.SplitStepCoarse(probs[0].shape, (dx,) * 3, S * probs[0].dt, 1.0, V0, 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses = pr.train_coarse(net, pairs, slices_train, pscale, vscale, lr=1e-3, lr_final=1e-4, epochs=1,
+                                 log=log, featurizer=feat)
+        torch.cuda.synchronize()
+        k = max(1, len(losses) // 10)
+        info = {"pairs": len(losses), "time_s": time.perf_counter() - t0,
+                "loss_first": float(np.mean(losses[:k])), "loss_last": float(np.mean(losses[-k:])),
+                "loss_note": "relative to the one-step split-step's own error (1 = no gain)"}
+        del pairs
+        return net, info
+
+    nets = {"coarse_grid": train(coarse_n, True)}
+    if full_fno:
+        nets["full_grid"] = train(n, False)
     # The training leaves millions of Python objects behind; a full collection of the cyclic
     # garbage collector inside a timed run then stalls it for ~37 ms (measured in the kernel
     # trace as one idle gap).  Collect once and move the survivors out of the collector's view.
@@ -319,9 +340,9 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
     psi64 = psi0.to(torch.complex128)
     p64.step(psi64, None, S * NS)
 
-    def rel(a):
-        a = a.to(torch.complex128)
-        return float((a - psi64).abs().pow(2).sum().sqrt() / psi64.abs().pow(2).sum().sqrt())
+    def errs(a):
+        acc = accuracy(a, psi64)
+        return {"rel_l2_vs_fp64": acc["rel_l2_vs_c128"], "phase_err_rad": acc["phase_err_rad"]}
 
     # plain: one call of all the steps (the fastest serial form; slicing it would add the entry and
     # exit passes of every call), best of three
@@ -354,30 +375,58 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
     t_plain_graph = min(gtimes)
     graph.close()
     base = pr.SplitStepCoarse(test.shape, (dx,) * 3, S * test.dt, 1.0, V, 1, axes=test.axes())
-    fno = pr.FNOCoarse(net, V, pscale, vscale, base=base)
+    eager = {"parareal_splitstep": base,
+             "parareal_fno": pr.CoarseGridFNOCoarse(nets["coarse_grid"][0], V, pscale, vscale, base)}
+    if full_fno:
+        eager["parareal_fno_full_grid"] = pr.FNOCoarse(nets["full_grid"][0], V, pscale, vscale, base=base)
+    # every coarse propagator replayed from a hipGraph of its call (its many small launches are
+    # otherwise bound by the host's launch rate)
+    coarse = {k: pr.GraphedCoarse(c, test.shape) for k, c in eager.items()}
+
+    def call_ms(fn, src, dst, reps=20):
+        fn(src, dst)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(src, dst)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    tmp = torch.empty_like(psi0)
+
+    def fine_slice(src, dst):
+        dst.copy_(src)
+        par.serial.set_step_index(0)
+        par.serial.step(dst, None, S)
+
+    slice_ms = call_ms(fine_slice, psi0, tmp)
     out = {}
-    for name, c in (("parareal_splitstep", base), ("parareal_fno", fno)):
+    for name, c in coarse.items():
         par.run(psi0, c, tol=1e-4)              # warm-up
         res = par.run(psi0, c, tol=1e-4)
+        ms = call_ms(c, psi0, tmp)
+        ms_eager = call_ms(eager[name], psi0, tmp)
         out[name] = {"time_to_T_s": res.timings["total_s"], "iterations": res.iterations,
-                     "converged": res.converged, "fallback": res.fallback,
-                     "rel_l2_vs_fp64": rel(res.state), "history": res.history,
+                     "converged": res.converged, "fallback": res.fallback, **errs(res.state),
+                     "history": res.history, "coarse_call_ms": ms, "coarse_call_over_fine_slice": ms / slice_ms,
+                     "coarse_call_ms_eager": ms_eager,
                      "phases_s": {k: v for k, v in res.timings.items() if k != "total_s"}}
-    # the network alone: inference forward time, matrix-core and HBM rates
-    a = torch.zeros(net.cin * net.P, dtype=torch.float32, device=V.device)
-    for _ in range(3):
-        net.forward(a)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(10):
-        net.forward(a)
-    torch.cuda.synchronize()
-    t_fwd = (time.perf_counter() - t0) / 10
-    cost = net.forward_cost()
-    fno_perf = {"forward_ms": t_fwd * 1e3, "mfma_tflops": cost["flops"] / t_fwd / 1e12,
-                "mfma_peak_tflops_bf16": 2500.0, "hbm_GBs": cost["hbm_bytes"] / t_fwd / 1e9,
-                "hbm_bytes_per_forward": cost["hbm_bytes"], "flops_per_forward": cost["flops"]}
-    k = max(1, len(losses) // 10)
+    # the networks alone: inference forward time, matrix-core and HBM rates
+    fno_perf = {}
+    for key, (net, _) in nets.items():
+        a = torch.zeros(net.cin * net.P, dtype=torch.float32, device=V.device)
+        for _ in range(3):
+            net.forward(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            net.forward(a)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / 10
+        cost = net.forward_cost()
+        fno_perf[key] = {"grid": net.n, "forward_ms": t_fwd * 1e3, "mfma_tflops": cost["flops"] / t_fwd / 1e12,
+                         "mfma_peak_tflops_bf16": 2500.0, "hbm_GBs": cost["hbm_bytes"] / t_fwd / 1e9,
+                         "hbm_bytes_per_forward": cost["hbm_bytes"], "flops_per_forward": cost["flops"]}
     return {
         "metric": "time-to-T at 1e-4 relative L2, FNO-parareal vs plain (128^3 lattice, 32 slices x 50 steps)",
         "value": out["parareal_fno"]["time_to_T_s"], "unit": "s", "n_gpus": 1, "steps": S * NS, "warmup": 1,
@@ -387,16 +436,14 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2):
         "config": {"workload": "128^3 harmonic + cos^2 lattice (config drawn with default_rng(2)), dx 0.25, "
                                "dt 0.005, 32 slices x 50 steps", "problem": test.__dict__,
                    "fno": {"layers": 4, "modes": 12, "width": 32, "base": "1-step split-step",
-                           "residual_scale": 1e-3}},
-        "plain": {"time_to_T_s": t_plain, "rel_l2_vs_fp64": rel(plain),
-                  "grid_pt_steps_per_s": n ** 3 * S * NS / t_plain,
-                  "graph_time_to_T_s": t_plain_graph, "graph_rel_l2_vs_fp64": rel(plain_g)},
+                           "residual_scale": 1e-3, "network_grid": coarse_n}},
+        "plain": {"time_to_T_s": t_plain, **errs(plain), "grid_pt_steps_per_s": n ** 3 * S * NS / t_plain,
+                  "graph_time_to_T_s": t_plain_graph, "graph_rel_l2_vs_fp64": errs(plain_g)["rel_l2_vs_fp64"]},
+        "fine_slice_ms": slice_ms,
         "speedup_fno_parareal_vs_plain": min(t_plain, t_plain_graph) / out["parareal_fno"]["time_to_T_s"],
         **out,
         "fno_forward": fno_perf,
-        "training": {"pairs": len(losses), "time_s": t_train, "loss_first": float(np.mean(losses[:k])),
-                     "loss_last": float(np.mean(losses[-k:])),
-                     "loss_note": "relative to the one-step split-step's own error (1 = no gain)"},
+        "training": {k: info for k, (_, info) in nets.items()},
     }
 
 
@@ -581,6 +628,8 @@ def main():
                          "split-step and power series against complex128; --steps, 2000 in the config) or dropin "
                          "(the 256^3 double slit through initialisation + core_sim.run_iteration)")
     ap.add_argument("--train-pairs", type=int, default=2000, help="FNO training pairs (parareal workload)")
+    ap.add_argument("--coarse-n", type=int, default=32, help="grid of the coarse-grid FNO (parareal workload)")
+    ap.add_argument("--full-fno", action="store_true", help="also run the FNO on the full 128^3 grid (parareal)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 5 device copies of a 128 MiB buffer (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -605,7 +654,7 @@ def main():
         print(json.dumps(batch_bench(args.steps, args.warmup)), flush=True)
         return
     if args.workload == "parareal":
-        print(json.dumps(parareal_bench(args.train_pairs)), flush=True)
+        print(json.dumps(parareal_bench(args.train_pairs, coarse_n=args.coarse_n, full_fno=args.full_fno)), flush=True)
         return
     if args.workload == "large":
         print(json.dumps(large_bench(args.steps)), flush=True)

@@ -196,3 +196,23 @@ def test_trained_coarse_grid_fno_lowers_its_base_error(dev):
           f"corrected {np.mean(e_net):.2e}")
     assert last < 0.8 * first
     assert np.mean(e_net) < 0.9 * np.mean(e_base)
+
+
+def test_graphed_coarse_replays_equal_eager_calls(dev):
+    """GraphedCoarse replays its capture of a coarse call: the same result as the eager call for
+    several inputs in a row (eager work and synchronisations between the replays)."""
+    prob, V, psi0, par = _setup(4)
+    base = pr.SplitStepCoarse(prob.shape, (DX,) * 3, STEPS * prob.dt, 1.0, V, 1)
+    net = FNO3d(16, width=8, modes=4, layers=2, in_channels=5, output_scale=0.1, residual_scale=1e-2)
+    for coarse in (base, pr.CoarseGridFNOCoarse(net, V, math.sqrt((N_GRID * DX) ** 3), 0.2, base)):
+        graphed = pr.GraphedCoarse(coarse, prob.shape)
+        _, states = par.serial_fine(psi0, keep=True)
+        for j, src in enumerate(states):
+            a = torch.empty_like(src)
+            b = torch.empty_like(src)
+            coarse(src, a)
+            graphed(src, b)
+            torch.cuda.synchronize()
+            junk = torch.ones(1000, device=dev).sum().item()
+            assert junk == 1000.0
+            assert torch.equal(a, b), f"{coarse.name} input {j}"

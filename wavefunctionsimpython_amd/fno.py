@@ -250,7 +250,15 @@ class FNO3d:
         pr = self.params
         saved_v: List[torch.Tensor] = []
         saved_u: List[torch.Tensor] = []
-        v = torch.empty(C * P, dtype=self.dtype, device=self.device)
+
+        def act_buffer(avoid=None):
+            # training keeps every activation for the backward pass; inference ping-pongs between
+            # the workspace's gradient buffers (no allocation, so a forward can be graph-captured)
+            if keep:
+                return torch.empty(C * P, dtype=self.dtype, device=self.device)
+            return ws["d2"] if avoid is ws["d1"] else ws["d1"]
+
+        v = act_buffer()
         if self.fused:
             # lift and the first z-stage DFT, then per layer the middle of the spectral path and one
             # fused pass: u = W v + (last spectral stage) + b, v' = GELU(u), next z-stage DFT
@@ -258,8 +266,9 @@ class FNO3d:
             for l in range(self.L):
                 self._spectral_mid(l)
                 last = l == self.L - 1
-                u = torch.empty(C * P, dtype=self.dtype, device=self.device) if (keep or last) else None
-                vn = None if last else torch.empty(C * P, dtype=self.dtype, device=self.device)
+                u = (torch.empty(C * P, dtype=self.dtype, device=self.device) if keep
+                     else ws["d3"] if last else None)
+                vn = None if last else act_buffer(v)
                 self._layer(pr[f"W{l}"], v, ws["Z"], pr[f"b{l}"], u, vn, None if last else ws["T"], C, act=True)
                 if keep:
                     saved_v.append(v)
@@ -269,10 +278,10 @@ class FNO3d:
             self._gemm(pr["P"], a, v, 1, C, P, self.cin, (0, self.cin, 1), (0, P, 1), (0, P, 1), flags=SB,
                        bias=pr["pb"])
             for l in range(self.L):
-                u = torch.empty(C * P, dtype=self.dtype, device=self.device)
+                u = torch.empty(C * P, dtype=self.dtype, device=self.device) if keep else ws["d3"]
                 self._spectral(v, l, u)
                 last = l == self.L - 1
-                vn = None if last else torch.empty(C * P, dtype=self.dtype, device=self.device)
+                vn = None if last else act_buffer(v)
                 # u = W v + S + b and v' = GELU(u) in the GEMM's epilogue; u is stored only when
                 # the backward pass needs it (or when it is the network's last activation)
                 self._gemm(pr[f"W{l}"], v, u, 1, C, P, C, (0, C, 1), (0, P, 1), (0, P, 1), beta=1.0,
