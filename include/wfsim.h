@@ -265,7 +265,8 @@ int wfs_potential_operator(long count, const void* V, double dt, int precision, 
 #define WFS_K_X_SLAB 12
 #define WFS_K_SLAB_EDGE 13
 #define WFS_K_SLAB_OBSERVE 14
-#define WFS_K_COUNT 15
+#define WFS_K_PV_PLANES 15
+#define WFS_K_COUNT 16
 int wfs_plan_set_profiling(wfs_plan* plan, int enable);
 
 /*
@@ -295,7 +296,11 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * among themselves are bit-identical.  WFS_K_X_SLAB: variant 1 (default at
  * 192) puts 16 z points in an x block, variant 2 (default at 256) 8, pairing
  * the two blocks of a 128-byte line on one XCD.  The autotuner keeps the
- * fastest.
+ * fastest.  WFS_K_PV_PLANES (no launches of its own): variant 1 (default)
+ * lets the slab scheme's y-z launch read one copy of each run of x planes
+ * whose potential phase factors are bitwise equal (the planes away from walls
+ * and absorbing x layers), so that those reads hit the caches instead of HBM;
+ * variant 0 reads every plane's own copy.  The results are bit-identical.
  */
 int wfs_plan_autotune(wfs_plan* plan, void* stream);
 int wfs_plan_set_variant(wfs_plan* plan, int kernel_class, int variant);

@@ -246,3 +246,61 @@ def test_potential_table_order_follows_the_scheme(dev):
         torch.cuda.synchronize()
         print(f"case: table set under yz_slab {first}, stepped under {then}", flush=True)
         assert torch.equal(a, b)
+
+
+def _repeating_planes(shape, batch, seed):
+    """Potential whose x planes repeat in runs, as walls and absorbing layers leave them: per packet
+    three distinct leading planes, a run of one plane, two distinct wall planes, then the same run
+    broken by planes that differ from their neighbours in one voxel (the last, the first and a
+    middle one of the plane)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    out = []
+    for _ in range(batch):
+        def plane():
+            return (rng.standard_normal((ny, nz)) * 0.5 + 1j * rng.uniform(-0.05, 0.0, (ny, nz))).astype(np.complex64)
+        base = plane()
+        V = np.empty(shape, np.complex64)
+        V[:] = base
+        V[0], V[1], V[2] = plane(), plane(), plane()
+        V[9], V[10] = plane(), plane()
+        V[13, -1, -1] += np.float32(0.25)
+        V[14:, :, :] = V[13]             # the changed plane starts a new run
+        V[16, 0, 0] -= np.float32(0.5)
+        V[17, ny // 2, nz // 3] += np.complex64(0.01j)
+        out.append(V)
+    return np.stack(out) if batch > 1 else out[0]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("batch", [1, 2])
+def test_shared_potential_planes_bit_identical(dev, form, batch):
+    """Reading one copy of each run of bitwise-equal x planes (kernel class pv_planes, variant 1)
+    gives the state, bit for bit, of reading every plane's own copy (variant 0); single-voxel
+    differences at a plane's first, last and a middle voxel end a run."""
+    shape = (32, 256, 256)
+    V = _repeating_planes(shape, batch, 77)
+    psi1, _ = _inputs(shape, 78)
+    psi0 = np.stack([psi1, np.roll(psi1, 5, axis=0)]) if batch > 1 else psi1
+    Vd = torch.from_numpy(V).to(dev)
+    out = {}
+    for share in (1, 0):
+        p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER, batch=batch)
+        assert p.variants()["pv_planes"] == 1
+        p.set_variant("yz_slab", form[0])
+        p.set_variant("x_slab", form[1])
+        p.set_variant("pv_planes", share)
+        p.set_potential(Vd)
+        psi = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+        p.step(psi, None, 4)
+        obs = p.step_observe(psi, None, 6, 3, [((0, 0, 0), (shape[0], 128, 128))])
+        torch.cuda.synchronize()
+        out[share] = (psi, obs)
+    print(f"case: slab {form}, batch {batch}, {shape}, shared planes against own copies", flush=True)
+    assert torch.equal(out[1][0], out[0][0])
+    assert torch.equal(out[1][1], out[0][1])
+    if batch == 1:
+        ref = oracle.split_step(psi0, V, DX, DT, 1.0, nsteps=10)
+        assert _rel(out[1][0].cpu().numpy(), ref) < TOL_ORACLE
+    with pytest.raises(_abi.WfsError):
+        p.set_variant("pv_planes", 2)

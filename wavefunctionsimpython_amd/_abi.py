@@ -100,7 +100,8 @@ SIGNATURES = {
 
 # kernel classes of wfs_plan_profile
 KERNEL_CLASSES = ("z_fwd", "z_inv", "z_inv_v_fwd", "y_fwd", "y_inv", "x_fwd_k_inv", "x_fwd", "x_inv",
-                  "horner_stage", "observables", "yz_plane", "yz_slab", "x_slab", "slab_edge", "slab_observe")
+                  "horner_stage", "observables", "yz_plane", "yz_slab", "x_slab", "slab_edge", "slab_observe",
+                  "pv_planes")
 
 
 class WfsError(RuntimeError):

@@ -77,6 +77,8 @@ struct YzPlaneParams {
   const void* tw_zf = nullptr;  // LINES_OBSERVE: z twiddles of the forward z transform
   LinesMoments mom;             // LINES_OBSERVE, LINES_ENTRY_OBSERVE: boxes and partial records
   const void* pv = nullptr;     // potential phase table, state layout (null when spx is set)
+  const int* pv_planes = nullptr;  // slab scheme: the table plane holding each x plane's factors
+                                   // (the first of a run of identical planes), or null
   const void* spx = nullptr;    // separable potential: per-axis phase factors [packet][n_d]
   const void* spy = nullptr;
   const void* spz = nullptr;
@@ -106,6 +108,10 @@ struct XSlabParams {
   long packets = 0;             // grids stacked along x
 };
 bool slab_supported(int precision, const int n[3]);
+// Per x plane (packets * nx planes of plane_elems complex64 factors, contiguous per plane): src[p]
+// = the first plane of the run of bitwise-identical consecutive planes holding p, within its
+// packet.  flags: workspace of packets * nx ints.
+int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets, int* flags, int* src, hipStream_t s);
 // thread geometry of the slab scheme's y-z launch: y lines split over uy threads, z lines
 // over uz threads, n1 points per thread; y-line twiddle stride b in the ny table
 struct SlabRows {

@@ -32,6 +32,9 @@ struct wfs_plan {
   long obs_lines_len;
   bool pv_nat_ok;         // pv holds the current potential
   bool pv_lines_ok;       // pv_lines holds the current potential
+  int* pv_planes;         // slab scheme: per x plane the first plane of its run of identical
+                          // planes of pv_lines, then as many ints of workspace (lazy)
+  bool pv_planes_ok;      // pv_planes describes the current pv_lines
   bool pv_set;
   void* tw_rows[2];       // slab scheme: the y and z twiddles of each thread's lines, [ntab][u][k1]
   void* sep[3];           // separable potential: per-axis exp(-i dt f_d), [ntab][batch][n_d]
@@ -291,6 +294,26 @@ int alloc_table(wfs_plan* p, void** t, hipStream_t s) {
   return 0;
 }
 
+// the map of x planes of pv_lines that repeat the previous plane (kernel class WFS_K_PV_PLANES,
+// variant 1): built with the lines table, so that graph capture finds it ready
+int share_planes(wfs_plan* p, hipStream_t s) {
+  p->pv_planes_ok = false;
+  if (p->variant[WFS_K_PV_PLANES] == 0) return 0;
+  const long planes = long(p->batch) * p->n[0];
+  if (!p->pv_planes) {
+    WFS_TRY(refuse_if_capturing(s, "the potential plane map"));
+    if (hipMalloc(&p->pv_planes, size_t(2 * planes) * sizeof(int)) != hipSuccess) {
+      p->pv_planes = nullptr;
+      set_last_error("hipMalloc for the potential plane map failed");
+      return WFS_ERR_ALLOC;
+    }
+  }
+  WFS_TRY(launch_plane_dedup(p->pv_lines, long(p->n[1]) * p->n[2], p->n[0], p->batch, p->pv_planes + planes,
+                             p->pv_planes, s));
+  p->pv_planes_ok = true;
+  return 0;
+}
+
 // the phase table in the order the current scheme reads (the other order is rebuilt from it
 // on demand)
 int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
@@ -301,6 +324,7 @@ int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
                                          slab_lines_order(p->n[1]), s));
     p->pv_lines_ok = true;
     p->pv_nat_ok = false;
+    WFS_TRY(share_planes(p, s));
   } else {
     WFS_TRY(alloc_table(p, &p->pv, s));
     WFS_TRY(launch_potential_phase(p->precision, count, static_cast<const float2*>(V), p->dt, p->pv, s));
@@ -323,7 +347,9 @@ int current_table(wfs_plan* p, hipStream_t s, const void** out) {
       WFS_TRY(alloc_table(p, &p->pv_lines, s));
       WFS_TRY(launch_table_reorder(p->precision, count, p->pv, p->pv_lines, slab_lines_order(p->n[1]), true, s));
       p->pv_lines_ok = true;
+      p->pv_planes_ok = false;
     }
+    if (!p->pv_planes_ok && p->variant[WFS_K_PV_PLANES] != 0) WFS_TRY(share_planes(p, s));
     *out = p->pv_lines;
   } else {
     if (!p->pv_nat_ok) {
@@ -406,6 +432,7 @@ int run_slab_lines(wfs_plan* p, int mode, void* data, const void* pv, hipStream_
       pp.spz = static_cast<const char*>(p->sep[2]) + size_t(c) * per * p->n[2];
     } else {
       pp.pv = pv;
+      if (p->pv_planes_ok && p->variant[WFS_K_PV_PLANES] != 0) pp.pv_planes = p->pv_planes;
     }
   }
   pp.nx = p->n[0];
@@ -579,6 +606,7 @@ extern "C" int wfs_plan_destroy(wfs_plan* p) {
   if (p->plane_out) (void)hipFree(p->plane_out);
   if (p->pv) (void)hipFree(p->pv);
   if (p->pv_lines) (void)hipFree(p->pv_lines);
+  if (p->pv_planes) (void)hipFree(p->pv_planes);
   if (p->obs_lines) (void)hipFree(p->obs_lines);
   for (int k = 0; k < 2; ++k)
     if (p->tw_rows[k]) (void)hipFree(p->tw_rows[k]);
@@ -849,7 +877,7 @@ extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, 
 }
 
 extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
-  const int vmax = kc == WFS_K_HORNER ? 3 : 2;
+  const int vmax = kc == WFS_K_HORNER ? 3 : kc == WFS_K_PV_PLANES ? 1 : 2;
   if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > vmax) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
   if (kc == WFS_K_YZ_PLANE && variant > 0 && !yz_plane_supported(p->precision, p->n[1], p->n[2], variant)) {
     set_last_error("wfs_plan_set_variant: fused plane pass variant %d unsupported for %d x %d planes", variant, p->n[1], p->n[2]);

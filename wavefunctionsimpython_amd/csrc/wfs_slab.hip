@@ -213,6 +213,34 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   }
 }
 
+// flags[p] = 1 when x plane p equals plane p - 1 of the same packet bit for bit, else 0
+__global__ __launch_bounds__(256) void plane_equal_kernel(const unsigned long long* __restrict__ table,
+                                                          long plane_elems, int nx, int* __restrict__ flags) {
+  const long p = blockIdx.x;
+  int differ = 0;
+  if (p % nx == 0) {
+    differ = 1;
+  } else {
+    const unsigned long long* a = table + p * plane_elems;
+    const unsigned long long* b = a - plane_elems;
+    for (long i = threadIdx.x; i < plane_elems && !differ; i += 256) differ = a[i] != b[i];
+  }
+  differ = __syncthreads_or(differ);
+  if (threadIdx.x == 0) flags[p] = differ ? 0 : 1;
+}
+
+// src[p] = the first plane of p's run of identical planes (one thread per packet)
+__global__ void plane_runs_kernel(const int* __restrict__ flags, int nx, int packets, int* __restrict__ src) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= packets) return;
+  const long base = long(k) * nx;
+  int first = int(base);
+  for (int x = 0; x < nx; ++x) {
+    if (!flags[base + x]) first = int(base + x);
+    src[base + x] = first;
+  }
+}
+
 // compute units of the current device (cached per device)
 int cu_count() {
   static int cache[64] = {0};
@@ -285,6 +313,20 @@ int launch_x_slab_ny(int nx, int variant, void* data, const XSlabParams& p, hipS
 }
 
 }  // namespace
+
+int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets, int* flags, int* src, hipStream_t s) {
+  const long planes = long(nx) * packets;
+  if (planes < 1 || planes > 0x7fffffffL || plane_elems < 1) {
+    set_last_error("potential plane sharing: bad sizes");
+    return WFS_ERR_INVALID;
+  }
+  hipLaunchKernelGGL(plane_equal_kernel, dim3(unsigned(planes)), dim3(256), 0, s,
+                     static_cast<const unsigned long long*>(table), plane_elems, nx, flags);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return int(e);
+  hipLaunchKernelGGL(plane_runs_kernel, dim3(unsigned((packets + 63) / 64)), dim3(64), 0, s, flags, nx, packets, src);
+  return int(hipGetLastError());
+}
 
 // complex64, ny = nz in {128, 192, 256}, nx a supported FFT length up to 256 (at 512 the x
 // lines of a lane pair do not fit in registers)

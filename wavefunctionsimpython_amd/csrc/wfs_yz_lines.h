@@ -327,7 +327,9 @@ __attribute__((amdgpu_waves_per_eu(H == 1 ? 1 : 4))) void yz_lines_kernel(cx<R>*
             }
           } else if constexpr (PV_LINES) {
             constexpr unsigned T = unsigned(yz_lines_threads<NZ, LY, H>());
-            const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + group * (long(LY) * NZ);
+            // x planes whose factors equal the previous plane's read that plane's copy
+            const long tg = p.pv_planes ? long(p.pv_planes[plane]) * B + bres : group;
+            const C* __restrict__ pvb = reinterpret_cast<const C*>(p.pv) + tg * (long(LY) * NZ);
 #pragma unroll
             for (int i = 0; i < N1; ++i) {
               a[r][i] = cmul(a[r][i], pvb[unsigned(r * N1 + i) * T + unsigned(tid)]);
