@@ -18,8 +18,13 @@
 // that does not shrink with the change of the input.
 //
 // Block tile (32*WM) x (32*WN) x 32, one wavefront per 32 x 32 sub-tile, using
-// v_mfma_f32_32x32x8_bf16: lane l holds A[l%32][4(l/32)..+3] and
-// B[4(l/32)..+3][l%32]; accumulator item v of lane l is C[8(v/4)+4(l/32)+v%4][l%32].
+// gfx950's v_mfma_f32_32x32x16_bf16 (16 k per instruction, twice the rate of the 8-k
+// form): lane l gives 8 k values of row l%32 of A and the same 8 k values of column l%32
+// of B; accumulator item v of lane l is C[8(v/4)+4(l/32)+v%4][l%32].  Which k a lane's
+// element stands for needs no care: the instruction sums A[i][k] B[k][j] over the 16
+// (lane half, element) slots, so any assignment of a step's 16 k values to those slots
+// that A and B share gives the same sum.  Lane l's elements e are k = k0 + 8(e/4) +
+// 4(l/32) + e%4 here (the two 4-element runs of the 8-k form's layout side by side).
 //
 // Split-K: gridDim.z = batch * splits; with splits > 1 each block writes its
 // partial tile to a workspace and a second kernel adds the splits in a fixed
@@ -29,7 +34,20 @@
 namespace wfs {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// the k values of two 4-element runs as one 8-element operand
+__device__ __forceinline__ s16x8 cat8(const bf16x4& a, const bf16x4& b) {
+  return s16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+}
+
+// C += A B over 16 k on the bf16 matrix cores (v_mfma_f32_32x32x16_bf16)
+__device__ __forceinline__ f32x16 mfma16(const s16x8& a, const s16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                  0, 0);
+}
 
 // round to nearest even (gfx950 converts in hardware: v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short f32_to_bf16(float f) {
@@ -168,26 +186,34 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(GemmArgs g) {
     const int col = wn * 32 + (lane & 31);
     const int kq = 4 * (lane >> 5);
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 8) {
-      bf16x4 a, bb;
+    for (int kk = 0; kk < kBK; kk += 16) {
+      s16x8 a, bb;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         a[j] = (short)As[row][kk + kq + j];
+        a[4 + j] = (short)As[row][kk + 8 + kq + j];
         bb[j] = (short)Bs[col][kk + kq + j];
+        bb[4 + j] = (short)Bs[col][kk + 8 + kq + j];
       }
       if constexpr (SA) {
-        bf16x4 al;
+        s16x8 al;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) al[j] = (short)Al[row][kk + kq + j];
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bb, acc, 0, 0, 0);
+        for (int j = 0; j < 4; ++j) {
+          al[j] = (short)Al[row][kk + kq + j];
+          al[4 + j] = (short)Al[row][kk + 8 + kq + j];
+        }
+        acc = mfma16(al, bb, acc);
       }
       if constexpr (SB) {
-        bf16x4 bl;
+        s16x8 bl;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bl[j] = (short)Bl[col][kk + kq + j];
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, acc, 0, 0, 0);
+        for (int j = 0; j < 4; ++j) {
+          bl[j] = (short)Bl[col][kk + kq + j];
+          bl[4 + j] = (short)Bl[col][kk + 8 + kq + j];
+        }
+        acc = mfma16(a, bl, acc);
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bb, acc, 0, 0, 0);
+      acc = mfma16(a, bb, acc);
     }
     __syncthreads();
   }
@@ -234,14 +260,15 @@ __global__ __launch_bounds__(256) void gemm_splitk_reduce_kernel(GemmArgs g) {
 // split into bf16 hi + lo; products accumulate in FP32 (in another order).
 //
 // One wavefront per line, no LDS and no barriers.  The transposed product is formed,
-// u^T = v^T W^T + Gz^T Z^T, so that (with the 32 x 32 x 8 matrix-core layout: lane l
-// holds A[l%32][4(l/32)..+3], B[4(l/32)..+3][l%32] and accumulator item i is
-// C[8(i/4)+4(l/32)+i%4][l%32]) every operand comes straight from memory or from the
-// previous product: the A fragment of v^T is four channel rows at one z (each load
-// instruction reads 2 x 128 contiguous bytes), W^T, Gz^T and Fz are per-lane
-// constants held in registers, Z^T is one 16-byte load per k step, and the
-// accumulator of a 32-point z chunk of u^T, items 4q..4q+3, is already the A
-// fragment of v' for k step q of T = v' Fz.  C, cin, tm <= 32.
+// u^T = v^T W^T + Gz^T Z^T, so that (in 8-k runs: lane l holds A[l%32][8s+4(l/32)..+3],
+// B[8s+4(l/32)..+3][l%32], two runs s = 2S, 2S + 1 per 16-k instruction, and
+// accumulator item i is C[8(i/4)+4(l/32)+i%4][l%32]) every operand comes straight from
+// memory or from the previous product: the A fragment of v^T is four channel rows at one
+// z (each load instruction reads 2 x 128 contiguous bytes), W^T, Gz^T and Fz are per-lane
+// constants held in registers, Z^T is one 16-byte load per run, and the accumulator of a
+// 32-point z chunk of u^T, items 4q..4q+3, is already the A run of v' for the z run q of
+// T = v' Fz.  A chunk takes 12 matrix instructions (W v, Z Gz and v' Fz, two each per
+// 16 k for the hi and lo parts).  C, cin, tm <= 32.
 // ---------------------------------------------------------------------------
 struct LayerArgs {
   const float* W;
@@ -353,47 +380,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
       f32x16 acc;
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+      // k steps of 16: the 8-k steps 2S and 2S + 1 side by side (past cin and tm the
+      // operands are zero)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        if (s < kin) {
-          bf16x4 ah, al;
-          float xs[4] = {x[4 * s], x[4 * s + 1], x[4 * s + 2], x[4 * s + 3]};
-          split4(xs, ah, al);
-          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, Wf[s], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, Wf[s], acc, 0, 0, 0);
+      for (int S = 0; S < 2; ++S) {
+        if (2 * S < kin) {
+          bf16x4 ah0, al0, ah1, al1;
+          float x0[4] = {x[8 * S], x[8 * S + 1], x[8 * S + 2], x[8 * S + 3]};
+          float x1[4] = {x[8 * S + 4], x[8 * S + 5], x[8 * S + 6], x[8 * S + 7]};
+          split4(x0, ah0, al0);
+          split4(x1, ah1, al1);
+          const s16x8 w = cat8(Wf[2 * S], Wf[2 * S + 1]);
+          acc = mfma16(cat8(al0, al1), w, acc);
+          acc = mfma16(cat8(ah0, ah1), w, acc);
         }
       }
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        if (s < ktm) {
-          bf16x4 a;
+      for (int S = 0; S < 2; ++S) {
+        if (2 * S < ktm) {
+          s16x8 a;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) a[j] = (short)GzT[32 * ch + lc][8 * s + kq + j];
-          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, Zl[s], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, Zh[s], acc, 0, 0, 0);
+          for (int j = 0; j < 4; ++j) {
+            a[j] = (short)GzT[32 * ch + lc][16 * S + kq + j];
+            a[4 + j] = (short)GzT[32 * ch + lc][16 * S + 8 + kq + j];
+          }
+          acc = mfma16(a, cat8(Zl[2 * S], Zl[2 * S + 1]), acc);
+          acc = mfma16(a, cat8(Zh[2 * S], Zh[2 * S + 1]), acc);
         }
       }
-      // lane: column c = lc; items 4q..4q+3: z = 32 ch + 8 q + kq + 0..3
+      // lane: column c = lc; items 4q..4q+3: z = 32 ch + 8 q + kq + 0..3; the items of q = 2Q
+      // and 2Q + 1 are the A operand of one 16-k step of T = v' Fz
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float val[4], gv[4];
+      for (int Q = 0; Q < 2; ++Q) {
+        bf16x4 ahq[2], alq[2], bq[2];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          val[j] = acc[4 * q + j] + bias_c;
-          gv[j] = lc < C ? (g.act ? gelu_f(val[j]) : val[j]) : 0.0f;
-        }
-        if (lc < C) {
-          const unsigned off = (unsigned(lc) * P + line + 32 * ch + 8 * q + kq) * 4u;
-          if (ub) *reinterpret_cast<float4*>(ub + off) = make_float4(val[0], val[1], val[2], val[3]);
-          if (nb) *reinterpret_cast<float4*>(nb + off) = make_float4(gv[0], gv[1], gv[2], gv[3]);
+        for (int h = 0; h < 2; ++h) {
+          const int q = 2 * Q + h;
+          float val[4], gv[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            val[j] = acc[4 * q + j] + bias_c;
+            gv[j] = lc < C ? (g.act ? gelu_f(val[j]) : val[j]) : 0.0f;
+          }
+          if (lc < C) {
+            const unsigned off = (unsigned(lc) * P + line + 32 * ch + 8 * q + kq) * 4u;
+            if (ub) *reinterpret_cast<float4*>(ub + off) = make_float4(val[0], val[1], val[2], val[3]);
+            if (nb) *reinterpret_cast<float4*>(nb + off) = make_float4(gv[0], gv[1], gv[2], gv[3]);
+          }
+          if (tb) {
+            split4(gv, ahq[h], alq[h]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bq[h][j] = (short)FzT[lc][32 * ch + 8 * q + kq + j];
+          }
         }
         if (tb) {
-          bf16x4 ah, al, b;
-          split4(gv, ah, al);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) b[j] = (short)FzT[lc][32 * ch + 8 * q + kq + j];
-          accT = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, b, accT, 0, 0, 0);
-          accT = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, b, accT, 0, 0, 0);
+          const s16x8 b = cat8(bq[0], bq[1]);
+          accT = mfma16(cat8(alq[0], alq[1]), b, accT);
+          accT = mfma16(cat8(ahq[0], ahq[1]), b, accT);
         }
       }
     };
