@@ -116,13 +116,13 @@ def run_chunks(plan, psi, V, boxes, steps, out=None):
     return plan.step_observe(psi, V, steps, OBS_EVERY, boxes, out=out)
 
 
-def gpu_bench(steps, warmup):
+def gpu_bench(steps, warmup, autotune=True):
     import torch
     from wavefunctionsimpython_amd import _abi
     dev = torch.device("cuda:0")
     shape, dx, psi, V, boxes = build_workload(dev)
     plan = _abi.Plan(shape, dx, DT, 1.0, _abi.WFS_C64, _abi.WFS_METHOD_FOURIER)
-    variants = plan.autotune()
+    variants = plan.autotune() if autotune else plan.variants()
     plan.set_potential(V)          # phase table cexp(-i dt V), kept by the plan
     nobs = max((max(steps, warmup, 40) + OBS_EVERY - 1) // OBS_EVERY, 1)
     out = torch.zeros((nobs, len(boxes), 7), dtype=torch.float64, device=dev)
@@ -259,6 +259,8 @@ def batch_bench(steps, warmup):
     from wavefunctionsimpython_amd.batch import PacketBatch, random_packets
     specs = random_packets(48, seed=0)
     batch = PacketBatch(specs)
+    # one packet per grid (the first of its class) is checked against its complex128 run
+    probes = {c.key[0][0]: (c.members[0], batch.packet_state(c.members[0]).clone()) for c in batch.classes}
     batch.run(warmup)
     torch.cuda.synchronize()
     g = batch.capture(steps)
@@ -268,6 +270,16 @@ def batch_bench(steps, warmup):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     pts = batch.total_points()
+    acc = {}
+    for grid, (i, psi0) in probes.items():
+        sp = specs[i]
+        p64 = _abi.Plan(sp.n, sp.dx, sp.dt, sp.mass, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+        p64.set_potential_axes(*_abi.separable_axes(sp.n, sp.dx, sp.omega, sp.mass, (0.0, 0.0, 0.0),
+                                                    sp.lattice_depth, sp.lattice_period))
+        ref = psi0.to(torch.complex128)
+        p64.step(ref, None, warmup + steps)
+        acc[str(grid)] = accuracy(batch.packet_state(i), ref)
+        del p64, ref
     # baseline: every packet with its own plan, launched eagerly one after another
     seq_steps = max(10, steps // 10)
     plans = []
@@ -296,6 +308,9 @@ def batch_bench(steps, warmup):
                                                  if k in ("yz_plane", "yz_slab", "x_slab")} for c in batch.classes},
         "unbatched_eager_single_step_calls": {"value": pts * seq_steps / seq, "unit": "grid-pt-steps/s",
                                               "steps": seq_steps},
+        "accuracy": {"rel_l2_vs_c128_max": max(a["rel_l2_vs_c128"] for a in acc.values()),
+                     "phase_err_rad_max": max(abs(a["phase_err_rad"]) for a in acc.values()),
+                     "per_grid_first_packet": acc, "steps": warmup + steps},
     }
 
 
@@ -504,6 +519,68 @@ def dropin_config(tmp, observe_every):
     }
 
 
+def fno_bench(steps, warmup):
+    """Inference forward passes of the parareal workload's Fourier neural operator (4 layers,
+    12 modes, width 32, 5 input planes) on the full 128^3 grid and on the 32^3 grid of the
+    coarse-grid propagator, random inputs and weights.  Each size is checked against the same
+    network run through its unfused GEMM path (same operand rounding, another summation order),
+    which runs first, so that every launch after the first fused layer belongs to the timed path."""
+    import torch
+    from wavefunctionsimpython_amd.fno import FNO3d
+    dev = torch.device("cuda", 0)
+    grids = (128, 32)
+
+    def make(n, fused):
+        return FNO3d(n, width=32, modes=12, layers=4, in_channels=5, output_scale=0.1, residual_scale=1e-3,
+                     device=dev, fused=fused)
+
+    inputs, refs = {}, {}
+    for n in grids:
+        gen = torch.Generator(device=dev).manual_seed(n)
+        net = make(n, False)
+        a = torch.randn(net.cin * net.P, generator=gen, device=dev)
+        y = net.forward(a)
+        # the network's own part of the output (the residual planes of the input taken off)
+        refs[n] = y - a[: y.numel()]
+        inputs[n] = a
+        del net
+    torch.cuda.synchronize()
+    out = {}
+    for n in grids:
+        net = make(n, True)
+        a = inputs[n]
+        for _ in range(warmup):
+            net.forward(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            y = net.forward(a)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        y = y - a[: y.numel()]
+        diff = float((y - refs[n]).norm() / refs[n].norm())
+        cost = net.forward_cost()
+        out[str(n)] = {"forward_ms": t * 1e3, "forwards_per_s": 1.0 / t,
+                       "mfma_tflops": cost["flops"] / t / 1e12, "hbm_GBs": cost["hbm_bytes"] / t / 1e9,
+                       "hbm_bytes_per_forward": cost["hbm_bytes"], "flops_per_forward": cost["flops"],
+                       "rel_diff_vs_gemm_path": diff}
+        del net
+        torch.cuda.empty_cache()
+    return {
+        "metric": "FNO inference forward passes/s at 128^3 (width 32, 12 modes, 4 layers)",
+        "value": out["128"]["forwards_per_s"], "unit": "forwards/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": out["128"]["forward_ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 matrix cores (activations hi + lo), f32 accumulation",
+        "data": "synthetic",
+        "config": {"workload": "FNO forward, 5 -> 2 planes, width 32, 12 modes, 4 layers, grids 128^3 and 32^3"},
+        "grids": out,
+        "roofline": {"bound": "hbm", "achieved": out["128"]["hbm_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": out["128"]["hbm_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "note": "minimal activation bytes of the fused forward over its time; the matrix cores run "
+                             "at mfma_tflops of 2500"},
+    }
+
+
 def dropin_bench(steps, warmup):
     """The drop-in surface at the reference's settings: initialisation.initialize + core_sim.run_iteration
     (SimState, MeasurementTools; dynamic potential mode, the reference default) on the 256^3 double
@@ -513,7 +590,7 @@ def dropin_bench(steps, warmup):
     import shutil
     import tempfile
     import torch
-    from wavefunctionsimpython_amd import core_sim, initialisation
+    from wavefunctionsimpython_amd import _abi, core_sim, initialisation
     tmp = tempfile.mkdtemp(prefix="wfs_dropin_")
     npts = N ** 3
     res = {}
@@ -522,6 +599,7 @@ def dropin_bench(steps, warmup):
             for every in (1, 10):
                 cfg = dropin_config(tmp, every)
                 sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
+                psi0 = sim.get_wave_function().clone()
                 it.total_iteration_count = warmup
                 core_sim.run_iteration(sim, tools, it, observe_every=every, exit_on_quit=False)
                 torch.cuda.synchronize()
@@ -531,9 +609,18 @@ def dropin_bench(steps, warmup):
                 torch.cuda.synchronize()
                 el = time.perf_counter() - t0
                 samples = tools.get_expected_location().get_expected_location_evolution().shape[0]
+                # the final state against the complex128 plan stepped as often with the same V
+                p64 = _abi.Plan(tuple(sim.get_number_of_voxels_3()), sim.get_delta_x_bohr_radii_3(),
+                                sim.get_delta_time_h_bar_per_hartree(), sim.get_particle_mass(), _abi.WFS_C128,
+                                _abi.WFS_METHOD_FOURIER)
+                ref = psi0.to(torch.complex128)
+                del psi0
+                p64.set_potential(sim.get_potential())
+                p64.step(ref, None, warmup + steps)
                 res[f"observe_every_{every}"] = {"value": npts * steps / el, "ms_per_step": el / steps * 1e3,
-                                                 "samples": samples}
-                del sim, tools, it
+                                                 "samples": samples,
+                                                 "accuracy": accuracy(sim.get_wave_function(), ref)}
+                del sim, tools, it, p64, ref
                 torch.cuda.empty_cache()
             # the reference's loop shape, one time_step per iteration
             cfg = dropin_config(tmp, 1)
@@ -563,6 +650,7 @@ def dropin_bench(steps, warmup):
                                "(expected location + 2 volume probabilities), dynamic potential mode",
                    "grid": [N, N, N], "dt": DT},
         "runs": res,
+        "accuracy": res["observe_every_1"]["accuracy"],
     }
 
 
@@ -594,10 +682,6 @@ def large_bench(steps):
     out["fourier_c128"] = {"seconds": time.perf_counter() - t0,
                            "grid_pt_steps_per_s": npts * steps / (time.perf_counter() - t0)}
     del p64
-    norm_ref = float(ref.abs().pow(2).sum().sqrt())
-
-    def rel(a):
-        return float((a.to(torch.complex128) - ref).abs().pow(2).sum().sqrt()) / norm_ref
 
     for name, method in (("fourier", _abi.WFS_METHOD_FOURIER), ("power_series", _abi.WFS_METHOD_POWER_SERIES)):
         plan = _abi.Plan(shape, dx, dt, 1.0, _abi.WFS_C64, method, order=10)
@@ -612,8 +696,9 @@ def large_bench(steps):
         plan.step(psi, V, steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        a = accuracy(psi, ref)
         out[name] = {"seconds": el, "grid_pt_steps_per_s": npts * steps / el, "ms_per_step": el / steps * 1e3,
-                     "rel_l2_vs_fourier_c128": rel(psi)}
+                     "rel_l2_vs_fourier_c128": a["rel_l2_vs_c128"], "phase_err_rad_vs_fourier_c128": a["phase_err_rad"]}
         print(f"{name}: {el:.2f} s", file=sys.stderr, flush=True)
         del plan
         if method == _abi.WFS_METHOD_POWER_SERIES:
@@ -621,8 +706,9 @@ def large_bench(steps):
             p64 = _abi.Plan(shape, dx, dt, 1.0, _abi.WFS_C128, method, order=10)
             own = psi0.to(torch.complex128)
             p64.step(own, V, steps)
-            d = float((psi.to(torch.complex128) - own).abs().pow(2).sum().sqrt() / own.abs().pow(2).sum().sqrt())
-            out[name]["rel_l2_vs_power_series_c128"] = d
+            a = accuracy(psi, own)
+            out[name]["rel_l2_vs_power_series_c128"] = a["rel_l2_vs_c128"]
+            out[name]["phase_err_rad_vs_power_series_c128"] = a["phase_err_rad"]
             del p64, own
         del psi
         torch.cuda.empty_cache()
@@ -646,12 +732,16 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal", "large", "dropin"],
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the plan's default tile variants instead of timing them first (profiling runs: "
+                         "the same kernels in every process)")
+    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal", "large", "dropin", "fno"],
                     default="splitstep256",
                     help="splitstep256 (default, the headline metric), batch (48 packets, one hipGraph), "
                          "parareal (time-to-T on the 128^3 lattice, FNO training included), large (512^3, "
                          "split-step and power series against complex128; --steps, 2000 in the config) or dropin "
-                         "(the 256^3 double slit through initialisation + core_sim.run_iteration)")
+                         "(the 256^3 double slit through initialisation + core_sim.run_iteration) or fno (the "
+                         "FNO forward at 128^3 and 32^3)")
     ap.add_argument("--train-pairs", type=int, default=2000, help="FNO training pairs (parareal workload)")
     ap.add_argument("--coarse-n", type=int, default=32, help="grid of the coarse-grid FNO (parareal workload)")
     ap.add_argument("--full-fno", action="store_true", help="also run the FNO on the full 128^3 grid (parareal)")
@@ -687,7 +777,10 @@ def main():
     if args.workload == "dropin":
         print(json.dumps(dropin_bench(args.steps, args.warmup)), flush=True)
         return
-    g = gpu_bench(args.steps, args.warmup)
+    if args.workload == "fno":
+        print(json.dumps(fno_bench(args.steps, args.warmup)), flush=True)
+        return
+    g = gpu_bench(args.steps, args.warmup, autotune=not args.no_autotune)
     value = g["npts"] * args.steps / g["elapsed"]
     dom = g["dominant"]
     k = g["kernels"][dom]

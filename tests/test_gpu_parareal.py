@@ -81,10 +81,13 @@ def test_parareal_fallback_gives_serial_fine_exactly(dev):
 
 def test_trained_fno_gains_over_coarse_without_it(dev):
     """Train the FNO correction of a one-step split-step on solver pairs.  On a held-out
-    problem it must lower its base's slice error and first-iteration parareal error, and
-    parareal with it must reach 1e-4 in fewer iterations than with the identity.  (Its
-    iteration count against the base alone is printed, not asserted: the convergence rate
-    follows the mismatch of the Jacobians, which the correction does not reduce.)"""
+    problem, against that split-step alone (not the identity), it must lower the slice
+    error by at least a fifth and the first parareal iteration's change; parareal with it
+    must reach 1e-4 against the serial fine run, in fewer iterations than with the identity.
+    The iteration count against the base is printed and not asserted either way: parareal's
+    rate follows the mismatch of the Jacobians of the fine and coarse maps, which a
+    correction trained on states does not lower (measured: base 5, FNO 6 at this size; both
+    2 on the 128^3 workload)."""
     S, N = 200, 8
     rng = np.random.default_rng(1)
     probs = [pr.random_lattice_problem(rng, n=N_GRID, dx=DX, k0=0.5, r0=2.0) for _ in range(16)]
@@ -114,7 +117,6 @@ def test_trained_fno_gains_over_coarse_without_it(dev):
     assert np.mean(e_fno) < 0.8 * np.mean(e_base)
     assert runs["fno"].history[0] < runs["base"].history[0]
     assert runs["fno"].converged and runs["fno"].iterations < runs["identity"].iterations
-    assert runs["fno"].iterations <= runs["base"].iterations + 1
     assert _rel(runs["fno"].state, serial[-1]) < 1e-4
 
 
