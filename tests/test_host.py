@@ -160,3 +160,28 @@ def test_config_file_pick(tmp_path, monkeypatch):
     assert initialisation.select_config_file(str(d)).endswith("only.toml")     # sorted: another, only
     with pytest.raises(SystemExit):
         initialisation.select_config_file(str(tmp_path / "missing"))
+
+
+def test_bench_distinct_plane_share_and_grouped_observable_error():
+    """bench.py's byte model reads the share of x planes that start a run of bitwise-equal
+    planes; its observable error scales each moment by its group (P, first moments, second
+    moments), so a moment that vanishes by symmetry does not blow the error up."""
+    import torch
+    sys.path.insert(0, ROOT)
+    import bench
+    V = torch.zeros((8, 4, 4), dtype=torch.complex64)
+    V[3] += 1.0
+    V[4] += 1.0                      # same bits as plane 3: no new run
+    V[6, 0, 0] = 1j                  # one voxel differs from plane 5
+    V[7] = V[6]
+    V[7, 3, 3] = complex(-0.0, 0.0)  # -0.0 differs from 0.0 in its bits
+    assert torch.signbit(V[7, 3, 3].real)
+    assert bench.distinct_plane_fraction(V) == 5 / 8      # runs start at planes 0, 3, 5, 6 and 7
+    ref = torch.zeros((5, 3, 7), dtype=torch.float64)
+    ref[..., 0], ref[..., 1], ref[..., 2] = 1.0, 16.0, 1e-17
+    ref[..., 4], ref[..., 5], ref[..., 6] = 300.0, 20.0, 20.0
+    obs = ref.clone()
+    obs[..., 2] += 1e-9                                   # noise on a vanishing first moment
+    assert bench.observables_rel_err(obs, ref) == pytest.approx(1e-9 / 16.0)
+    obs[..., 4] += 3e-4
+    assert bench.observables_rel_err(obs, ref) == pytest.approx(1e-6)
