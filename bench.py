@@ -158,10 +158,12 @@ def gpu_bench(steps, warmup, autotune=True):
     plan.set_profiling(False)
     kernels = {}
     npts = shape[0] * shape[1] * shape[2]
-    # the y-z launch reads psi and writes it (16 B/point) and reads each distinct x plane of the
-    # phase table once (8 B/point times the distinct share; repeated planes come from the caches)
+    # algorithmic bytes follow the step's model (psi read and written per launch, V read once per
+    # step: 24 B/point for the y-z launch); with shared table planes the y-z launch fetches only
+    # the distinct x planes of the phase table from HBM (the share is reported, and the measured
+    # traffic falls below the algorithmic bytes by it)
     table_share = distinct_plane_fraction(V) if plan.variants()["pv_planes"] else 1.0
-    per_point = dict(BYTES_PER_POINT, yz_slab=16.0 + 8.0 * table_share)
+    per_point = BYTES_PER_POINT
     for name, (launches, ms) in prof.items():
         avg_ms = ms / launches
         rec = {"launches": launches, "avg_us": round(avg_ms * 1e3, 2)}
@@ -790,12 +792,11 @@ def main():
                 "algorithmic_bytes_per_launch": k["algorithmic_bytes"]}
     cpu = None if args.no_cpu_baseline else cpu_baseline(g["V_host"])
     ms = g["elapsed"] / args.steps * 1e3
-    # whole-step rates from the timed ms_per_step: the slab scheme's algorithmic 32 B per point and
-    # step for the state, 8 B times the distinct share of the phase table's x planes, plus 32 per
-    # sample (the fused observing launch and the split half kick's second x launch), and the pass
-    # model of four-FFT split steps (2 transforms x 3 axis passes x 16 B + 8 B of V = 104 B per
-    # step, + 96 B per sampled step)
-    step_bytes = 32.0 + 8.0 * g["table_share"] + 32.0 / OBS_EVERY
+    # whole-step rates from the timed ms_per_step: the slab scheme's algorithmic 40 B per point and
+    # step (32 for the state, 8 for V) plus 32 per sample (the fused observing launch and the split
+    # half kick's second x launch), and the pass model of four-FFT split steps (2 transforms x 3
+    # axis passes x 16 B + 8 B of V = 104 B per step, + 96 B per sampled step)
+    step_bytes = 40.0 + 32.0 / OBS_EVERY
     pass_bytes = 104.0 + 96.0 / OBS_EVERY
     step_gbs = step_bytes * g["npts"] / (ms * 1e-3) / 1e9
     pass_gbs = pass_bytes * g["npts"] / (ms * 1e-3) / 1e9
