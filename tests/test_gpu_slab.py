@@ -304,3 +304,41 @@ def test_shared_potential_planes_bit_identical(dev, form, batch):
         assert _rel(out[1][0].cpu().numpy(), ref) < TOL_ORACLE
     with pytest.raises(_abi.WfsError):
         p.set_variant("pv_planes", 2)
+
+
+def test_shared_planes_map_is_not_built_under_capture(dev):
+    """Plane sharing switched on after the table was built, then a capture: the captured steps
+    read every plane's own copy (the map is not built inside a capture, where its kernels would
+    run only at replay), the first uncaptured call after the capture builds it, and every
+    result equals the eager runs of a plan that shared from the start, bit for bit."""
+    shape = (32, 256, 256)
+    V = torch.from_numpy(_repeating_planes(shape, 1, 79)).to(dev)
+    psi0, _ = _inputs(shape, 80)
+    start = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+    p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    p.set_variant("pv_planes", 0)
+    p.set_potential(V)
+    p.set_variant("pv_planes", 1)
+    ref = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    ref.set_potential(V)
+    graphed = start.clone()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with _abi.Graph(s) as g:
+        p.step(graphed, None, 3, stream=s)           # table copies 0..2
+    eager = start.clone()
+    p.step(eager, None, 3)                           # uncaptured: builds the map; copies 3..5
+    want = start.clone()
+    ref.set_step_index(3)
+    ref.step(want, None, 3)
+    torch.cuda.synchronize()
+    print("case: shared planes switched on, capture, then an uncaptured call", flush=True)
+    assert torch.equal(eager, want)
+    want = start.clone()
+    for replay in range(2):
+        g.launch()
+        s.synchronize()
+        ref.set_step_index(0)                        # a replay uses the copies of its capture
+        ref.step(want, None, 3)
+        torch.cuda.synchronize()
+        assert torch.equal(graphed, want), f"replay {replay}"

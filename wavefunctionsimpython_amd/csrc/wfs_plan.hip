@@ -295,13 +295,17 @@ int alloc_table(wfs_plan* p, void** t, hipStream_t s) {
 }
 
 // the map of x planes of pv_lines that repeat the previous plane (kernel class WFS_K_PV_PLANES,
-// variant 1): built with the lines table, so that graph capture finds it ready
+// variant 1): built with the lines table, so that graph capture finds it ready.  Under capture
+// it is not built (its kernels would run only at replay, and an uncaptured call before that
+// would read an unwritten map): the captured steps read every plane's own copy, and the next
+// uncaptured call builds it.
 int share_planes(wfs_plan* p, hipStream_t s) {
   p->pv_planes_ok = false;
   if (p->variant[WFS_K_PV_PLANES] == 0) return 0;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone) return 0;
   const long planes = long(p->batch) * p->n[0];
   if (!p->pv_planes) {
-    WFS_TRY(refuse_if_capturing(s, "the potential plane map"));
     if (hipMalloc(&p->pv_planes, size_t(2 * planes) * sizeof(int)) != hipSuccess) {
       p->pv_planes = nullptr;
       set_last_error("hipMalloc for the potential plane map failed");
