@@ -342,3 +342,46 @@ def test_shared_planes_map_is_not_built_under_capture(dev):
         ref.step(want, None, 3)
         torch.cuda.synchronize()
         assert torch.equal(graphed, want), f"replay {replay}"
+
+
+def test_table_reorder_inside_a_capture(dev):
+    """The lines-ordered table holds an older potential, the new one is set under the four
+    passes, and the plan goes back to the slab scheme inside a capture: the reorder is captured
+    (it runs at every replay) and not taken as done, so the first uncaptured call reorders
+    again instead of stepping with the older potential; replays and that call equal a plan
+    that used the slab scheme from the start, bit for bit."""
+    shape = (8, 256, 256)
+    psi0, V = _inputs(shape, 81)
+    _, V_old = _inputs(shape, 82)
+    Vd = torch.from_numpy(V).to(dev)
+    start = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+    p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    p.set_variant("yz_slab", 2)
+    p.set_potential(torch.from_numpy(V_old).to(dev))       # the lines table exists, older V
+    p.set_variant("yz_slab", 0)
+    p.set_potential(Vd)
+    p.set_variant("yz_slab", 2)
+    direct = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    direct.set_variant("yz_slab", 2)
+    direct.set_potential(Vd)
+    graphed = start.clone()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with _abi.Graph(s) as g:
+        p.step(graphed, None, 3, stream=s)
+    eager = start.clone()
+    p.step(eager, None, 3)
+    want = start.clone()
+    direct.set_step_index(3)
+    direct.step(want, None, 3)
+    torch.cuda.synchronize()
+    print("case: table reorder captured, then an uncaptured call and two replays", flush=True)
+    assert torch.equal(eager, want)
+    want = start.clone()
+    for replay in range(2):
+        g.launch()
+        s.synchronize()
+        direct.set_step_index(0)
+        direct.step(want, None, 3)
+        torch.cuda.synchronize()
+        assert torch.equal(graphed, want), f"replay {replay}"

@@ -170,6 +170,12 @@ void kinetic_table(long n, double dx, double dt, double mass, double f, std::vec
 
 // Lazy allocations cannot be recorded into a graph: a call that would allocate on a stream under
 // capture fails with a message instead (the caller makes one uncaptured call first).
+// stream s is being captured into a graph (its launches run only at replay)
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 int refuse_if_capturing(hipStream_t s, const char* what) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone) {
@@ -302,8 +308,7 @@ int alloc_table(wfs_plan* p, void** t, hipStream_t s) {
 int share_planes(wfs_plan* p, hipStream_t s) {
   p->pv_planes_ok = false;
   if (p->variant[WFS_K_PV_PLANES] == 0) return 0;
-  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-  if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone) return 0;
+  if (capturing(s)) return 0;
   const long planes = long(p->batch) * p->n[0];
   if (!p->pv_planes) {
     if (hipMalloc(&p->pv_planes, size_t(2 * planes) * sizeof(int)) != hipSuccess) {
@@ -341,7 +346,8 @@ int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
 }
 
 // the table the next steps read: C order for the axis passes and the plane pass, lines order
-// for the slab scheme
+// for the slab scheme.  A reorder issued under graph capture runs only at replay, so it does
+// not mark the other order current: the next uncaptured call reorders again.
 int current_table(wfs_plan* p, hipStream_t s, const void** out) {
   *out = nullptr;
   if (p->sep_set) return 0;
@@ -350,7 +356,7 @@ int current_table(wfs_plan* p, hipStream_t s, const void** out) {
     if (!p->pv_lines_ok) {
       WFS_TRY(alloc_table(p, &p->pv_lines, s));
       WFS_TRY(launch_table_reorder(p->precision, count, p->pv, p->pv_lines, slab_lines_order(p->n[1]), true, s));
-      p->pv_lines_ok = true;
+      p->pv_lines_ok = !capturing(s);
       p->pv_planes_ok = false;
     }
     if (!p->pv_planes_ok && p->variant[WFS_K_PV_PLANES] != 0) WFS_TRY(share_planes(p, s));
@@ -359,7 +365,7 @@ int current_table(wfs_plan* p, hipStream_t s, const void** out) {
     if (!p->pv_nat_ok) {
       WFS_TRY(alloc_table(p, &p->pv, s));
       WFS_TRY(launch_table_reorder(p->precision, count, p->pv_lines, p->pv, slab_lines_order(p->n[1]), false, s));
-      p->pv_nat_ok = true;
+      p->pv_nat_ok = !capturing(s);
     }
     *out = p->pv;
   }
