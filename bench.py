@@ -198,7 +198,23 @@ def gpu_bench(steps, warmup, autotune=True):
         "horner_launch_avg_us": round(hms / hl * 1e3, 2),
         "horner_launch_GBs": round(32 * npts / (hms / hl * 1e-3) / 1e9, 1),
     }
+    # the 64^3 complex128 free packet (sigma 3, v = (1, 0, 0), box 60, dt 0.01), 200 steps
+    n1 = 64
+    s1, dx1 = (n1,) * 3, (60.0 / n1,) * 3
+    p1 = _abi.Plan(s1, dx1, 0.01, 1.0, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+    psi1 = torch.empty(s1, dtype=torch.complex128, device=dev)
+    _abi.init_gaussian(s1, dx1, _abi.WFS_C128, (3.0, 3.0, 3.0), (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), psi1)
+    p1.set_potential(torch.zeros(s1, dtype=torch.complex64, device=dev))
+    p1.step(psi1, None, 2)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    p1.step(psi1, None, 200)
+    torch.cuda.synchronize()
+    e3 = time.perf_counter() - t3
+    free_packet = {"value": n1 ** 3 * 200 / e3, "unit": "grid-pt-steps/s", "ms_per_step": e3 / 200 * 1e3,
+                   "steps": 200, "dtype": "f64"}
     return {
+        "free_packet_64_c128": free_packet,
         "elapsed": elapsed, "npts": npts, "norm": norm, "kernels": kernels, "dominant": dominant, "accuracy": acc,
         "power_series": power_series, "V_host": V.cpu().numpy(), "table_share": table_share,
         "variants": {k: v for k, v in variants.items() if k in prof},
@@ -221,11 +237,13 @@ def read_traffic(kernel):
     return None
 
 
-def cpu_baseline(V, seconds_target=15.0):
+def cpu_baseline(V, seconds_target=10.0):
     """Complex128 oracle of the reference's 4-FFT split-step on the host cores.
 
     V is the workload's complex64 potential (host copy); the initial state is
-    the same Gaussian packet evaluated by the oracle.
+    the same Gaussian packet evaluated by the oracle.  Beside it (`others`): the
+    64^3 free packet of the complex128 configuration (200 split steps) and one
+    256^3 power-series step (p = 10, numpy, one thread) on the same potential.
     """
     import numpy as np
     import scipy
@@ -245,11 +263,34 @@ def cpu_baseline(V, seconds_target=15.0):
         if time.perf_counter() - t0 > seconds_target or steps >= 50:
             break
     el = time.perf_counter() - t0
+    others = {}
+    # 64^3, complex128, V = 0: sigma 3, v = (1, 0, 0), box 60, dt 0.01
+    n1 = 64
+    s1, dx1 = (n1,) * 3, (60.0 / n1,) * 3
+    V1 = np.zeros(s1, np.complex64)
+    p1 = oracle.gaussian_wave_packet(s1, dx1, (3, 3, 3), (0, 0, 0), (1, 0, 0), np.complex128)
+    ph1 = (oracle.kinetic_half_phase(s1, dx1, 0.01, 1.0), oracle.potential_phase(V1, 0.01))
+    for _ in range(2):
+        p1 = oracle.split_step(p1, V1, dx1, 0.01, 1.0, nsteps=1, workers=cores, phases=ph1)
+    t1 = time.perf_counter()
+    for _ in range(200):
+        p1 = oracle.split_step(p1, V1, dx1, 0.01, 1.0, nsteps=1, workers=cores, phases=ph1)
+    e1 = time.perf_counter() - t1
+    others["free_packet_64_splitstep_c128"] = {
+        "value": n1 ** 3 * 200 / e1, "unit": "grid-pt-steps/s", "cores": cores, "ms_per_step": e1 / 200 * 1e3,
+        "sample": "200 split-steps after 2 warm-up steps"}
+    t2 = time.perf_counter()
+    oracle.power_series_step(psi, V, DT, dx, 1.0, order=10, nsteps=1)
+    e2 = time.perf_counter() - t2
+    others["double_slit_256_power_series_p10"] = {
+        "value": n ** 3 / e2, "unit": "grid-pt-steps/s", "cores": 1, "ms_per_step": e2 * 1e3,
+        "sample": "1 step of the reference recurrence (10 stencil stages, numpy, complex128)"}
     return {
         "value": n ** 3 * steps / el, "unit": "grid-pt-steps/s", "cores": cores, "kind": "port",
         "cpu_model": cpu_model(),
         "sample": f"{steps} split-steps (4 scipy FFTs each, complex128) of the 256^3 double-slit workload "
-                  f"after 1 warm-up step, scipy {scipy.__version__}",
+                  f"after 1 warm-up step, scipy {scipy.__version__}, numpy {np.__version__}",
+        "others": others,
     }
 
 
@@ -819,6 +860,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": g["kernels"],
         "power_series": g["power_series"],
+        "free_packet_64_c128": g["free_packet_64_c128"],
         "tile_variants": g["variants"],
         "norm_after_warmup": g["norm"],
     }
