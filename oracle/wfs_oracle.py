@@ -21,8 +21,15 @@ below; the pins live in ``tests/test_oracle_*.py``.
                                  unitarity, 2nd-order convergence, Ehrenfest
   power_series_step              dense-H brute force (matrix powers), expm limit,
                                  Horner identity
+  power_series_horner            equal to power_series_step to rounding (p = 1, 2, 10)
+  laplacian_clamped              dense 7-point matrix with edge-replicated faces
   simpson_weights_1d / volume_probability / expected_location
                                  exact for x^a y^b z^c (a,b,c<=3), Gaussian moments
+  standard_deviation             Gaussian sigma from its first two moments
+  plane_current / plane_sample_count
+                                 plane wave flux |A|^2 k/m (tilted normal, mass),
+                                 exact for degree-6 polynomials, Gaussian flux at
+                                 matched pitch, odd-count rule
   observation_box_voxels / volume_probability_box / expected_location_box
                                  hand-worked voxel examples
   absorbing_potential            faces: 1-D polynomial ramp closed form; corners
@@ -35,6 +42,7 @@ below; the pins live in ``tests/test_oracle_*.py``.
   harmonic_potential / lattice_potential   closed form (no reference counterpart)
   predict_free_space_*           closed form (doi:10.1088/0143-0807/18/3/022)
   indefinite_simpson_integral    exact for cubics; hand-worked small n
+  relative_l2                    definition (used as the comparison, not compared)
 """
 
 from __future__ import annotations
