@@ -21,8 +21,9 @@ below; the pins live in ``tests/test_oracle_*.py``.
                                  unitarity, 2nd-order convergence, Ehrenfest
   power_series_step              dense-H brute force (matrix powers), expm limit,
                                  Horner identity
-  power_series_horner            equal to power_series_step to rounding (p = 1, 2, 10)
-  laplacian_clamped              dense 7-point matrix with edge-replicated faces
+  power_series_horner            equal to power_series_step to rounding (p = 1, 3, 10)
+  laplacian_clamped              dense 7-point matrix with edge-replicated faces; a
+                                 per-axis-spacing "fix" is caught
   simpson_weights_1d / volume_probability / expected_location
                                  exact for x^a y^b z^c (a,b,c<=3), Gaussian moments
   standard_deviation             Gaussian sigma from its first two moments
