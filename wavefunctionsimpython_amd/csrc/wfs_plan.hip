@@ -442,7 +442,8 @@ int run_slab_lines(wfs_plan* p, int mode, void* data, const void* pv, hipStream_
       pp.spz = static_cast<const char*>(p->sep[2]) + size_t(c) * per * p->n[2];
     } else {
       pp.pv = pv;
-      if (p->pv_planes_ok && p->variant[WFS_K_PV_PLANES] != 0) pp.pv_planes = p->pv_planes;
+      // the map describes the plan's own lines table (the autotuner steps scratch tables)
+      if (pv == p->pv_lines && p->pv_planes_ok && p->variant[WFS_K_PV_PLANES] != 0) pp.pv_planes = p->pv_planes;
     }
   }
   pp.nx = p->n[0];
