@@ -55,12 +55,16 @@ __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
   return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
 
+// ZB = 8 runs four waves per SIMD (at most 128 VGPRs: the x transforms' table twiddles are read
+// from LDS one at a time where they are applied, so no twiddle array is held; at nx = 256 one
+// spilled register, 52.4 against 55.8 us per launch with three waves, bit-identical).
 // ZB = 8: a block reads and writes 64-byte halves of 128-byte lines; the two blocks of a line
 // get the block indices 16 g + x and 16 g + 8 + x, which the dispatcher sends to the same XCD
 // (blocks go round-robin over the 8 XCDs) one after the other, so the second half comes from
 // that XCD's L2.
 template <typename R, int NX, int NY, int NZ, int B, int ZB>
-__global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>* __restrict__ data,
+__global__ __launch_bounds__(2 * ZB * split_of<NX>::n2)
+__attribute__((amdgpu_waves_per_eu(ZB == 8 ? 4 : 1))) void x_slab_kernel(cx<R>* __restrict__ data,
                                                                            const XSlabParams p) {
   constexpr int N2 = split_of<NX>::n2;
   constexpr int N1 = NX / N2;
@@ -153,16 +157,23 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
   const C* __restrict__ ky = reinterpret_cast<const C*>(p.ky);
   const C* __restrict__ kz = reinterpret_cast<const C*>(p.kz);
   __syncthreads();                         // tws, kxs staged
+  // one column transform: the in-register DFT, then each table twiddle read from LDS where it
+  // is applied (no twiddle array held across the DFT), then the LDS exchange
+  auto column = [&](C (&col)[N1], auto sign_c) {
+    constexpr int SG = decltype(sign_c)::value;
+    DFT<N1, SG>::run(col);
+    const C* tr = tws + opaque_v(t) * N1;
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+      const C w = tr[k1];
+      col[k1] = SG < 0 ? cmul(col[k1], w) : cmulc(col[k1], w);
+    }
+    column_dft_second<NX, N2, SG>(col, lds, t, lidx);
+  };
 #pragma unroll
   for (int j = 0; j < RPL; ++j) {
-    C w[N1];
     if (j) __syncthreads();
-    {
-      const C* tr = tws + opaque_v(t) * N1;
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tr[k1];
-    }
-    column_dft<NX, N2, -1>(a[j], w, lds, t, lidx);
+    column(a[j], std::integral_constant<int, -1>{});
     {
       const C* kr = kxs + opaque_v(t);
       const C kyz = cmul(ky[ka + A * (q + 2 * j)], kz[zt * ZB + zl]);
@@ -170,12 +181,7 @@ __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2) void x_slab_kernel(cx<R>
       for (int i = 0; i < N1; ++i) a[j][i] = cmul(a[j][i], cmul(kr[N2 * i], kyz));
     }
     __syncthreads();
-    {
-      const C* tr = tws + opaque_v(t) * N1;
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) w[k1] = tr[k1];
-    }
-    column_dft<NX, N2, +1>(a[j], w, lds, t, lidx);
+    column(a[j], std::integral_constant<int, +1>{});
   }
 
   if constexpr (B == 2) {
