@@ -530,6 +530,123 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2, coarse_n=32, full_fno
     }
 
 
+def parareal512_bench(train_slices=8, ntraj=2, coarse_n=64):
+    """Time-to-T of the 512^3 configuration: V = 3 sum_d cos^2(pi r_d / 4) + a cubic CAP of 16
+    Bohr (box 128 Bohr, observation box +-48), packet sigma 5 at the origin with k0 = (1, 1, 0),
+    dt 0.002, 2000 steps as 16 slices x 125.  (i) plain serial split-step; (ii) parareal with a
+    one-step split-step coarse propagator; (iii) parareal with the FNO correction of that step on
+    a coarse_n^3 grid (CoarseGridFNOCoarse).  The network (4 layers, 12 modes, width 32) is trained
+    first on ntraj x train_slices solver pairs of the lattice without the absorber (random packets
+    from default_rng(1); the absorber lies beyond where the packet goes in this time).  Errors are
+    relative L2 and phase against the GPU complex128 run of the same problem."""
+    import math
+    import numpy as np
+    import torch
+    from wavefunctionsimpython_amd import _abi
+    from wavefunctionsimpython_amd import parareal as pr
+    from wavefunctionsimpython_amd.fno import FNO3d
+    n, box, dt, S, NS = 512, 128.0, 0.002, 125, 16
+    dx = box / n
+    shape = (n,) * 3
+    dev = torch.device("cuda", 0)
+    pscale, vscale = math.sqrt(box ** 3), 0.2
+    rng = np.random.default_rng(1)
+    probs = [pr.LatticeProblem(n=n, dx=dx, omega=0.0, depth=3.0, period=4.0, sigma=5.0,
+                               r0=tuple(float(v) for v in rng.uniform(-2.0, 2.0, 3)),
+                               k0=tuple(float(v) for v in (1.0, 1.0, 0.0) + rng.uniform(-0.3, 0.3, 3)), dt=dt)
+             for _ in range(ntraj)]
+    net = FNO3d(coarse_n, width=32, modes=12, layers=4, in_channels=5, output_scale=0.0, residual_scale=1e-3)
+    pairs = pr.TrajectoryPairs(probs, S, base_steps=1)
+    V0 = pairs.V[0]
+    feat = pr.CoarseGridFNOCoarse(net, V0, pscale, vscale,
+                                  pr.SplitStepCoarse(shape, (dx,) * 3, S * dt, 1.0, V0, 1, axes=probs[0].axes()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = pr.train_coarse(net, pairs, train_slices, pscale, vscale, lr=1e-3, lr_final=1e-4, epochs=1,
+                             featurizer=feat)
+    torch.cuda.synchronize()
+    k = max(1, len(losses) // 4)
+    training = {"pairs": len(losses), "time_s": time.perf_counter() - t0,
+                "loss_first": float(np.mean(losses[:k])), "loss_last": float(np.mean(losses[-k:])),
+                "loss_note": "relative to the one-step split-step's own error (1 = no gain)"}
+    del pairs, feat
+    torch.cuda.empty_cache()
+    gc.collect()
+    gc.freeze()
+    V = torch.zeros(shape, dtype=torch.complex64, device=dev)
+    _abi.add_lattice(shape, (dx,) * 3, 3.0, 4.0, V)
+    _abi.add_cap(shape, (dx,) * 3, (-48.0,) * 3, (48.0,) * 3, 0.0, (-100.0,) * 3, (-100.0,) * 3, (3.0,) * 3,
+                 (3.0,) * 3, V)
+    psi0 = torch.empty(shape, dtype=torch.complex64, device=dev)
+    _abi.init_gaussian(shape, (dx,) * 3, _abi.WFS_C64, (5.0,) * 3, (0.0, 0.0, 0.0), (1.0, 1.0, 0.0), psi0)
+    p64 = _abi.Plan(shape, (dx,) * 3, dt, 1.0, _abi.WFS_C128, _abi.WFS_METHOD_FOURIER)
+    p64.set_potential(V)
+    psi64 = psi0.to(torch.complex128)
+    p64.step(psi64, None, S * NS)
+    del p64
+    torch.cuda.empty_cache()
+
+    def errs(a):
+        acc = accuracy(a, psi64)
+        return {"rel_l2_vs_fp64": acc["rel_l2_vs_c128"], "phase_err_rad": acc["phase_err_rad"]}
+
+    par = pr.Parareal(shape, (dx,) * 3, dt, 1.0, V, S, NS, autotune=True)
+    plain = psi0.clone()
+    par.serial.step(plain, None, S)              # warm-up
+    plain.copy_(psi0)
+    par.serial.set_step_index(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    par.serial.step(plain, None, S * NS)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    base = pr.SplitStepCoarse(shape, (dx,) * 3, S * dt, 1.0, V, 1)
+    eager = {"parareal_splitstep": base, "parareal_fno": pr.CoarseGridFNOCoarse(net, V, pscale, vscale, base)}
+    coarse = {k: pr.GraphedCoarse(c, shape) for k, c in eager.items()}
+    tmp = torch.empty_like(psi0)
+
+    def call_ms(fn, reps=5):
+        fn(psi0, tmp)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            fn(psi0, tmp)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / reps * 1e3
+
+    def fine_slice(src, dst):
+        dst.copy_(src)
+        par.serial.set_step_index(0)
+        par.serial.step(dst, None, S)
+
+    slice_ms = call_ms(fine_slice, reps=2)
+    out = {}
+    for name, c in coarse.items():
+        res = par.run(psi0, c, tol=1e-4)
+        print(f"{name}: {res.iterations} iterations, {res.timings['total_s']:.2f} s", file=sys.stderr, flush=True)
+        ms = call_ms(c)
+        out[name] = {"time_to_T_s": res.timings["total_s"], "iterations": res.iterations,
+                     "converged": res.converged, "fallback": res.fallback, **errs(res.state),
+                     "history": res.history, "coarse_call_ms": ms, "coarse_call_over_fine_slice": ms / slice_ms,
+                     "phases_s": {k: v for k, v in res.timings.items() if k != "total_s"}}
+    return {
+        "metric": "time-to-T at 1e-4 relative L2, FNO-parareal vs plain (512^3 lattice + CAP, 16 slices x 125 steps)",
+        "value": out["parareal_fno"]["time_to_T_s"], "unit": "s", "n_gpus": 1, "steps": S * NS, "warmup": 1,
+        "ms_per_step": out["parareal_fno"]["time_to_T_s"] / (S * NS) * 1e3, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32 state, bf16 FNO, f64 correction",
+        "data": "synthetic",
+        "config": {"workload": "512^3 cos^2 lattice (3 Ha, period 4 Bohr) + CAP 16 Bohr, box 128 Bohr, sigma 5, "
+                               "k0 (1,1,0), dt 0.002, 2000 steps as 16 slices x 125",
+                   "fno": {"layers": 4, "modes": 12, "width": 32, "base": "1-step split-step",
+                           "residual_scale": 1e-3, "network_grid": coarse_n}},
+        "plain": {"time_to_T_s": t_plain, **errs(plain), "grid_pt_steps_per_s": n ** 3 * S * NS / t_plain},
+        "fine_slice_ms": slice_ms,
+        "speedup_fno_parareal_vs_plain": t_plain / out["parareal_fno"]["time_to_T_s"],
+        **out,
+        "training": training,
+    }
+
+
 def dropin_config(tmp, observe_every):
     """The 256^3 double-slit workload as a reference-schema config (TOML keys of
     sources/sim_state.py, potential.py, wave_packet.py, measurement.py): observation box +-32
@@ -778,13 +895,14 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the plan's default tile variants instead of timing them first (profiling runs: "
                          "the same kernels in every process)")
-    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal", "large", "dropin", "fno"],
+    ap.add_argument("--workload", choices=["splitstep256", "batch", "parareal", "parareal512", "large", "dropin",
+                                           "fno"],
                     default="splitstep256",
                     help="splitstep256 (default, the headline metric), batch (48 packets, one hipGraph), "
                          "parareal (time-to-T on the 128^3 lattice, FNO training included), large (512^3, "
                          "split-step and power series against complex128; --steps, 2000 in the config) or dropin "
-                         "(the 256^3 double slit through initialisation + core_sim.run_iteration) or fno (the "
-                         "FNO forward at 128^3 and 32^3)")
+                         "(the 256^3 double slit through initialisation + core_sim.run_iteration), fno (the "
+                         "FNO forward at 128^3 and 32^3) or parareal512 (time-to-T of the 512^3 configuration)")
     ap.add_argument("--train-pairs", type=int, default=2000, help="FNO training pairs (parareal workload)")
     ap.add_argument("--coarse-n", type=int, default=32, help="grid of the coarse-grid FNO (parareal workload)")
     ap.add_argument("--full-fno", action="store_true", help="also run the FNO on the full 128^3 grid (parareal)")
@@ -816,6 +934,9 @@ def main():
         return
     if args.workload == "large":
         print(json.dumps(large_bench(args.steps)), flush=True)
+        return
+    if args.workload == "parareal512":
+        print(json.dumps(parareal512_bench()), flush=True)
         return
     if args.workload == "dropin":
         print(json.dumps(dropin_bench(args.steps, args.warmup)), flush=True)
