@@ -882,7 +882,7 @@ def large_bench(steps):
         "methods": out,
         "note": "power_series rel_l2_vs_fourier_c128 is mostly method error (7-point Laplacian and Taylor "
                 "truncation against the spectral propagator); rel_l2_vs_power_series_c128 is its precision error. "
-                "The FNO-parareal leg is not run at 512^3 (one width-32 activation is 17 GiB)",
+                "The FNO-parareal leg of this configuration is the parareal512 workload (network on a 64^3 grid)",
     }
 
 
