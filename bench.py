@@ -11,8 +11,8 @@ split-step Fourier method over the whole grid; inputs are resident in HBM
 before the timed region starts.
 
 Prints one JSON line with value = grid-point-steps per second (all GPUs),
-the roofline of the dominant kernel (HIP events around every launch of a
-separate instrumented window on the same stream) and a CPU baseline (the
+the roofline of the dominant kernel (HIP start/stop events recorded by every
+launch of a separate instrumented window on the same stream) and a CPU baseline (the
 complex128 oracle of the reference's 4-FFT step, timed on a few steps on the
 host cores).
 """
@@ -150,7 +150,7 @@ def gpu_bench(steps, warmup, autotune=True):
     del p64, ref, out64
     torch.cuda.empty_cache()
 
-    # instrumented window: HIP events around every launch, same stream, same workload
+    # instrumented window: HIP events recorded by every launch, same stream, same workload
     plan.set_profiling(True)
     run_chunks(plan, psi, None, boxes, 40, out)
     torch.cuda.synchronize()

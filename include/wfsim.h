@@ -244,8 +244,11 @@ int wfs_potential_operator(long count, const void* V, double dt, int precision, 
 /*
  * Per-kernel timing (no reference counterpart: the reference times whole
  * iterations with time.time(), sources/core_sim.py:43-49).  While enabled,
- * every kernel launched for the plan is bracketed by a pair of HIP events on
- * its stream.  wfs_plan_profile() waits for the recorded events and returns,
+ * every kernel launched for the plan records a pair of HIP events on its
+ * stream: the kernel's own start and end (hipExtLaunchKernel events; on a
+ * stream under capture, events recorded around the launch).  Kernels launched
+ * together for one class (a moments pass and its sum) share one pair.
+ * wfs_plan_profile() waits for the recorded events and returns,
  * per kernel class, the launch count and the summed duration in ms since the
  * last reset (reset != 0 clears the sums after reading).
  * Kernel classes: WFS_K_* below.

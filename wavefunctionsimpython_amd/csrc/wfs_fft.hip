@@ -158,7 +158,7 @@ static int launch_yz_plane_n(void* data, const YzPlaneParams& p, hipStream_t s) 
     set_last_error("yz plane pass: bad plane count %ld", p.planes);
     return WFS_ERR_INVALID;
   }
-  hipLaunchKernelGGL((yz_lines_kernel<R, N, N, 1, H, LINES_FULL>), dim3(unsigned(p.planes)),
+  WFS_LAUNCH((yz_lines_kernel<R, N, N, 1, H, LINES_FULL>), dim3(unsigned(p.planes)),
                      dim3(yz_lines_threads<N, N, H>()), 0, s, reinterpret_cast<cx<R>*>(data), p);
   return int(hipGetLastError());
 }
@@ -190,7 +190,7 @@ static int launch_g(void* data, const PassParams& p, long outer, hipStream_t s) 
   constexpr int N2 = split_of<N>::n2;
   const long ncols = (LAYOUT == LAYOUT_ROWS) ? p.rows : p.inner;
   dim3 grid(unsigned((ncols + G - 1) / G), unsigned(LAYOUT == LAYOUT_ROWS ? 1 : outer), 1);
-  hipLaunchKernelGGL((fft_pass_kernel<R, N, N2, LAYOUT, MODE, G>), grid, dim3(N2 * G), 0, s,
+  WFS_LAUNCH((fft_pass_kernel<R, N, N2, LAYOUT, MODE, G>), grid, dim3(N2 * G), 0, s,
                      reinterpret_cast<cx<R>*>(data), p);
   return int(hipGetLastError());
 }

@@ -1,8 +1,38 @@
 // Internal declarations shared by the wfsim translation units.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "wfs_common.h"
 #include "../../include/wfsim.h"
+
+namespace wfs {
+
+// Per-kernel timing (wfs_plan_set_profiling): inside a timed launch scope the host thread's
+// pointer names an event pair.  The first launch made through WFS_LAUNCH records its start
+// timestamp into `start`, every launch its end timestamp into `stop` (hipExtLaunchKernel takes
+// them from the dispatch itself), so the pair spans the scope's kernels alone, without the
+// dispatch time that events recorded beside a launch add to it.  Null outside a timed scope.
+struct LaunchTiming {
+  hipEvent_t start = nullptr, stop = nullptr;
+  int launches = 0;
+};
+LaunchTiming*& launch_timing();
+
+}  // namespace wfs
+
+// hipLaunchKernelGGL, or hipExtLaunchKernelGGL with the events of the current timed scope
+#define WFS_LAUNCH(kernel, grid, block, lds, stream, ...)                                              \
+  do {                                                                                               \
+    ::wfs::LaunchTiming* wfs_lt_ = ::wfs::launch_timing();                                          \
+    if (wfs_lt_) {                                                                                   \
+      hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), std::uint32_t(lds), stream,             \
+                            wfs_lt_->launches++ ? nullptr : wfs_lt_->start, wfs_lt_->stop, 0u,        \
+                            __VA_ARGS__);                                                            \
+    } else {                                                                                         \
+      hipLaunchKernelGGL(kernel, grid, block, lds, stream, __VA_ARGS__);                            \
+    }                                                                                                \
+  } while (0)
 
 namespace wfs {
 

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kObsThreads) void sum_partials_kernel(const double*
 
 int launch_sum_partials(const double* partials, long nparts, int nb, int batch, double scale, double* out,
                         long out_stride, hipStream_t s) {
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(nb), unsigned(batch)), dim3(kObsThreads), 0, s, partials,
+  WFS_LAUNCH(sum_partials_kernel, dim3(unsigned(nb), unsigned(batch)), dim3(kObsThreads), 0, s, partials,
                      nparts, nb, scale, out, out_stride);
   return int(hipGetLastError());
 }
@@ -220,15 +220,15 @@ int launch_box_moments(int precision, int batch, int nx, int ny, int nz, const d
   // the Simpson weight products carry 1/27 and the volume element dx dy dz
   const double sc = dx[0] * dx[1] * dx[2] / 27.0;
   if (precision == WFS_C128) {
-    hipLaunchKernelGGL(box_moments_kernel<double>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
+    WFS_LAUNCH(box_moments_kernel<double>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
                        static_cast<const double2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], set, partials);
   } else {
-    hipLaunchKernelGGL(box_moments_kernel<float>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
+    WFS_LAUNCH(box_moments_kernel<float>, dim3(unsigned(nblocks), unsigned(batch)), dim3(kObsThreads), 0, s,
                        static_cast<const float2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], set, partials);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(nb), unsigned(batch)), dim3(kObsThreads), 0, s, partials,
+  WFS_LAUNCH(sum_partials_kernel, dim3(unsigned(nb), unsigned(batch)), dim3(kObsThreads), 0, s, partials,
                      nblocks, nb, sc, out, out_stride);
   return int(hipGetLastError());
 }
@@ -372,17 +372,17 @@ int launch_plane_current(int precision, int nx, int ny, int nz, const double dx[
   p.cv = count[1];
   p.inv_mass = 1.0 / mass;
   if (precision == WFS_C128)
-    hipLaunchKernelGGL(plane_current_kernel<double>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
+    WFS_LAUNCH(plane_current_kernel<double>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
                        static_cast<const double2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], p,
                        static_cast<double*>(density), partials);
   else
-    hipLaunchKernelGGL(plane_current_kernel<float>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
+    WFS_LAUNCH(plane_current_kernel<float>, dim3(unsigned(nblocks)), dim3(kObsThreads), 0, s,
                        static_cast<const float2*>(psi), nx, ny, nz, dx[0], dx[1], dx[2], p,
                        static_cast<float*>(density), partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
   // Simpson 1/9 and the pixel area dW dH
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kObsThreads), 0, s, partials, nblocks, 1,
+  WFS_LAUNCH(sum_partials_kernel, dim3(1), dim3(kObsThreads), 0, s, partials, nblocks, 1,
                      p.dWd * p.dHd / 9.0, out7, 7L);
   return int(hipGetLastError());
 }

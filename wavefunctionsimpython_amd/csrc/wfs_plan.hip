@@ -52,6 +52,11 @@ namespace wfs {
 
 static thread_local char g_last_error[512] = "";
 
+LaunchTiming*& launch_timing() {
+  static thread_local LaunchTiming* current = nullptr;
+  return current;
+}
+
 void set_last_error(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -197,17 +202,29 @@ int ensure_scratch(wfs_plan* p, hipStream_t s) {
   return 0;
 }
 
-// event pair around one launch when profiling is on
+// Timing of the launches of one kernel class when profiling is on: the launches made through
+// WFS_LAUNCH inside the scope record the kernels' own start and end timestamps into the event
+// pair (the durations a kernel trace reports).  On a stream under capture the pair is recorded
+// around the launches instead, as graph nodes.
 struct ProfileScope {
   wfs_plan* p;
   int kclass;
   hipStream_t s;
-  hipEvent_t a = nullptr, b = nullptr;
+  LaunchTiming t;
+  LaunchTiming* outer = nullptr;
+  bool hooked = false;
   ProfileScope(wfs_plan* plan, int kc, hipStream_t st) : p(plan), kclass(kc), s(st) {
     if (!p->profiling) return;
-    a = take();
-    b = take();
-    if (a && b) (void)hipEventRecord(a, s);
+    t.start = take();
+    t.stop = take();
+    if (!t.start || !t.stop) return;
+    if (capturing(s)) {
+      (void)hipEventRecord(t.start, s);
+    } else {
+      outer = launch_timing();
+      launch_timing() = &t;
+      hooked = true;
+    }
   }
   hipEvent_t take() {
     if (!p->event_pool->empty()) {
@@ -221,12 +238,14 @@ struct ProfileScope {
   }
   ~ProfileScope() {
     if (!p->profiling) return;
-    if (a && b) {
-      (void)hipEventRecord(b, s);
-      p->pending[kclass].push_back({a, b});
+    if (hooked) launch_timing() = outer;
+    const bool timed = t.start && t.stop && (!hooked || t.launches > 0);
+    if (timed) {
+      if (!hooked) (void)hipEventRecord(t.stop, s);
+      p->pending[kclass].push_back({t.start, t.stop});
     } else {
-      if (a) p->event_pool->push_back(a);
-      if (b) p->event_pool->push_back(b);
+      if (t.start) p->event_pool->push_back(t.start);
+      if (t.stop) p->event_pool->push_back(t.stop);
     }
   }
 };

@@ -266,7 +266,7 @@ int launch_lines_m(void* data, const YzPlaneParams& p, hipStream_t s) {
   static_assert(yz_lines_lds_bytes<float, N, LY, 1>() <= 160 * 1024, "rows do not fit in LDS");
   const long groups = p.planes * B;
   const long grid = PERSIST ? std::min<long>(groups, cu_count()) : groups;
-  hipLaunchKernelGGL((yz_lines_kernel<float, N, LY, B, 1, MODE, PERSIST, true>), dim3(unsigned(grid)),
+  WFS_LAUNCH((yz_lines_kernel<float, N, LY, B, 1, MODE, PERSIST, true>), dim3(unsigned(grid)),
                      dim3(yz_lines_threads<N, LY, 1>()), 0, s, reinterpret_cast<float2*>(data), p);
   return int(hipGetLastError());
 }
@@ -291,7 +291,7 @@ int launch_x_slab_z(void* data, const XSlabParams& p, hipStream_t s) {
   constexpr int B = SlabShape<N>::B;
   constexpr long per_packet = long(N / B) * (N / ZB);
   static_assert(per_packet % 16 == 0, "the XCD pairing needs a multiple of 16 blocks");
-  hipLaunchKernelGGL((x_slab_kernel<float, NX, N, N, B, ZB>), dim3(unsigned(p.packets * per_packet)),
+  WFS_LAUNCH((x_slab_kernel<float, NX, N, N, B, ZB>), dim3(unsigned(p.packets * per_packet)),
                      dim3(2 * ZB * split_of<NX>::n2), 0, s, reinterpret_cast<float2*>(data), p);
   return int(hipGetLastError());
 }
@@ -326,11 +326,11 @@ int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets,
     set_last_error("potential plane sharing: bad sizes");
     return WFS_ERR_INVALID;
   }
-  hipLaunchKernelGGL(plane_equal_kernel, dim3(unsigned(planes)), dim3(256), 0, s,
+  WFS_LAUNCH(plane_equal_kernel, dim3(unsigned(planes)), dim3(256), 0, s,
                      static_cast<const unsigned long long*>(table), plane_elems, nx, flags);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
-  hipLaunchKernelGGL(plane_runs_kernel, dim3(unsigned((packets + 63) / 64)), dim3(64), 0, s, flags, nx, packets, src);
+  WFS_LAUNCH(plane_runs_kernel, dim3(unsigned((packets + 63) / 64)), dim3(64), 0, s, flags, nx, packets, src);
   return int(hipGetLastError());
 }
 

@@ -297,7 +297,7 @@ static void launch_pair_f32(int batch, int nx, int ny, int nz, const void* u_in,
                             int ntiles, hipStream_t s) {
   constexpr int NT = 256;
   dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
-  hipLaunchKernelGGL((horner2_kernel<float, TZ, TY, NT>), grid, dim3(NT), 0, s, static_cast<const float2*>(u_in),
+  WFS_LAUNCH((horner2_kernel<float, TZ, TY, NT>), grid, dim3(NT), 0, s, static_cast<const float2*>(u_in),
                      static_cast<float2*>(u_out), static_cast<const float2*>(psi), V, nx, ny, nz, float(c1),
                      float(c2), float(inv_2m), float(inv_dx2), tile_x, ntiles);
 }
@@ -314,7 +314,7 @@ int launch_horner_pair(int precision, int batch, int nx, int ny, int nz, const v
   if (precision == WFS_C128) {
     constexpr int TZ = 32, TY = 8, NT = 256;
     dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
-    hipLaunchKernelGGL((horner2_kernel<double, TZ, TY, NT>), grid, dim3(NT), 0, s,
+    WFS_LAUNCH((horner2_kernel<double, TZ, TY, NT>), grid, dim3(NT), 0, s,
                        static_cast<const double2*>(u_in), static_cast<double2*>(u_out),
                        static_cast<const double2*>(psi), V, nx, ny, nz, c1, c2, inv_2m, inv_dx2, tile_x, ntiles);
   } else if (variant == 1) {
@@ -338,11 +338,11 @@ int launch_horner_stage(int precision, int batch, int nx, int ny, int nz, const 
   dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
   dim3 block(TZ, TY, 1);
   if (precision == WFS_C128) {
-    hipLaunchKernelGGL((horner_stage_kernel<double, TZ, TY>), grid, block, 0, s,
+    WFS_LAUNCH((horner_stage_kernel<double, TZ, TY>), grid, block, 0, s,
                        static_cast<const double2*>(u_in), static_cast<double2*>(u_out),
                        static_cast<const double2*>(psi), V, nx, ny, nz, dt_over_n, inv_2m, inv_dx2, tile_x, ntiles);
   } else {
-    hipLaunchKernelGGL((horner_stage_kernel<float, TZ, TY>), grid, block, 0, s,
+    WFS_LAUNCH((horner_stage_kernel<float, TZ, TY>), grid, block, 0, s,
                        static_cast<const float2*>(u_in), static_cast<float2*>(u_out),
                        static_cast<const float2*>(psi), V, nx, ny, nz, float(dt_over_n), float(inv_2m),
                        float(inv_dx2), tile_x, ntiles);
