@@ -385,3 +385,52 @@ def test_table_reorder_inside_a_capture(dev):
         direct.step(want, None, 3)
         torch.cuda.synchronize()
         assert torch.equal(graphed, want), f"replay {replay}"
+
+
+def test_potential_set_inside_a_capture_retires_the_plane_map(dev):
+    """A potential whose planes repeat is set uncaptured (the plan builds its plane map), then a
+    potential with no repeats is set inside a capture.  An uncaptured call before the first
+    replay still finds the older table; after the replay the table holds the new potential,
+    which the plan cannot see, so it must not describe the table by a map built before: every
+    result equals plans that hold each potential from the start, bit for bit."""
+    shape = (32, 256, 256)
+    V0 = torch.from_numpy(_repeating_planes(shape, 1, 83)).to(dev)
+    psi0, V1n = _inputs(shape, 84)
+    V1 = torch.from_numpy(V1n).to(dev)
+    start = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+    p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    p.set_potential(V0)
+    assert p.variants()["pv_planes"] == 1
+    graphed = start.clone()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with _abi.Graph(s) as g:
+        p.step(graphed, V1, 3, stream=s)             # the new table is written at each replay
+    before = start.clone()
+    p.step(before, None, 3)                          # no replay yet: the older table
+    torch.cuda.synchronize()                         # the replay rewrites the table this call reads
+    g.launch()
+    s.synchronize()
+    after = start.clone()
+    p.step(after, None, 3)                           # the replay's table
+    torch.cuda.synchronize()
+    r0 = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    r0.set_potential(V0)
+    r1 = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    r1.set_potential(V1)
+    want = start.clone()
+    r0.set_step_index(3)
+    r0.step(want, None, 3)
+    torch.cuda.synchronize()
+    print("case: potential set inside a capture, uncaptured calls before and after a replay", flush=True)
+    assert torch.equal(before, want)
+    want = start.clone()
+    r1.set_step_index(6)
+    r1.step(want, None, 3)
+    torch.cuda.synchronize()
+    assert torch.equal(after, want)
+    want = start.clone()
+    r1.set_step_index(0)
+    r1.step(want, None, 3)
+    torch.cuda.synchronize()
+    assert torch.equal(graphed, want)

@@ -35,6 +35,8 @@ struct wfs_plan {
   int* pv_planes;         // slab scheme: per x plane the first plane of its run of identical
                           // planes of pv_lines, then as many ints of workspace (lazy)
   bool pv_planes_ok;      // pv_planes describes the current pv_lines
+  bool pv_in_graph;       // pv_lines was written inside a captured graph: its content changes at
+                          // replays the plan does not see, so the plane map is not used again
   bool pv_set;
   void* tw_rows[2];       // slab scheme: the y and z twiddles of each thread's lines, [ntab][u][k1]
   void* sep[3];           // separable potential: per-axis exp(-i dt f_d), [ntab][batch][n_d]
@@ -323,10 +325,12 @@ int alloc_table(wfs_plan* p, void** t, hipStream_t s) {
 // variant 1): built with the lines table, so that graph capture finds it ready.  Under capture
 // it is not built (its kernels would run only at replay, and an uncaptured call before that
 // would read an unwritten map): the captured steps read every plane's own copy, and the next
-// uncaptured call builds it.
+// uncaptured call builds it.  Once the table itself has been written inside a captured graph
+// (a potential set or a reorder under capture), every replay may change it behind the map, so
+// from then on the plan reads every plane's own copy.
 int share_planes(wfs_plan* p, hipStream_t s) {
   p->pv_planes_ok = false;
-  if (p->variant[WFS_K_PV_PLANES] == 0) return 0;
+  if (p->variant[WFS_K_PV_PLANES] == 0 || p->pv_in_graph) return 0;
   if (capturing(s)) return 0;
   const long planes = long(p->batch) * p->n[0];
   if (!p->pv_planes) {
@@ -352,6 +356,7 @@ int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
                                          slab_lines_order(p->n[1]), s));
     p->pv_lines_ok = true;
     p->pv_nat_ok = false;
+    if (capturing(s)) p->pv_in_graph = true;
     WFS_TRY(share_planes(p, s));
   } else {
     WFS_TRY(alloc_table(p, &p->pv, s));
@@ -377,6 +382,7 @@ int current_table(wfs_plan* p, hipStream_t s, const void** out) {
       WFS_TRY(launch_table_reorder(p->precision, count, p->pv, p->pv_lines, slab_lines_order(p->n[1]), true, s));
       p->pv_lines_ok = !capturing(s);
       p->pv_planes_ok = false;
+      if (capturing(s)) p->pv_in_graph = true;
     }
     if (!p->pv_planes_ok && p->variant[WFS_K_PV_PLANES] != 0) WFS_TRY(share_planes(p, s));
     *out = p->pv_lines;
