@@ -146,6 +146,21 @@ __device__ __forceinline__ T twiddle_ct(T a) {
     T r;
     if constexpr (SIGN > 0) { r.x = a.y; r.y = -a.x; } else { r.x = -a.y; r.y = a.x; }
     return r;
+  } else if constexpr (sizeof(R) == 4 && (8 * e) % M == 0) {
+    // odd multiples of pi/4: w = (cx + i cy) sqrt(1/2) with cx, cy = +-1, so
+    // a w = ((cx a.x - cy a.y) + i (cx a.y + cy a.x)) sqrt(1/2): one addition per part, then
+    // the product with sqrt(1/2) as hi + lo (6 instructions instead of the 8 of cmul2)
+    constexpr int o = 8 * e / M;                               // 1, 3, 5 or 7
+    constexpr R cx = (o == 1 || o == 7) ? R(1) : R(-1);
+    constexpr R cy = ((o == 1 || o == 3) == (SIGN > 0)) ? R(1) : R(-1);
+    constexpr R h = R(kSqrtHalf);
+    constexpr R l = R(kSqrtHalf - double(R(kSqrtHalf)));
+    const R px = cx * a.x - cy * a.y;
+    const R py = cx * a.y + cy * a.x;
+    T r;
+    r.x = __builtin_fmaf(px, h, px * l);
+    r.y = __builtin_fmaf(py, h, py * l);
+    return r;
   } else {
     constexpr cs_pair cs = cos_sin_2pi(e, M);
     T w;
