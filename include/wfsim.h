@@ -277,9 +277,11 @@ int wfs_plan_set_profiling(wfs_plan* plan, int enable);
  * (0, 1, 2 = half, default, double the columns per block) of every
  * split-step pass on plan-owned scratch buffers and keeps the fastest per
  * kernel class.  Variants give bit-identical results.  Synchronises stream.
- * WFS_K_HORNER: variant 0 runs one launch per power-series stage, 1, 2 and
- * 3 run the stages in fused pairs on 16 x 64, 8 x 64 and 8 x 128 (y, z)
- * tiles (default 3 where nz >= 128, else 1); the results are bit-identical.
+ * WFS_K_HORNER: variant 0 runs one launch per power-series stage, 1 to 5
+ * run the stages in fused pairs on 16 x 64, 8 x 64 and 8 x 128 (y, z) tiles
+ * of 256 threads, 8 x 128 tiles of 512 threads and 8 x 256 tiles of 1024
+ * threads (default 5 where nz >= 256, 4 where nz >= 128, else 1); the
+ * results are bit-identical.
  * WFS_K_YZ_PLANE: variant 1 (default where supported: complex64 planes of
  * ny = nz <= 128, complex128 planes of ny = nz <= 64) runs the y inverse, z
  * inverse / potential / z forward and y forward passes of each step as one

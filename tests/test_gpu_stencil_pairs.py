@@ -30,7 +30,7 @@ def test_pairs_equal_single_stages(dev, precision, n, order, batch):
     single.set_variant("horner_stage", 0)
     a = psi.clone()
     single.step(a, V, 3)
-    for variant in (1, 2, 3):
+    for variant in (1, 2, 3, 4, 5):
         paired = _abi.Plan(n, dx, 0.01, 1.0, precision, _abi.WFS_METHOD_POWER_SERIES, order=order, batch=batch)
         paired.set_variant("horner_stage", variant)
         b = psi.clone()

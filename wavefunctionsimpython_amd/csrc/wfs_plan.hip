@@ -691,7 +691,7 @@ extern "C" int wfs_plan_create_batched(const int n[3], const double dx[3], doubl
   for (int k = 0; k < WFS_K_COUNT; ++k) p->variant[k] = 1;
   // fused stage pairs: 8 x 128 tiles where the z axis fills them (measured fastest at 256^3),
   // else 16 x 64
-  p->variant[WFS_K_HORNER] = p->n[2] >= 128 ? 3 : 1;
+  p->variant[WFS_K_HORNER] = p->n[2] >= 256 ? 5 : p->n[2] >= 128 ? 4 : 1;
   // fused plane pass where the plane fits in LDS; at 128^2 two rounds (two blocks per CU) once
   // there are enough planes to fill the CUs twice over (measured: 1024 planes 13% faster, 128
   // planes 11% slower)
@@ -913,7 +913,7 @@ extern "C" int wfs_plan_profile(wfs_plan* p, int kc, int reset, long* launches, 
 }
 
 extern "C" int wfs_plan_set_variant(wfs_plan* p, int kc, int variant) {
-  const int vmax = kc == WFS_K_HORNER ? 3 : kc == WFS_K_PV_PLANES ? 1 : 2;
+  const int vmax = kc == WFS_K_HORNER ? 5 : kc == WFS_K_PV_PLANES ? 1 : 2;
   if (!p || kc < 0 || kc >= WFS_K_COUNT || variant < 0 || variant > vmax) { set_last_error("wfs_plan_set_variant: bad argument"); return WFS_ERR_INVALID; }
   if (kc == WFS_K_YZ_PLANE && variant > 0 && !yz_plane_supported(p->precision, p->n[1], p->n[2], variant)) {
     set_last_error("wfs_plan_set_variant: fused plane pass variant %d unsupported for %d x %d planes", variant, p->n[1], p->n[2]);

@@ -291,21 +291,23 @@ __global__ __launch_bounds__(NT) void horner2_kernel(const cx<R>* __restrict__ u
   }
 }
 
-template <int TZ, int TY>
+template <int TZ, int TY, int NT = 256>
 static void launch_pair_f32(int batch, int nx, int ny, int nz, const void* u_in, void* u_out, const void* psi,
                             const float2* V, double c1, double c2, double inv_2m, double inv_dx2, int tile_x,
                             int ntiles, hipStream_t s) {
-  constexpr int NT = 256;
   dim3 grid((nz + TZ - 1) / TZ, (ny + TY - 1) / TY, unsigned(ntiles * batch));
   WFS_LAUNCH((horner2_kernel<float, TZ, TY, NT>), grid, dim3(NT), 0, s, static_cast<const float2*>(u_in),
                      static_cast<float2*>(u_out), static_cast<const float2*>(psi), V, nx, ny, nz, float(c1),
                      float(c2), float(inv_2m), float(inv_dx2), tile_x, ntiles);
 }
 
-// variant (complex64): 1 = 16 x 64 (y, z) tile, 2 = 8 x 64, 3 = 8 x 128.  A wider z tile wastes
-// less of the partly used memory lines at its z halo, a deeper y tile less y halo; both cost LDS
-// and so blocks per CU.  Measured at 256^3: 127, 128 and 123 us per launch; the plan takes 3
-// where nz >= 128, else 1.
+// variant (complex64): 1 = 16 x 64 (y, z) tile, 2 = 8 x 64, 3 = 8 x 128 with 256 threads, 4 = 8 x 128
+// with 512 threads, 5 = 8 x 256 with 1024 threads.  A wider z tile wastes less of the partly used
+// memory lines at its z halo, a deeper y tile less y halo; both cost LDS and so blocks per CU.
+// More threads per tile give each fewer cells (fewer registers) and the CU more wavefronts to
+// hide the plane loads with: 4 and 5 run 16 wavefronts per CU where the LDS of 3 holds 8.
+// Measured at 256^3 (one box, alternating): 120, 120, 120-125, 113 and 111 us per launch; the
+// plan takes 5 where nz >= 256, 4 where nz >= 128, else 1.
 int launch_horner_pair(int precision, int batch, int nx, int ny, int nz, const void* u_in, void* u_out,
                        const void* psi, const float2* V, double c1, double c2, double inv_2m, double inv_dx2,
                        int variant, hipStream_t s) {
@@ -321,6 +323,10 @@ int launch_horner_pair(int precision, int batch, int nx, int ny, int nz, const v
     launch_pair_f32<64, 16>(batch, nx, ny, nz, u_in, u_out, psi, V, c1, c2, inv_2m, inv_dx2, tile_x, ntiles, s);
   } else if (variant == 3) {
     launch_pair_f32<128, 8>(batch, nx, ny, nz, u_in, u_out, psi, V, c1, c2, inv_2m, inv_dx2, tile_x, ntiles, s);
+  } else if (variant == 4) {
+    launch_pair_f32<128, 8, 512>(batch, nx, ny, nz, u_in, u_out, psi, V, c1, c2, inv_2m, inv_dx2, tile_x, ntiles, s);
+  } else if (variant == 5) {
+    launch_pair_f32<256, 8, 1024>(batch, nx, ny, nz, u_in, u_out, psi, V, c1, c2, inv_2m, inv_dx2, tile_x, ntiles, s);
   } else {
     launch_pair_f32<64, 8>(batch, nx, ny, nz, u_in, u_out, psi, V, c1, c2, inv_2m, inv_dx2, tile_x, ntiles, s);
   }
