@@ -194,6 +194,40 @@ def test_slab_autotune_and_profile_classes(dev):
         assert prof.get(k, (0, 0.0))[0] == 0
 
 
+def test_profile_times_lie_within_the_calls(dev):
+    """Per-kernel timing comes from start and end events recorded by the launches themselves:
+    every class's summed kernel time is positive, the classes of a call together take no more
+    than the call's own wall time on the stream (events around the whole call), and the two
+    launches of a step account for most of a long call.  The observables class (moments and
+    their sum, two launches) counts one timed pair per call."""
+    shape = (32, 256, 256)
+    psi0, V = _inputs(shape, 46)
+    p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
+    psi = torch.from_numpy(psi0).to(dev).to(torch.complex64)
+    p.set_potential(torch.from_numpy(V).to(dev))
+    p.step(psi, None, 2)
+    boxes = [((0, 0, 0), (31, 255, 255)), ((4, 10, 20), (9, 31, 63))]
+    out = p.observables(psi, boxes)
+    torch.cuda.synchronize()
+    p.set_profiling(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    p.step(psi, None, 20)
+    p.observables(psi, boxes, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    prof = p.profile(reset=True)
+    p.set_profiling(False)
+    wall = e0.elapsed_time(e1)
+    print(f"case: profile vs wall, {shape}: {prof}, wall {wall:.3f} ms", flush=True)
+    assert prof["yz_slab"][0] == 20 and prof["x_slab"][0] == 21 and prof["slab_edge"][0] == 2
+    assert prof["observables"][0] == 1
+    assert all(ms > 0.0 for _, ms in prof.values())
+    total = sum(ms for _, ms in prof.values())
+    assert total <= wall * 1.001
+    assert prof["yz_slab"][1] + prof["x_slab"][1] >= 0.6 * wall
+
+
 @pytest.mark.parametrize("form", FORMS)
 def test_slab_graph_replays_match_eager(dev, form):
     shape = (16, 256, 256)
