@@ -373,14 +373,17 @@ int wfs_fno_layer(const float* W, const float* v, const float* Z, const float* b
 int wfs_fno_gelu_backward(const float* u, const float* dv, float* du, long n, void* stream);
 /*
  * wfs_spectral_resample (no reference counterpart): the spectrum of an ndst grid from that of
- * an nsrc grid, complex64, batch grids stacked: dst[k] = scale * src[k] for every signed
- * frequency k (numpy fftfreq order) both grids hold, 0 for those only dst holds.  Between
- * ortho wfs_fft3d transforms, scale = (|ndst| / |nsrc|)^(1/2) (point counts) keeps the values
- * of a band-limited function: the coarse-grid coarse propagator of parareal runs its network
- * on such a resampled state.  src and dst must not overlap.
+ * an nsrc grid, complex64, batch grids stacked: dst[k] = scale * src[k + shift] for every
+ * signed frequency k of dst (numpy fftfreq order) whose k + shift src holds, 0 for the others;
+ * shift (integer frequencies per axis, |shift| <= nsrc) may be NULL for none.  Between ortho
+ * wfs_fft3d transforms, scale = (|ndst| / |nsrc|)^(1/2) (point counts) keeps the values of a
+ * band-limited function, and a shift s takes the band around frequency s to the band around 0
+ * (the function times exp(-2 pi i s.j / n)); the inverse map uses -s.  The coarse-grid coarse
+ * propagator of parareal runs its network on such a resampled state.  src and dst must not
+ * overlap.
  */
 int wfs_spectral_resample(const void* src, const int nsrc[3], void* dst, const int ndst[3], long batch, double scale,
-                          void* stream);
+                          const int shift[3], void* stream);
 int wfs_fno_axpby(float* y, const float* x, float alpha, float beta, long n, void* stream);
 int wfs_fno_pack(const void* psi, int precision, const void* V, float pscale, float vscale, float* a, long P,
                  void* stream);

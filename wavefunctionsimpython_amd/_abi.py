@@ -60,7 +60,7 @@ SIGNATURES = {
                                     ctypes.c_float, ctypes.c_float, ctypes.c_int, _vp]),
     "wfs_parareal_update": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_long, _vp, ctypes.c_long, _vp, _vp]),
     "wfs_narrow_c128": (ctypes.c_int, [_vp, _vp, ctypes.c_long, _vp]),
-    "wfs_spectral_resample": (ctypes.c_int, [_vp, _c_int3, _vp, _c_int3, ctypes.c_long, ctypes.c_double, _vp]),
+    "wfs_spectral_resample": (ctypes.c_int, [_vp, _c_int3, _vp, _c_int3, ctypes.c_long, ctypes.c_double, _vp, _vp]),
     "wfs_graph_begin": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "wfs_graph_end": (ctypes.c_int, [_vp, _vp]),
     "wfs_graph_launch": (ctypes.c_int, [_vp, _vp]),
@@ -481,10 +481,12 @@ def fno_modes(Rr: torch.Tensor, Ri: torch.Tensor, X: torch.Tensor, O: torch.Tens
           "wfs_fno_modes")
 
 
-def spectral_resample(src: torch.Tensor, dst: torch.Tensor, scale: float | None = None, stream=None) -> None:
+def spectral_resample(src: torch.Tensor, dst: torch.Tensor, scale: float | None = None, shift=None,
+                      stream=None) -> None:
     """dst spectrum from src spectrum (``wfs_spectral_resample``): complex64 device tensors of shape
-    (..., n0, n1, n2) with the same leading (batch) size; frequencies both hold are copied times
-    scale (default (|dst| / |src|)^(1/2), the ortho-transform value-preserving factor), the rest 0."""
+    (..., n0, n1, n2) with the same leading (batch) size; dst frequency k takes src frequency
+    k + shift (shift: three integers, default none) times scale (default (|dst| / |src|)^(1/2), the
+    ortho-transform value-preserving factor), 0 where src does not hold k + shift."""
     for t, name in ((src, "src"), (dst, "dst")):
         _require_device(t, name)
         if t.dtype != torch.complex64 or t.dim() < 3:
@@ -497,7 +499,8 @@ def spectral_resample(src: torch.Tensor, dst: torch.Tensor, scale: float | None 
         raise ValueError("src and dst must differ")
     if scale is None:
         scale = math.sqrt((nd[0] * nd[1] * nd[2]) / (ns[0] * ns[1] * ns[2]))
-    check(lib().wfs_spectral_resample(ptr(src), i3(ns), ptr(dst), i3(nd), int(batch), float(scale),
+    sh = None if shift is None else i3(tuple(int(v) for v in shift))
+    check(lib().wfs_spectral_resample(ptr(src), i3(ns), ptr(dst), i3(nd), int(batch), float(scale), sh,
                                       stream_handle(stream)), "wfs_spectral_resample")
 
 

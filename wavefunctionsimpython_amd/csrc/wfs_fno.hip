@@ -156,7 +156,7 @@ __device__ __forceinline__ int signed_freq(int j, int n) { return j <= (n - 1) /
 
 __global__ __launch_bounds__(kT) void resample_kernel(const float2* __restrict__ src, float2* __restrict__ dst,
                                                       int s0, int s1, int s2, int d0, int d1, int d2, long batch,
-                                                      float scale) {
+                                                      float scale, int h0, int h1, int h2) {
   const long per = long(d0) * d1 * d2;
   const long n = per * batch;
   for (long i = long(blockIdx.x) * kT + threadIdx.x; i < n; i += long(gridDim.x) * kT) {
@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kT) void resample_kernel(const float2* __restrict__
     r /= d2;
     const int j1 = int(r % d1);
     const int j0 = int(r / d1);
-    const int f0 = signed_freq(j0, d0), f1 = signed_freq(j1, d1), f2 = signed_freq(j2, d2);
+    // destination frequency f reads source frequency f + h (h = 0: the same frequency)
+    const int f0 = signed_freq(j0, d0) + h0, f1 = signed_freq(j1, d1) + h1, f2 = signed_freq(j2, d2) + h2;
     // the source holds the signed frequencies -s/2 .. (s-1)/2 of each axis
     const bool in = f0 >= -(s0 / 2) && f0 <= (s0 - 1) / 2 && f1 >= -(s1 / 2) && f1 <= (s1 - 1) / 2 &&
                     f2 >= -(s2 / 2) && f2 <= (s2 - 1) / 2;
@@ -284,15 +285,24 @@ extern "C" int wfs_narrow_c128(const void* U, void* out, long n, void* stream) {
 }
 
 extern "C" int wfs_spectral_resample(const void* src, const int nsrc[3], void* dst, const int ndst[3], long batch,
-                                     double scale, void* stream) {
+                                     double scale, const int shift[3], void* stream) {
   if (!src || !dst || !nsrc || !ndst || batch < 1) { set_last_error("wfs_spectral_resample: bad argument"); return WFS_ERR_INVALID; }
   for (int a = 0; a < 3; ++a)
     if (nsrc[a] < 1 || ndst[a] < 1) { set_last_error("wfs_spectral_resample: bad grid"); return WFS_ERR_INVALID; }
   if (src == dst) { set_last_error("wfs_spectral_resample: source and destination must differ"); return WFS_ERR_INVALID; }
+  int h[3] = {0, 0, 0};
+  if (shift)
+    for (int a = 0; a < 3; ++a) {
+      if (shift[a] < -nsrc[a] || shift[a] > nsrc[a]) {
+        set_last_error("wfs_spectral_resample: shift %d on axis %d beyond the source grid of %d", shift[a], a, nsrc[a]);
+        return WFS_ERR_INVALID;
+      }
+      h[a] = shift[a];
+    }
   const long n = long(ndst[0]) * ndst[1] * ndst[2] * batch;
   hipLaunchKernelGGL(resample_kernel, dim3(blocks_of(n)), dim3(kT), 0, static_cast<hipStream_t>(stream),
                      static_cast<const float2*>(src), static_cast<float2*>(dst), nsrc[0], nsrc[1], nsrc[2], ndst[0],
-                     ndst[1], ndst[2], batch, float(scale));
+                     ndst[1], ndst[2], batch, float(scale), h[0], h[1], h[2]);
   WFS_LAUNCH_CHECK();
   return 0;
 }

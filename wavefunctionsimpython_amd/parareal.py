@@ -221,7 +221,9 @@ class GridResampler:
     """Band-limited resampling between an n^3 and an nc^3 grid (complex64): ortho transform on
     one grid, the shared low frequencies copied (``wfs_spectral_resample``, zeros above the
     coarse band when going up), ortho inverse on the other.  Values of a function band-limited
-    to the coarse grid are kept exactly (up to float rounding)."""
+    to the coarse grid are kept exactly (up to float rounding).  With a shift s (integer
+    frequencies per axis) the coarse grid holds the band around fine frequency s instead of 0:
+    ``down`` returns the function times exp(-2 pi i s.j / n) resampled, ``up_add`` multiplies back."""
 
     def __init__(self, n: int, nc: int, device=None):
         self.n, self.nc = int(n), int(nc)
@@ -232,20 +234,22 @@ class GridResampler:
         self.fine_buf = torch.empty((self.n,) * 3, dtype=C64, device=dev)
         self.coarse_buf = torch.empty((self.nc,) * 3, dtype=C64, device=dev)
 
-    def down(self, x: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
-        """out (nc^3) = x (n^3) resampled; x is not changed."""
+    def down(self, x: torch.Tensor, out: torch.Tensor, shift=None) -> torch.Tensor:
+        """out (nc^3) = x (n^3) resampled (the band around fine frequency shift); x is not changed."""
         self.fine_buf.copy_(x)
         self.fine_plan.fft3d(self.fine_buf, -1)
-        _abi.spectral_resample(self.fine_buf, out)
+        _abi.spectral_resample(self.fine_buf, out, shift=shift)
         self.coarse_plan.fft3d(out, +1)
         return out
 
-    def up_add(self, xc: torch.Tensor, dst: torch.Tensor) -> None:
-        """dst (n^3, contiguous complex64) += xc (nc^3) resampled (xc is overwritten by its spectrum)."""
+    def up_add(self, xc: torch.Tensor, dst: torch.Tensor, shift=None) -> None:
+        """dst (n^3, contiguous complex64) += xc (nc^3) resampled, its band placed around fine
+        frequency shift (xc is overwritten by its spectrum)."""
         if dst.dtype != C64 or not dst.is_contiguous() or dst.numel() != self.fine_buf.numel():
             raise ValueError("dst must be a contiguous complex64 tensor of the fine grid")
         self.coarse_plan.fft3d(xc, -1)
-        _abi.spectral_resample(xc, self.fine_buf)
+        back = None if shift is None else tuple(-int(v) for v in shift)
+        _abi.spectral_resample(xc, self.fine_buf, shift=back)
         self.fine_plan.fft3d(self.fine_buf, +1)
         n2 = 2 * self.fine_buf.numel()
         _abi.check(_abi.lib().wfs_fno_axpby(_abi.ptr(dst), _abi.ptr(self.fine_buf), 1.0, 1.0, n2, _abi.stream_handle()),
@@ -261,15 +265,23 @@ class CoarseGridFNOCoarse:
     operator (4 layers, 12 modes, width 32 in the parareal workload) at nc^3 instead of n^3
     points: a call costs the base step, four transforms on each grid and one network pass at
     (nc/n)^3 of the points.  The correction is band-limited to the coarse grid.
+
+    ``shift`` (integer frequencies per axis, default none) centres the coarse band on a carrier
+    wave number, the packet's k0 L / (2 pi): the states enter the network as their envelopes
+    (times exp(-i k.r) for that carrier), whose spectrum the network's low modes cover, and the
+    correction is modulated back.  V enters unshifted (a pointwise factor commutes with the
+    modulation).
     """
 
     name = "fno_coarse_grid"
 
-    def __init__(self, net: FNO3d, V: torch.Tensor, pscale: float, vscale: float, base: SplitStepCoarse):
+    def __init__(self, net: FNO3d, V: torch.Tensor, pscale: float, vscale: float, base: SplitStepCoarse,
+                 shift=None):
         if net.cin != 5 or net.cout != 2 or not net.residual:
             raise ValueError("network must map 5 feature planes to 2 with a residual")
         self.net, self.base = net, base
         self.pscale, self.vscale = float(pscale), float(vscale)
+        self.shift = None if shift is None or not any(shift) else tuple(int(v) for v in shift)
         n = int(round(V.numel() ** (1.0 / 3.0)))
         if n ** 3 != V.numel() or net.n > n:
             raise ValueError("the potential's grid must be cubic and at least the network's")
@@ -285,13 +297,13 @@ class CoarseGridFNOCoarse:
 
     def features(self, src: torch.Tensor, base_out: torch.Tensor, Vc: torch.Tensor, a: torch.Tensor) -> None:
         """Network input at the coarse grid for state src with base output base_out (fine)."""
-        self.rs.down(base_out, self.base_c)
-        self.rs.down(src, self.src_c)
+        self.rs.down(base_out, self.base_c, self.shift)
+        self.rs.down(src, self.src_c, self.shift)
         pack_features(self.src_c, Vc, self.pscale, self.vscale, a, self.base_c)
 
     def target(self, psi: torch.Tensor, out: torch.Tensor) -> None:
         """Training target: psi (fine) resampled to the coarse grid, scaled planes."""
-        self.rs.down(psi, self.src_c)
+        self.rs.down(psi, self.src_c, self.shift)
         pack_target(self.src_c, self.pscale, out)
 
     def __call__(self, src: torch.Tensor, dst: torch.Tensor) -> None:
@@ -304,7 +316,7 @@ class CoarseGridFNOCoarse:
         _abi.check(lib.wfs_fno_axpby(_abi.ptr(self.corr), _abi.ptr(self.a), -1.0, 1.0, P2, s), "wfs_fno_axpby")
         _abi.check(lib.wfs_fno_unpack(_abi.ptr(self.corr), 1.0 / self.pscale, _abi.ptr(self.corr_c), self.Pc, s),
                    "wfs_fno_unpack")
-        self.rs.up_add(self.corr_c, dst)
+        self.rs.up_add(self.corr_c, dst, self.shift)
 
 
 # ---------------------------------------------------------------------------- parareal driver
