@@ -16,7 +16,7 @@ from wavefunctionsimpython_amd.sim_state import SimState, SimulationMethod
 pytestmark = pytest.mark.gpu
 
 
-def config1(tmp):
+def free_packet_64_config(tmp):
     """64^3 complex128 free packet (the free-packet workload)."""
     return {
         "simulation": {"method": "fourier", "delta_time_h_bar_per_hartree": 0.01, "total_iteration_count": 200,
@@ -37,8 +37,8 @@ def config1(tmp):
     }
 
 
-def test_config1_free_packet_matches_analytic_spread(dev, tmp_path):
-    cfg = config1(str(tmp_path))
+def test_free_packet_64_matches_analytic_spread(dev, tmp_path):
+    cfg = free_packet_64_config(str(tmp_path))
     sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
     assert sim.get_simulation_method() == SimulationMethod.FOURIER
     core_sim.run_iteration(sim, tools, it, observe_every=10)
@@ -65,8 +65,8 @@ def test_config1_free_packet_matches_analytic_spread(dev, tmp_path):
     assert np.array_equal(raw.reshape(20, 3), sd)
 
 
-def test_config1_state_matches_oracle(dev, tmp_path):
-    cfg = config1(str(tmp_path))
+def test_free_packet_64_state_matches_oracle(dev, tmp_path):
+    cfg = free_packet_64_config(str(tmp_path))
     cfg["simulation"]["total_iteration_count"] = 30
     sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
     psi0 = sim.get_wave_function().cpu().numpy()
@@ -232,7 +232,7 @@ def test_snapshot_roundtrip_resumes_psi(dev, tmp_path):
 
 
 def test_expected_location_mutates_top_voxel_like_reference(dev, tmp_path):
-    cfg = config1(str(tmp_path))
+    cfg = free_packet_64_config(str(tmp_path))
     sim = SimState(cfg)
     top_before = sim.get_observation_box_top_corner_voxel_3().copy()
     sim.initialize_state()

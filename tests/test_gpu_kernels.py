@@ -63,7 +63,7 @@ def test_fft3d_delta_and_constant(dev):
     assert np.allclose(got, 1.0 / np.sqrt(16 ** 3), atol=1e-15)
 
 
-def _config1_state(prec):
+def _free_packet_64_state(prec):
     N, L = 64, 60.0
     dx = [L / N] * 3
     dtype = np.complex128 if prec == _abi.WFS_C128 else np.complex64
@@ -73,7 +73,7 @@ def _config1_state(prec):
 
 def test_split_step_free_packet_c128(dev):
     """64^3 complex128 free packet, 200 steps of dt = 0.01 (the free-packet workload)."""
-    shape, dx, psi0 = _config1_state(_abi.WFS_C128)
+    shape, dx, psi0 = _free_packet_64_state(_abi.WFS_C128)
     V = np.zeros(shape, np.complex64)
     p = _plan(shape, dx, 0.01, 1.0, _abi.WFS_C128)
     t = _to_dev(psi0, dev)

@@ -270,7 +270,7 @@ def test_gaussian_moments_and_norm():
 
 
 def test_box_voxel_rules_hand_worked():
-    # config 1: L = 60, N = 64, dx = 0.9375; observation box +-25
+    # the 64^3 free packet: L = 60, N = 64, dx = 0.9375; observation box +-25
     b, t, bv, tv = oracle.observation_box_voxels([60.0] * 3, [64] * 3, [-25.0] * 3, [25.0] * 3)
     assert list(bv) == [5, 5, 5] and list(tv) == [58, 58, 58]
     bE, cnt, tv2 = oracle.expected_location_box(bv, tv)
