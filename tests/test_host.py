@@ -185,3 +185,32 @@ def test_bench_distinct_plane_share_and_grouped_observable_error():
     assert bench.observables_rel_err(obs, ref) == pytest.approx(1e-9 / 16.0)
     obs[..., 4] += 3e-4
     assert bench.observables_rel_err(obs, ref) == pytest.approx(1e-6)
+
+
+def test_duplicate_measurement_names_take_the_first_free_suffix(capsys):
+    """The rename rule for two measurement tools of one kind with equal names: the later one
+    gets the first suffix _1, _2, ... that no item has (sources/measurement.py:502-518)."""
+    from wavefunctionsimpython_amd.measurement import MeasurementTools
+
+    class Named:
+        def __init__(self, name):
+            self.name = name
+
+        def get_name(self):
+            return self.name
+
+        def set_name(self, name):
+            self.name = name
+
+    items = [Named("slit"), Named("slit_1"), Named("slit_3"), Named("screen")]
+    dup = Named("slit")
+    MeasurementTools._resolve_naming_conflicts(items, dup)
+    assert dup.get_name() == "slit_2"
+    out = capsys.readouterr().out
+    assert 'equal name: "slit"' in out and 'Renamed one to "slit_2".' in out
+    fresh = Named("detector")
+    MeasurementTools._resolve_naming_conflicts(items, fresh)
+    assert fresh.get_name() == "detector" and capsys.readouterr().out == ""
+    single = Named("screen")
+    MeasurementTools._resolve_naming_conflicts(items, single)
+    assert single.get_name() == "screen_1"

@@ -11,6 +11,7 @@ package; their configuration keys are accepted and ignored.
 
 from __future__ import annotations
 
+import itertools
 import os
 from typing import Dict, List
 
@@ -188,23 +189,16 @@ class MeasurementTools:
 
     @staticmethod
     def _resolve_naming_conflicts(items, new_item):
-        """Append _1, _2, ... to a duplicate name (sources/measurement.py:502-518)."""
-        for item in items:
-            if item.get_name() == new_item.get_name():
-                print(_RED + f"Found multiple measurement tools of the same type with equal name: "
-                      f"\"{new_item.get_name()}\"" + _RESET)
-                index = 1
-                unique = False
-                while not unique:
-                    unique = True
-                    for item2 in items:
-                        if new_item.get_name() + f"_{index}" == item2.get_name():
-                            unique = False
-                            index += 1
-                            break
-                new_item.set_name(new_item.get_name() + f"_{index}")
-                print(_RED + f"Renamed one to \"{new_item.get_name()}\"." + _RESET)
-                break
+        """A name an item of the same kind already has gets the first free suffix _1, _2, ...,
+        with the reference's two warnings (rename rule of sources/measurement.py:502-518)."""
+        taken = {item.get_name() for item in items}
+        name = new_item.get_name()
+        if name not in taken:
+            return
+        print(_RED + f"Found multiple measurement tools of the same type with equal name: \"{name}\"" + _RESET)
+        suffix = next(k for k in itertools.count(1) if f"{name}_{k}" not in taken)
+        new_item.set_name(f"{name}_{suffix}")
+        print(_RED + f"Renamed one to \"{new_item.get_name()}\"." + _RESET)
 
     def set_observe_every(self, k: int):
         """Samples are k steps apart (time integrals of the series use k * dt)."""
