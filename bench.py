@@ -357,15 +357,17 @@ def batch_bench(steps, warmup):
     }
 
 
-def parareal_bench(train_pairs, seed_train=1, seed_test=2, coarse_n=32, full_fno=False):
+def parareal_bench(train_pairs, seed_train=1, seed_test=2, coarse_n=32, full_fno=False, epochs=10):
     """Time-to-T on the 128^3 harmonic + cos^2 lattice problem: 32 slices x 50 steps (dt 0.005).
 
     (i) plain serial split-step; (ii) parareal with a one-step split-step coarse propagator;
     (iii) parareal with the FNO correction of that coarse step, the network (4 layers, 12
     modes, width 32) run on a coarse_n^3 grid with the features resampled spectrally
     (CoarseGridFNOCoarse); with full_fno also (iv) the same network on the full 128^3 grid.
-    Each network is trained first on solver pairs (40 trajectories from default_rng(1), online,
-    one pass); training time is reported separately.  Errors are relative L2 and phase against
+    Each network is trained first on solver pairs (40 trajectories from default_rng(1)); with
+    epochs > 1 the pairs are stored at the network's grid and visited `epochs` times, half of the
+    batches replaced by random superpositions of two stored pairs (epochs = 1: online, one pass);
+    training time is reported separately.  Errors are relative L2 and phase against
     the GPU complex128 run of the same problem.  The cost of one coarse call is reported beside
     the 50-step fine slice it stands in for.
     """
@@ -395,8 +397,8 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2, coarse_n=32, full_fno
                                           pr.SplitStepCoarse(probs[0].shape, (dx,) * 3, S * probs[0].dt, 1.0, V0, 1))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        losses = pr.train_coarse(net, pairs, slices_train, pscale, vscale, lr=1e-3, lr_final=1e-4, epochs=1,
-                                 log=log, featurizer=feat)
+        losses = pr.train_coarse(net, pairs, slices_train, pscale, vscale, lr=1e-3, lr_final=1e-4, epochs=epochs,
+                                 mix=0.5 if epochs > 1 else 0.0, log=log, featurizer=feat)
         torch.cuda.synchronize()
         k = max(1, len(losses) // 10)
         info = {"pairs": len(losses), "time_s": time.perf_counter() - t0,
@@ -483,16 +485,27 @@ def parareal_bench(train_pairs, seed_train=1, seed_test=2, coarse_n=32, full_fno
         par.serial.step(dst, None, S)
 
     slice_ms = call_ms(fine_slice, psi0, tmp)
+    # the fine slices of the test run's first eight slice boundaries: the coarse propagators'
+    # error per slice, |G(u_n) - F(u_n)| / |F(u_n)|, shows what the network adds to its base
+    bounds = [psi0.clone()]
+    for _ in range(8):
+        nxt = torch.empty_like(psi0)
+        fine_slice(bounds[-1], nxt)
+        bounds.append(nxt)
     out = {}
     for name, c in coarse.items():
         par.run(psi0, c, tol=1e-4)              # warm-up
         res = par.run(psi0, c, tol=1e-4)
         ms = call_ms(c, psi0, tmp)
         ms_eager = call_ms(eager[name], psi0, tmp)
+        slice_err = []
+        for k in range(8):
+            c(bounds[k], tmp)
+            slice_err.append(float(torch.linalg.vector_norm(tmp - bounds[k + 1]) / torch.linalg.vector_norm(bounds[k + 1])))
         out[name] = {"time_to_T_s": res.timings["total_s"], "iterations": res.iterations,
                      "converged": res.converged, "fallback": res.fallback, **errs(res.state),
                      "history": res.history, "coarse_call_ms": ms, "coarse_call_over_fine_slice": ms / slice_ms,
-                     "coarse_call_ms_eager": ms_eager,
+                     "coarse_call_ms_eager": ms_eager, "slice_rel_l2_vs_fine_mean": float(np.mean(slice_err)),
                      "phases_s": {k: v for k, v in res.timings.items() if k != "total_s"}}
     # the networks alone: inference forward time, matrix-core and HBM rates
     fno_perf = {}
@@ -912,6 +925,8 @@ def main():
                          "(the 256^3 double slit through initialisation + core_sim.run_iteration), fno (the "
                          "FNO forward at 128^3 and 32^3) or parareal512 (time-to-T of the 512^3 configuration)")
     ap.add_argument("--train-pairs", type=int, default=2000, help="FNO training pairs (parareal workload)")
+    ap.add_argument("--train-epochs", type=int, default=10,
+                    help="visits of the stored training pairs (parareal workload; 1 = online, one pass)")
     ap.add_argument("--coarse-n", type=int, default=32, help="grid of the coarse-grid FNO (parareal workload)")
     ap.add_argument("--full-fno", action="store_true", help="also run the FNO on the full 128^3 grid (parareal)")
     ap.add_argument("--carrier", action="store_true",
@@ -940,7 +955,8 @@ def main():
         print(json.dumps(batch_bench(args.steps, args.warmup)), flush=True)
         return
     if args.workload == "parareal":
-        print(json.dumps(parareal_bench(args.train_pairs, coarse_n=args.coarse_n, full_fno=args.full_fno)), flush=True)
+        print(json.dumps(parareal_bench(args.train_pairs, coarse_n=args.coarse_n, full_fno=args.full_fno,
+                                        epochs=args.train_epochs)), flush=True)
         return
     if args.workload == "large":
         print(json.dumps(large_bench(args.steps)), flush=True)
