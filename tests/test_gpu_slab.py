@@ -200,13 +200,13 @@ def test_profile_times_lie_within_the_calls(dev):
     than the call's own wall time on the stream (events around the whole call), and the two
     launches of a step account for most of a long call.  The observables class (moments and
     their sum, two launches) counts one timed pair per call."""
-    shape = (32, 256, 256)
+    shape = (128, 256, 256)
     psi0, V = _inputs(shape, 46)
     p = _abi.Plan(shape, DX, DT, 1.0, C64, _abi.WFS_METHOD_FOURIER)
     psi = torch.from_numpy(psi0).to(dev).to(torch.complex64)
     p.set_potential(torch.from_numpy(V).to(dev))
     p.step(psi, None, 2)
-    boxes = [((0, 0, 0), (31, 255, 255)), ((4, 10, 20), (9, 31, 63))]
+    boxes = [((0, 0, 0), (127, 255, 255)), ((4, 10, 20), (9, 31, 63))]
     out = p.observables(psi, boxes)
     torch.cuda.synchronize()
     p.set_profiling(True)
