@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kObsThreads) void box_moments_kernel(const cx<R>* _
     if ((lane & 7) == 0 && q < kObsQ) red[wave][b * kObsQ + q] = v;
   }
   __syncthreads();
-  if (threadIdx.x < set.n * kObsQ) {
+  if (int(threadIdx.x) < set.n * kObsQ) {
     double v = 0.0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];

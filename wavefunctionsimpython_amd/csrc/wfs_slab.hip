@@ -55,17 +55,18 @@ __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
   return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
 
-// ZB = 8 runs four waves per SIMD (at most 128 VGPRs: the x transforms' table twiddles are read
-// from LDS one at a time where they are applied, so no twiddle array is held; at nx = 256 one
-// spilled register, 52.4 against 55.8 us per launch with three waves, bit-identical).
+// ZB = 8 runs four waves per SIMD where a lane holds at most 16 points of an x line (at most 128
+// VGPRs: the x transforms' table twiddles are read from LDS one at a time where they are applied,
+// so no twiddle array is held; at nx = 256 one spilled register, 52.4 against 55.8 us per launch
+// with three waves, bit-identical); the 24 points of nx = 96 or 192 do not fit in 128.
 // ZB = 8: a block reads and writes 64-byte halves of 128-byte lines; the two blocks of a line
 // get the block indices 16 g + x and 16 g + 8 + x, which the dispatcher sends to the same XCD
 // (blocks go round-robin over the 8 XCDs) one after the other, so the second half comes from
 // that XCD's L2.
 template <typename R, int NX, int NY, int NZ, int B, int ZB>
 __global__ __launch_bounds__(2 * ZB * split_of<NX>::n2)
-__attribute__((amdgpu_waves_per_eu(ZB == 8 ? 4 : 1))) void x_slab_kernel(cx<R>* __restrict__ data,
-                                                                           const XSlabParams p) {
+__attribute__((amdgpu_waves_per_eu(ZB == 8 && NX / split_of<NX>::n2 <= 16 ? 4 : 1))) void x_slab_kernel(
+    cx<R>* __restrict__ data, const XSlabParams p) {
   constexpr int N2 = split_of<NX>::n2;
   constexpr int N1 = NX / N2;
   constexpr int A = NY / B;
