@@ -55,6 +55,21 @@ __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
   return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
 
+// the partner's value, partner = tid ^ ZB: for ZB = 8 the partner sits in the same row of 16
+// lanes, 8 along, so a DPP row rotation by 8 (row_ror:8, one VALU move per part) replaces the
+// LDS-unit permute; other widths use the permute
+template <int ZB, typename R>
+__device__ __forceinline__ cx<R> partner_c(cx<R> v) {
+  if constexpr (ZB == 8 && sizeof(R) == 4) {
+    constexpr int kRowRor8 = 0x128;
+    const int x = __builtin_amdgcn_update_dpp(0, __float_as_int(v.x), kRowRor8, 0xF, 0xF, false);
+    const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(v.y), kRowRor8, 0xF, 0xF, false);
+    return mk<R>(__int_as_float(x), __int_as_float(y));
+  } else {
+    return shfl_xor_c<R>(v, ZB);
+  }
+}
+
 // ZB = 8 runs four waves per SIMD where a lane holds at most 16 points of an x line (at most 128
 // VGPRs: the x transforms' table twiddles are read from LDS one at a time where they are applied,
 // so no twiddle array is held; at nx = 256 one spilled register, 52.4 against 55.8 us per launch
@@ -131,7 +146,7 @@ __attribute__((amdgpu_waves_per_eu(ZB == 8 && NX / split_of<NX>::n2 <= 16 ? 4 : 
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const C z = cmul(a[0][i], wq[0]);
-      const C recv = shfl_xor_c<R>(z, ZB);
+      const C recv = partner_c<ZB, R>(z);
       a[0][i] = mk<R>(fma_r(sgn, z.x, recv.x), fma_r(sgn, z.y, recv.y));
     }
   } else {
@@ -144,7 +159,7 @@ __attribute__((amdgpu_waves_per_eu(ZB == 8 && NX / split_of<NX>::n2 <= 16 ? 4 : 
       const C z0 = cmul(a[0][i], wq[0]);
       const C z1 = cmul(a[RPL - 1][i], wq[RPL - 1]);
       const C keep = cadd(z0, z1);
-      const C recv = shfl_xor_c<R>(csub(z0, z1), ZB);
+      const C recv = partner_c<ZB, R>(csub(z0, z1));
       const C kk = q ? mk<R>(keep.y, -keep.x) : keep;     // W_4^q keep, W_4 = -i
       a[0][i] = cadd(kk, recv);
       a[RPL - 1][i] = csub(kk, recv);
@@ -190,7 +205,7 @@ __attribute__((amdgpu_waves_per_eu(ZB == 8 && NX / split_of<NX>::n2 <= 16 ? 4 : 
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const C v = a[0][i];
-      const C recv = shfl_xor_c<R>(v, ZB);
+      const C recv = partner_c<ZB, R>(v);
       a[0][i] = cmulc(mk<R>(fma_r(sgn, v.x, recv.x), fma_r(sgn, v.y, recv.y)), wq[0]);
     }
   } else {
@@ -201,7 +216,7 @@ __attribute__((amdgpu_waves_per_eu(ZB == 8 && NX / split_of<NX>::n2 <= 16 ? 4 : 
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const C keep = cadd(a[0][i], a[RPL - 1][i]);
-      const C recv = shfl_xor_c<R>(csub(a[0][i], a[RPL - 1][i]), ZB);
+      const C recv = partner_c<ZB, R>(csub(a[0][i], a[RPL - 1][i]));
       const C kk = q ? mk<R>(-keep.y, keep.x) : keep;     // W_4^{-q} keep, W_4^{-1} = +i
       a[0][i] = cmulc(cadd(kk, recv), wq[0]);
       a[RPL - 1][i] = cmulc(csub(kk, recv), wq[RPL - 1]);
