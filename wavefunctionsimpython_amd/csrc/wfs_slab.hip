@@ -57,7 +57,10 @@ __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m) {
 
 // the partner's value, partner = tid ^ ZB: for ZB = 8 the partner sits in the same row of 16
 // lanes, 8 along, so a DPP row rotation by 8 (row_ror:8, one VALU move per part) replaces the
-// LDS-unit permute; other widths use the permute
+// LDS-unit permute; for ZB = 16 it sits in the neighbouring row, and gfx950's row swap
+// (v_permlane16_swap: odd rows of the first operand with even rows of the second, both the
+// value itself) leaves it in one of the two results, picked by the lane's row; other widths use
+// the permute
 template <int ZB, typename R>
 __device__ __forceinline__ cx<R> partner_c(cx<R> v) {
   if constexpr (ZB == 8 && sizeof(R) == 4) {
@@ -65,6 +68,11 @@ __device__ __forceinline__ cx<R> partner_c(cx<R> v) {
     const int x = __builtin_amdgcn_update_dpp(0, __float_as_int(v.x), kRowRor8, 0xF, 0xF, false);
     const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(v.y), kRowRor8, 0xF, 0xF, false);
     return mk<R>(__int_as_float(x), __int_as_float(y));
+  } else if constexpr (ZB == 16 && sizeof(R) == 4) {
+    const bool odd_row = threadIdx.x & 16;
+    const auto sx = __builtin_amdgcn_permlane16_swap(__float_as_uint(v.x), __float_as_uint(v.x), false, false);
+    const auto sy = __builtin_amdgcn_permlane16_swap(__float_as_uint(v.y), __float_as_uint(v.y), false, false);
+    return mk<R>(__uint_as_float(odd_row ? sx[0] : sx[1]), __uint_as_float(odd_row ? sy[0] : sy[1]));
   } else {
     return shfl_xor_c<R>(v, ZB);
   }
