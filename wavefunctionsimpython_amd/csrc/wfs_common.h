@@ -178,6 +178,39 @@ __device__ __forceinline__ T twiddle_ct(T a) {
   }
 }
 
+// The value of lane (lane ^ M) of the wavefront, M in {1, 2, 4, 8, 16, 32}, by cross-lane VALU
+// moves instead of LDS-unit permutes (exact moves, so sums formed from them are unchanged):
+// DPP quad permutations for 1 and 2; for 4, row rotations by 4 and 12 written to alternate
+// 4-lane banks; a row rotation by 8 for 8; gfx950's row and half-wave swaps of the value with
+// itself (v_permlane16_swap, v_permlane32_swap) for 16 and 32, each leaving the partner's value
+// in one of its two results, picked by the lane.
+template <int M>
+__device__ __forceinline__ int xor_lane_b32(int v) {
+  if constexpr (M == 1) {
+    return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);      // quad_perm [1, 0, 3, 2]
+  } else if constexpr (M == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);      // quad_perm [2, 3, 0, 1]
+  } else if constexpr (M == 4) {
+    const int odd = __builtin_amdgcn_update_dpp(0, v, 0x124, 0xF, 0xA, false);   // row_ror:4, banks 1, 3
+    return __builtin_amdgcn_update_dpp(odd, v, 0x12C, 0xF, 0x5, false);          // row_ror:12, banks 0, 2
+  } else if constexpr (M == 8) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);     // row_ror:8
+  } else if constexpr (M == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(unsigned(v), unsigned(v), false, false);
+    return int((__lane_id() & 16) ? r[0] : r[1]);
+  } else {
+    static_assert(M == 32, "xor distance 1, 2, 4, 8, 16 or 32");
+    const auto r = __builtin_amdgcn_permlane32_swap(unsigned(v), unsigned(v), false, false);
+    return int((__lane_id() & 32) ? r[0] : r[1]);
+  }
+}
+template <int M>
+__device__ __forceinline__ double xor_lane(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = xor_lane_b32<M>(int(b & 0xffffffffLL)), hi = xor_lane_b32<M>(int(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
 // Sums of seven values over the 64 lanes of a wavefront by recursive halving: at the xor-32,
 // -16 and -8 steps each lane keeps half of its remaining values and sends the other half to
 // its partner, then xor 4, 2, 1 finish one value; 10 exchanges instead of 7 x 6.  Lane l
@@ -188,13 +221,14 @@ __device__ __forceinline__ double wave_sum7(const double (&v)[7], int lane) {
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const double lo = v[j], hi = j + 4 < 7 ? v[j + 4] : 0.0;
-    y[j] = (h5 ? hi : lo) + __shfl_xor(h5 ? lo : hi, 32, 64);
+    y[j] = (h5 ? hi : lo) + xor_lane<32>(h5 ? lo : hi);
   }
 #pragma unroll
-  for (int j = 0; j < 2; ++j) z[j] = (h4 ? y[j + 2] : y[j]) + __shfl_xor(h4 ? y[j] : y[j + 2], 16, 64);
-  double s = (h3 ? z[1] : z[0]) + __shfl_xor(h3 ? z[0] : z[1], 8, 64);
-#pragma unroll
-  for (int off = 4; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  for (int j = 0; j < 2; ++j) z[j] = (h4 ? y[j + 2] : y[j]) + xor_lane<16>(h4 ? y[j] : y[j + 2]);
+  double s = (h3 ? z[1] : z[0]) + xor_lane<8>(h3 ? z[0] : z[1]);
+  s += xor_lane<4>(s);
+  s += xor_lane<2>(s);
+  s += xor_lane<1>(s);
   return s;
 }
 
