@@ -120,6 +120,17 @@ long wfs_plan_get_step_index(wfs_plan* plan);
 int wfs_set_potential(wfs_plan* plan, const void* V, void* stream);
 
 /*
+ * Rebuild x planes [x_begin, x_end) of the stored table from V (no reference counterpart: the
+ * dynamic-potential mode of SimState.update_potential, sources/sim_state.py:466-509, rebuilds V
+ * whole each step; where only the planes a moving wall sweeps change, this refreshes only
+ * those).  The caller promises that V differs from the V of the table's last build in those
+ * planes only; the result is then the table wfs_set_potential(V) builds, bit for bit.  Without
+ * a table of the plan's current kind it builds the whole table.  One packet only
+ * (WFS_ERR_UNSUPPORTED for batched plans); 0 <= x_begin <= x_end <= n[0].
+ */
+int wfs_set_potential_planes(wfs_plan* plan, const void* V, int x_begin, int x_end, void* stream);
+
+/*
  * Separable real potential V(x, y, z) = fx(x) + fy(y) + fz(z) (no reference
  * counterpart; harmonic wells and cos^2 lattices are of this form).  fx, fy,
  * fz are host arrays of batch * n[d] doubles (Hartree, packet-major).  The plan
@@ -207,6 +218,15 @@ int wfs_add_cap(const int n[3], const double dx[3], const double bottom[3], cons
                 const double exp_neg[3], void* V, void* stream);
 
 /*
+ * wfs_add_cap on x planes [x_begin, x_end) of the n-grid V only (no reference counterpart; for
+ * wfs_set_potential_planes): those planes get exactly the values of the whole-grid call, the
+ * others are not touched.  0 <= x_begin <= x_end <= n[0].
+ */
+int wfs_add_cap_planes(const int n[3], const double dx[3], const double bottom[3], const double top[3],
+                       double offset, const double pot_pos[3], const double pot_neg[3], const double exp_pos[3],
+                       const double exp_neg[3], int x_begin, int x_end, void* V, void* stream);
+
+/*
  * Potential wall with 0-4 slits: V += pot (potential_wall.cu:4-83, launched by
  * PotentialWall.add_potential, sources/potential.py:201-242).  normal must be
  * unit length.
@@ -214,6 +234,15 @@ int wfs_add_cap(const int n[3], const double dx[3], const double bottom[3], cons
 int wfs_add_wall(const int n[3], const double dx[3], double potential, const double center[3],
                  const double normal[3], double plateau, double slope, double slope_exponent, int slit_count,
                  double slit_spacing, double slit_width, double slit_rotation, void* V, void* stream);
+
+/*
+ * wfs_add_wall on x planes [x_begin, x_end) of the n-grid V only (no reference counterpart; for
+ * wfs_set_potential_planes), as wfs_add_cap_planes.
+ */
+int wfs_add_wall_planes(const int n[3], const double dx[3], double potential, const double center[3],
+                        const double normal[3], double plateau, double slope, double slope_exponent,
+                        int slit_count, double slit_spacing, double slit_width, double slit_rotation, int x_begin,
+                        int x_end, void* V, void* stream);
 
 /*
  * Harmonic potential V += 1/2 m w^2 |r - c|^2 (formula of the uncalled helper

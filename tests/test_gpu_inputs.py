@@ -155,3 +155,140 @@ def test_moving_wall_advects_clamps_and_regenerates(dev, tmp_path):
     ea = tools.get_expected_location().get_expected_location_evolution()
     eb = tools_b.get_expected_location().get_expected_location_evolution()
     assert np.array_equal(ea, eb)
+
+
+# --- plane-wise regeneration of a moving wall (no reference counterpart: the reference rebuilds
+# V whole every step, sources/sim_state.py:466-509; the plane-wise path must give the same V,
+# table and state bit for bit) ---
+
+_WALL_ARGS = (2.5, (0.4, 0.2, -0.1), (1.0, 0.0, 0.0), 1.2, 0.6, 2.0, 2, 1.5, 0.4, 0.3)
+_CAP_ARGS = ((-3.0, -3.0, -3.0), (3.0, 2.5, 3.5), 0.5, (-20.0, -30.0, -25.0), (-22.0, -18.0, -26.0),
+             (2.0, 3.0, 2.5), (1.5, 2.0, 3.0))
+
+
+def test_wall_and_absorber_on_plane_windows(dev):
+    """wfs_add_wall_planes / wfs_add_cap_planes: the window's planes get the whole-grid values bit
+    for bit, the other planes are not touched; windows outside the grid are refused."""
+    from wavefunctionsimpython_amd import _abi
+    shape, dx = (40, 24, 32), (0.3, 0.4, 0.35)
+    rng = np.random.default_rng(5)
+    V0 = torch.from_numpy((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)).to(dev)
+    full = V0.clone()
+    _abi.add_wall(shape, dx, *_WALL_ARGS, full)
+    _abi.add_cap(shape, dx, *_CAP_ARGS, full)
+    assert not torch.equal(full[15:30].real, V0[15:30].real)      # the wall lies in planes 15..29
+    for lo, hi in ((0, 40), (15, 30), (7, 19), (0, 1), (39, 40), (12, 12)):
+        part = V0.clone()
+        _abi.add_wall(shape, dx, *_WALL_ARGS, part, planes=(lo, hi))
+        _abi.add_cap(shape, dx, *_CAP_ARGS, part, planes=(lo, hi))
+        torch.cuda.synchronize()
+        assert torch.equal(part[lo:hi], full[lo:hi]), (lo, hi)
+        assert torch.equal(part[:lo], V0[:lo]) and torch.equal(part[hi:], V0[hi:]), (lo, hi)
+    for bad in ((-1, 3), (5, 4), (0, 41)):
+        with pytest.raises(_abi.WfsError):
+            _abi.add_wall(shape, dx, *_WALL_ARGS, V0.clone(), planes=bad)
+        with pytest.raises(_abi.WfsError):
+            _abi.add_cap(shape, dx, *_CAP_ARGS, V0.clone(), planes=bad)
+
+
+@pytest.mark.parametrize("shape,precision,slab", [((64, 128, 128), "c64", 1), ((64, 128, 128), "c64", 0),
+                                                  ((32, 32, 32), "c128", 0)])
+def test_table_planes_rebuilt_equal_the_whole_rebuild(dev, shape, precision, slab):
+    """wfs_set_potential_planes after a change of planes [10, 26) steps like a plan whose table was
+    built from the new V whole -- including the map of repeated planes (runs that split, merge
+    and end at the window's edges)."""
+    from wavefunctionsimpython_amd import _abi
+    prec = _abi.WFS_C64 if precision == "c64" else _abi.WFS_C128
+    dt = torch.complex64 if precision == "c64" else torch.complex128
+    rng = np.random.default_rng(11)
+    base = (rng.uniform(-1, 1, shape[1:]) - 0.1j * rng.uniform(0, 1, shape[1:])).astype(np.complex64)
+    V1 = np.broadcast_to(base, shape).copy()                       # one run of identical planes
+    V1[10:14] = base + 0.5                                         # a second run, inside the window
+    V1[30] = base - 0.25
+    V2 = V1.copy()
+    V2[10:14] = base                                               # merges back into the first run
+    V2[20:26] = base * 1.5 + 0.1j                                  # a new run ending at the window's edge
+    V2[23] = base * 0.5
+    V1, V2 = torch.from_numpy(V1).to(dev), torch.from_numpy(V2).to(dev)
+    psi0 = torch.from_numpy((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.1).to(dev, dt)
+    dx = (0.3, 0.25, 0.25)
+    plans = []
+    for _ in range(2):
+        p = _abi.Plan(shape, dx, 0.01, 1.0, prec, _abi.WFS_METHOD_FOURIER)
+        if precision == "c64":
+            p.set_variant("yz_slab", slab)
+        plans.append(p)
+    a, b = plans
+    a.set_potential(V1)
+    a.set_potential_planes(V2, 10, 26)
+    b.set_potential(V2)
+    pa, pb = psi0.clone(), psi0.clone()
+    a.step(pa, None, 4)
+    b.step(pb, None, 4)
+    torch.cuda.synchronize()
+    assert torch.equal(pa, pb)
+    # and again after a second window, on a table the first window already changed
+    V3 = V2.clone()
+    V3[26:28] = V3[0] * 2.0
+    a.set_potential_planes(V3, 26, 28)
+    b.set_potential(V3)
+    a.step(pa, None, 3)
+    b.step(pb, None, 3)
+    torch.cuda.synchronize()
+    assert torch.equal(pa, pb)
+    with pytest.raises(_abi.WfsError):
+        a.set_potential_planes(V3, 5, shape[0] + 1)
+
+
+def _sweeping_wall_config(tmp, n, double):
+    L = 16.0 if n <= 64 else 32.0
+    cfg = _base_config(tmp, n=n, L=L, double=double)
+    cfg["simulation"]["total_iteration_count"] = 10
+    cfg["wave_packet"]["initial_position_bohr_radii_3"] = [-2.0, 0.0, 0.0]
+    cfg["wave_packet"]["initial_velocity_bohr_radii_hartree_per_h_bar_3"] = [2.0, 0.0, 0.0]
+    cfg["volume"]["observation_box_bottom_corner_bohr_radii_3"] = [-0.35 * L] * 3
+    cfg["volume"]["observation_box_top_corner_bohr_radii_3"] = [0.35 * L] * 3
+    cfg["absorbing_boundary_condition"] = {"enable": True, "start_offset": 0.0}
+    cfg["potential"] = {"walls": [
+        {"potential_hartree": 5.0, "center_bohr_radii_3": [1.0, 0.2, 0.0], "normal_vector_3": [1.0, 0.0, 0.0],
+         "plateau_thickness_bohr_radii": 0.8, "slope_thickness_bohr_radii": 0.4, "slope_exponent": 2.0,
+         "slit_count": 2, "slit_spacing_bohr_radii": 1.5, "slit_width_bohr_radii": 0.5,
+         "velocity_bohr_radius_hartree_per_h_bar_3": [30.0, 5.0, 0.0]},
+        {"potential_hartree": 2.0, "center_bohr_radii_3": [-4.0, 0.0, 0.0], "normal_vector_3": [-1.0, 0.0, 0.0],
+         "plateau_thickness_bohr_radii": 0.5, "slope_thickness_bohr_radii": 0.0, "slit_count": 1,
+         "slit_width_bohr_radii": 1.0}]}
+    return cfg
+
+
+@pytest.mark.parametrize("n,double", [(64, True), (128, False)])
+def test_moving_wall_rebuilds_only_the_planes_it_sweeps(dev, tmp_path, monkeypatch, n, double):
+    """A wall moving along its x normal (beside a static one): after the first whole rebuild only
+    the planes it covered before or covers after each move are rebuilt, and V and the state equal
+    those of whole rebuilds bit for bit (complex128 on the plane pass, complex64 on the slab scheme)."""
+    from wavefunctionsimpython_amd import _abi
+    calls = []
+    original = _abi.Plan.set_potential_planes
+
+    def spy(self, V, lo, hi, stream=None):
+        calls.append((lo, hi))
+        return original(self, V, lo, hi, stream)
+
+    monkeypatch.setattr(_abi.Plan, "set_potential_planes", spy)
+    out = []
+    for partial in (True, False):
+        sim, tools, it = initialisation.initialize(use_cache=False,
+                                                   config=_sweeping_wall_config(str(tmp_path / str(partial)), n, double))
+        sim.partial_regeneration = partial
+        for k in range(it.total_iteration_count):
+            it.i = k
+            core_sim.time_step(sim, tools, it)
+        out.append((sim.get_wave_function().clone(), sim.get_potential().clone()))
+        if partial:
+            # iteration 0 rebuilds whole (the first update); iterations 2.. refresh the planes the
+            # previous update swept (each evolve_state refreshes before stepping)
+            assert len(calls) == it.total_iteration_count - 2
+            assert all(0 < hi - lo <= n // 4 for lo, hi in calls), calls
+            calls.clear()
+    assert not calls
+    assert torch.equal(out[0][1], out[1][1])
+    assert torch.equal(out[0][0], out[1][0])

@@ -81,6 +81,11 @@ SIGNATURES = {
     "wfs_add_wall": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, _c_dbl3, _c_dbl3, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "wfs_add_cap_planes": (ctypes.c_int, [_c_int3, _c_dbl3, _c_dbl3, _c_dbl3, ctypes.c_double, _c_dbl3, _c_dbl3,
+                                          _c_dbl3, _c_dbl3, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "wfs_add_wall_planes": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, _c_dbl3, _c_dbl3, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "wfs_add_harmonic": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, _c_dbl3, _vp, _vp]),
     "wfs_add_lattice": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, _vp, _vp]),
     "wfs_kinetic_operator": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -90,6 +95,7 @@ SIGNATURES = {
     "wfs_plan_set_step_index": (ctypes.c_int, [_vp, ctypes.c_long]),
     "wfs_plan_get_step_index": (ctypes.c_long, [_vp]),
     "wfs_set_potential": (ctypes.c_int, [_vp, _vp, _vp]),
+    "wfs_set_potential_planes": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "wfs_set_potential_axes": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "wfs_plan_autotune": (ctypes.c_int, [_vp, _vp]),
     "wfs_plan_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
@@ -244,6 +250,15 @@ class Plan:
     def set_potential(self, V: torch.Tensor, stream=None) -> None:
         check_potential(V, self.shape)
         check(lib().wfs_set_potential(self._h, ptr(V), stream_handle(stream)), "wfs_set_potential")
+        self._potential_set = True
+        self._V_ref = V if self.method == WFS_METHOD_POWER_SERIES else None
+
+    def set_potential_planes(self, V: torch.Tensor, x_begin: int, x_end: int, stream=None) -> None:
+        """Rebuild x planes [x_begin, x_end) of the stored table from V, which must differ from the V
+        of the table's last build in those planes only (one packet)."""
+        check_potential(V, self.shape)
+        check(lib().wfs_set_potential_planes(self._h, ptr(V), int(x_begin), int(x_end), stream_handle(stream)),
+              "wfs_set_potential_planes")
         self._potential_set = True
         self._V_ref = V if self.method == WFS_METHOD_POWER_SERIES else None
 
@@ -579,19 +594,33 @@ def observables_noalloc(n, dx, precision: int, psi: torch.Tensor, boxes, out: to
     return out
 
 
-def add_cap(n, dx, bottom, top, offset, pot_pos, pot_neg, exp_pos, exp_neg, V: torch.Tensor, stream=None) -> None:
+def add_cap(n, dx, bottom, top, offset, pot_pos, pot_neg, exp_pos, exp_neg, V: torch.Tensor, stream=None,
+            planes=None) -> None:
+    """planes=(x_begin, x_end): only those x planes of V (wfs_add_cap_planes)."""
     check_potential(V, n)
-    check(lib().wfs_add_cap(i3(n), d3(dx), d3(bottom), d3(top), float(offset), d3(pot_pos), d3(pot_neg),
-                            d3(exp_pos), d3(exp_neg), ptr(V), stream_handle(stream)), "wfs_add_cap")
+    if planes is None:
+        check(lib().wfs_add_cap(i3(n), d3(dx), d3(bottom), d3(top), float(offset), d3(pot_pos), d3(pot_neg),
+                                d3(exp_pos), d3(exp_neg), ptr(V), stream_handle(stream)), "wfs_add_cap")
+    else:
+        check(lib().wfs_add_cap_planes(i3(n), d3(dx), d3(bottom), d3(top), float(offset), d3(pot_pos), d3(pot_neg),
+                                       d3(exp_pos), d3(exp_neg), int(planes[0]), int(planes[1]), ptr(V),
+                                       stream_handle(stream)), "wfs_add_cap_planes")
 
 
 def add_wall(n, dx, potential, center, normal, plateau, slope, slope_exponent, slit_count, slit_spacing,
-             slit_width, slit_rotation, V: torch.Tensor, stream=None) -> None:
+             slit_width, slit_rotation, V: torch.Tensor, stream=None, planes=None) -> None:
+    """planes=(x_begin, x_end): only those x planes of V (wfs_add_wall_planes)."""
     check_potential(V, n)
-    check(lib().wfs_add_wall(i3(n), d3(dx), float(potential), d3(center), d3(normal), float(plateau),
-                             float(slope), float(slope_exponent), int(slit_count), float(slit_spacing),
-                             float(slit_width), float(slit_rotation), ptr(V), stream_handle(stream)),
-          "wfs_add_wall")
+    if planes is None:
+        check(lib().wfs_add_wall(i3(n), d3(dx), float(potential), d3(center), d3(normal), float(plateau),
+                                 float(slope), float(slope_exponent), int(slit_count), float(slit_spacing),
+                                 float(slit_width), float(slit_rotation), ptr(V), stream_handle(stream)),
+              "wfs_add_wall")
+    else:
+        check(lib().wfs_add_wall_planes(i3(n), d3(dx), float(potential), d3(center), d3(normal), float(plateau),
+                                        float(slope), float(slope_exponent), int(slit_count), float(slit_spacing),
+                                        float(slit_width), float(slit_rotation), int(planes[0]), int(planes[1]),
+                                        ptr(V), stream_handle(stream)), "wfs_add_wall_planes")
 
 
 def add_harmonic(n, dx, omega, mass, center, V: torch.Tensor, stream=None) -> None:

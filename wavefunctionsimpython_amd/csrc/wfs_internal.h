@@ -141,7 +141,10 @@ bool slab_supported(int precision, const int n[3]);
 // Per x plane (packets * nx planes of plane_elems complex64 factors, contiguous per plane): src[p]
 // = the first plane of the run of bitwise-identical consecutive planes holding p, within its
 // packet.  flags: workspace of packets * nx ints.
-int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets, int* flags, int* src, hipStream_t s);
+// flags[p] = plane p of the table equals plane p - 1 of its packet, for planes first .. first +
+// count - 1 (the others keep theirs); then src[p] = the first plane of p's run, for every plane
+int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets, long first, long count, int* flags,
+                       int* src, hipStream_t s);
 // thread geometry of the slab scheme's y-z launch: y lines split over uy threads, z lines
 // over uz threads, n1 points per thread; y-line twiddle stride b in the ny table
 struct SlabRows {

@@ -340,8 +340,8 @@ int share_planes(wfs_plan* p, hipStream_t s) {
       return WFS_ERR_ALLOC;
     }
   }
-  WFS_TRY(launch_plane_dedup(p->pv_lines, long(p->n[1]) * p->n[2], p->n[0], p->batch, p->pv_planes + planes,
-                             p->pv_planes, s));
+  WFS_TRY(launch_plane_dedup(p->pv_lines, long(p->n[1]) * p->n[2], p->n[0], p->batch, 0, planes,
+                             p->pv_planes + planes, p->pv_planes, s));
   p->pv_planes_ok = true;
   return 0;
 }
@@ -366,6 +366,38 @@ int set_potential(wfs_plan* p, const void* V, hipStream_t s) {
   }
   p->pv_set = true;
   p->sep_set = false;
+  return 0;
+}
+
+// x planes [x_begin, x_end) of V changed since the current table was built from V (one packet):
+// only those planes of the table are rebuilt, in the order the steps read, and only the map
+// flags of planes x_begin .. x_end (plane x_end is compared with its changed predecessor).
+// The table's values are those set_potential writes: every element depends on its own voxel
+// of V alone.  Without a current table of V the whole table is built.
+int set_potential_planes(wfs_plan* p, const void* V, int x_begin, int x_end, hipStream_t s) {
+  if (x_begin == x_end) return 0;
+  const bool current = p->pv_set && !p->sep_set && (slab_on(p) ? p->pv_lines_ok : p->pv_nat_ok);
+  if (!current) return set_potential(p, V, s);
+  const long plane = long(p->n[1]) * p->n[2];
+  const long off = long(x_begin) * plane, count = long(x_end - x_begin) * plane;
+  const float2* v = static_cast<const float2*>(V) + off;
+  if (slab_on(p)) {
+    WFS_TRY(launch_potential_phase_lines(p->precision, count, v, p->dt, static_cast<char*>(p->pv_lines) + off * p->elem,
+                                         slab_lines_order(p->n[1]), s));
+    p->pv_nat_ok = false;
+    if (capturing(s)) p->pv_in_graph = true;
+    if (p->pv_planes_ok && !capturing(s)) {
+      const long planes = p->n[0];
+      const long last = x_end < p->n[0] ? x_end : p->n[0] - 1;
+      WFS_TRY(launch_plane_dedup(p->pv_lines, plane, p->n[0], 1, x_begin, last - x_begin + 1, p->pv_planes + planes,
+                                 p->pv_planes, s));
+    } else {
+      WFS_TRY(share_planes(p, s));
+    }
+  } else {
+    WFS_TRY(launch_potential_phase(p->precision, count, v, p->dt, static_cast<char*>(p->pv) + off * p->elem, s));
+    p->pv_lines_ok = false;
+  }
   return 0;
 }
 
@@ -757,6 +789,17 @@ extern "C" int wfs_set_potential(wfs_plan* p, const void* V, void* stream) {
   if (!p || !V) { set_last_error("wfs_set_potential: null argument"); return WFS_ERR_INVALID; }
   if (p->method != WFS_METHOD_FOURIER) return 0;   // the power series reads V directly
   return set_potential(p, V, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int wfs_set_potential_planes(wfs_plan* p, const void* V, int x_begin, int x_end, void* stream) {
+  if (!p || !V) { set_last_error("wfs_set_potential_planes: null argument"); return WFS_ERR_INVALID; }
+  if (x_begin < 0 || x_begin > x_end || x_end > p->n[0]) {
+    set_last_error("wfs_set_potential_planes: planes [%d, %d) outside 0 .. %d", x_begin, x_end, p->n[0]);
+    return WFS_ERR_INVALID;
+  }
+  if (p->batch != 1) { set_last_error("wfs_set_potential_planes: batched plans are not supported"); return WFS_ERR_UNSUPPORTED; }
+  if (p->method != WFS_METHOD_FOURIER) return 0;   // the power series reads V directly
+  return set_potential_planes(p, V, x_begin, x_end, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int wfs_set_potential_axes(wfs_plan* p, const double* fx, const double* fy, const double* fz) {

@@ -10,16 +10,23 @@ namespace {
 
 constexpr int kGenThreads = 256;
 
+// grid size and the window of x planes one launch covers: element i of the launch's
+// (offset) array is voxel (x0 + i / (ny nz), (i / nz) mod ny, i mod nz)
 struct Dims {
   int nx, ny, nz;
+  int x0, planes;
 };
+
+Dims full_dims(const int n[3]) { return Dims{n[0], n[1], n[2], 0, n[0]}; }
 
 __device__ __forceinline__ void voxel_of(long i, const Dims& d, int& x, int& y, int& z) {
   z = int(i % d.nz);
   const long r = i / d.nz;
   y = int(r % d.ny);
-  x = int(r / d.ny);
+  x = d.x0 + int(r / d.ny);
 }
+
+__device__ __forceinline__ long window_points(const Dims& d) { return long(d.planes) * d.ny * d.nz; }
 
 unsigned blocks_for(long n) { return unsigned((n + kGenThreads - 1) / kGenThreads); }
 
@@ -35,7 +42,7 @@ struct GaussParams {
 
 template <typename R>
 __global__ __launch_bounds__(kGenThreads) void gaussian_kernel(cx<R>* __restrict__ psi, Dims d, GaussParams p) {
-  const long n = long(d.nx) * d.ny * d.nz;
+  const long n = window_points(d);
   const long i = long(blockIdx.x) * kGenThreads + threadIdx.x;
   if (i >= n) return;
   int v[3];
@@ -78,7 +85,7 @@ __device__ __forceinline__ float len3f(float a, float b, float c) { return sqrtf
 
 __global__ __launch_bounds__(kGenThreads) void cap_kernel(float2* __restrict__ V, Dims dm, CapParams p) {
 #pragma clang fp contract(off)
-  const long n = long(dm.nx) * dm.ny * dm.nz;
+  const long n = window_points(dm);
   const long i = long(blockIdx.x) * kGenThreads + threadIdx.x;
   if (i >= n) return;
   int vx[3];
@@ -171,7 +178,7 @@ struct WallParams {
 
 __global__ __launch_bounds__(kGenThreads) void wall_kernel(float2* __restrict__ V, Dims dm, WallParams p) {
 #pragma clang fp contract(off)
-  const long n = long(dm.nx) * dm.ny * dm.nz;
+  const long n = window_points(dm);
   const long i = long(blockIdx.x) * kGenThreads + threadIdx.x;
   if (i >= n) return;
   int vx[3];
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(kGenThreads) void wall_kernel(float2* __restrict__ 
 __global__ __launch_bounds__(kGenThreads) void harmonic_kernel(float2* __restrict__ V, Dims dm, double dx, double dy,
                                                                double dz, double k, double cx_, double cy_,
                                                                double cz_) {
-  const long n = long(dm.nx) * dm.ny * dm.nz;
+  const long n = window_points(dm);
   const long i = long(blockIdx.x) * kGenThreads + threadIdx.x;
   if (i >= n) return;
   int x, y, z;
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(kGenThreads) void harmonic_kernel(float2* __restric
 // cos^2 lattice: V += depth * sum_d cos^2(pi r_d / period) (FP64, rounded)
 __global__ __launch_bounds__(kGenThreads) void lattice_kernel(float2* __restrict__ V, Dims dm, double dx, double dy,
                                                               double dz, double depth, double period) {
-  const long n = long(dm.nx) * dm.ny * dm.nz;
+  const long n = window_points(dm);
   const long i = long(blockIdx.x) * kGenThreads + threadIdx.x;
   if (i >= n) return;
   int x, y, z;
@@ -243,7 +250,7 @@ __global__ __launch_bounds__(kGenThreads) void lattice_kernel(float2* __restrict
 template <typename R>
 __global__ __launch_bounds__(kGenThreads) void kinetic_table_kernel(cx<R>* __restrict__ out, Dims dm, double dx,
                                                                     double dy, double dz, double c) {
-  const long n = long(dm.nx) * dm.ny * dm.nz;
+  const long n = window_points(dm);
   const long i = long(blockIdx.x) * kGenThreads + threadIdx.x;
   if (i >= n) return;
   int v[3];
@@ -313,7 +320,7 @@ using namespace wfs;
 extern "C" int wfs_init_gaussian(const int n[3], const double dx[3], int precision, const double sigma[3],
                                  const double r0[3], const double k0[3], void* psi, void* stream) {
   if (!dims_ok(n) || !dx || !sigma || !r0 || !k0 || !psi) { set_last_error("wfs_init_gaussian: bad argument"); return WFS_ERR_INVALID; }
-  Dims d{n[0], n[1], n[2]};
+  const Dims d = full_dims(n);
   GaussParams p;
   for (int k = 0; k < 3; ++k) { p.dx[k] = dx[k]; p.sigma[k] = sigma[k]; p.r0[k] = r0[k]; p.k0[k] = k0[k]; }
   const long total = long(n[0]) * n[1] * n[2];
@@ -328,14 +335,22 @@ extern "C" int wfs_init_gaussian(const int n[3], const double dx[3], int precisi
   return 0;
 }
 
-extern "C" int wfs_add_cap(const int n[3], const double dx[3], const double bottom[3], const double top[3],
-                           double offset, const double pot_pos[3], const double pot_neg[3], const double exp_pos[3],
-                           const double exp_neg[3], void* V, void* stream) {
-  if (!dims_ok(n) || !dx || !bottom || !top || !pot_pos || !pot_neg || !exp_pos || !exp_neg || !V) {
-    set_last_error("wfs_add_cap: bad argument");
+// The x planes [x_begin, x_end) of an n[0] x n[1] x n[2] grid: the generators below write only
+// those planes' elements of V and compute every voxel there exactly as a whole-grid launch does.
+static bool window_ok(const int n[3], int x_begin, int x_end) {
+  return x_begin >= 0 && x_begin <= x_end && x_end <= n[0];
+}
+
+static int add_cap_window(const int n[3], const double dx[3], const double bottom[3], const double top[3],
+                          double offset, const double pot_pos[3], const double pot_neg[3], const double exp_pos[3],
+                          const double exp_neg[3], int x_begin, int x_end, void* V, void* stream, const char* who) {
+  if (!dims_ok(n) || !dx || !bottom || !top || !pot_pos || !pot_neg || !exp_pos || !exp_neg || !V ||
+      !window_ok(n, x_begin, x_end)) {
+    set_last_error("%s: bad argument", who);
     return WFS_ERR_INVALID;
   }
-  Dims d{n[0], n[1], n[2]};
+  if (x_begin == x_end) return 0;
+  const Dims d{n[0], n[1], n[2], x_begin, x_end - x_begin};
   CapParams p;
   for (int k = 0; k < 3; ++k) {
     p.dr[k] = float(dx[k]); p.bottom[k] = float(bottom[k]); p.top[k] = float(top[k]);
@@ -343,23 +358,40 @@ extern "C" int wfs_add_cap(const int n[3], const double dx[3], const double bott
     p.epos[k] = float(exp_pos[k]); p.eneg[k] = float(exp_neg[k]);
   }
   p.offset = float(offset);
-  const long total = long(n[0]) * n[1] * n[2];
+  const long total = long(d.planes) * n[1] * n[2];
   hipLaunchKernelGGL(cap_kernel, dim3(blocks_for(total)), dim3(kGenThreads), 0, static_cast<hipStream_t>(stream),
-                     static_cast<float2*>(V), d, p);
+                     static_cast<float2*>(V) + long(x_begin) * n[1] * n[2], d, p);
   WFS_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-extern "C" int wfs_add_wall(const int n[3], const double dx[3], double potential, const double center[3],
-                            const double normal[3], double plateau, double slope, double slope_exponent,
-                            int slit_count, double slit_spacing, double slit_width, double slit_rotation, void* V,
-                            void* stream) {
+extern "C" int wfs_add_cap(const int n[3], const double dx[3], const double bottom[3], const double top[3],
+                           double offset, const double pot_pos[3], const double pot_neg[3], const double exp_pos[3],
+                           const double exp_neg[3], void* V, void* stream) {
+  return add_cap_window(n, dx, bottom, top, offset, pot_pos, pot_neg, exp_pos, exp_neg, 0, dims_ok(n) ? n[0] : 0, V,
+                        stream, "wfs_add_cap");
+}
+
+extern "C" int wfs_add_cap_planes(const int n[3], const double dx[3], const double bottom[3], const double top[3],
+                                  double offset, const double pot_pos[3], const double pot_neg[3],
+                                  const double exp_pos[3], const double exp_neg[3], int x_begin, int x_end, void* V,
+                                  void* stream) {
+  return add_cap_window(n, dx, bottom, top, offset, pot_pos, pot_neg, exp_pos, exp_neg, x_begin, x_end, V, stream,
+                        "wfs_add_cap_planes");
+}
+
+static int add_wall_window(const int n[3], const double dx[3], double potential, const double center[3],
+                           const double normal[3], double plateau, double slope, double slope_exponent,
+                           int slit_count, double slit_spacing, double slit_width, double slit_rotation, int x_begin,
+                           int x_end, void* V, void* stream, const char* who) {
 #pragma clang fp contract(off)
-  if (!dims_ok(n) || !dx || !center || !normal || !V || slit_count < 0 || slit_count > 4) {
-    set_last_error("wfs_add_wall: bad argument");
+  if (!dims_ok(n) || !dx || !center || !normal || !V || slit_count < 0 || slit_count > 4 ||
+      !window_ok(n, x_begin, x_end)) {
+    set_last_error("%s: bad argument", who);
     return WFS_ERR_INVALID;
   }
-  Dims d{n[0], n[1], n[2]};
+  if (x_begin == x_end) return 0;
+  const Dims d{n[0], n[1], n[2], x_begin, x_end - x_begin};
   WallParams p;
   for (int k = 0; k < 3; ++k) { p.dr[k] = float(dx[k]); p.center[k] = float(center[k]); p.normal[k] = float(normal[k]); }
   // right = rotate(normalize(normal x up), normal, rotation) in float32 (potential_wall.cu:39-45,
@@ -385,17 +417,33 @@ extern "C" int wfs_add_wall(const int n[3], const double dx[3], double potential
   p.slit_spacing = float(slit_spacing);
   p.slit_width = float(slit_width);
   p.slit_count = slit_count;
-  const long total = long(n[0]) * n[1] * n[2];
+  const long total = long(d.planes) * n[1] * n[2];
   hipLaunchKernelGGL(wall_kernel, dim3(blocks_for(total)), dim3(kGenThreads), 0, static_cast<hipStream_t>(stream),
-                     static_cast<float2*>(V), d, p);
+                     static_cast<float2*>(V) + long(x_begin) * n[1] * n[2], d, p);
   WFS_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+extern "C" int wfs_add_wall(const int n[3], const double dx[3], double potential, const double center[3],
+                            const double normal[3], double plateau, double slope, double slope_exponent,
+                            int slit_count, double slit_spacing, double slit_width, double slit_rotation, void* V,
+                            void* stream) {
+  return add_wall_window(n, dx, potential, center, normal, plateau, slope, slope_exponent, slit_count, slit_spacing,
+                         slit_width, slit_rotation, 0, dims_ok(n) ? n[0] : 0, V, stream, "wfs_add_wall");
+}
+
+extern "C" int wfs_add_wall_planes(const int n[3], const double dx[3], double potential, const double center[3],
+                                   const double normal[3], double plateau, double slope, double slope_exponent,
+                                   int slit_count, double slit_spacing, double slit_width, double slit_rotation,
+                                   int x_begin, int x_end, void* V, void* stream) {
+  return add_wall_window(n, dx, potential, center, normal, plateau, slope, slope_exponent, slit_count, slit_spacing,
+                         slit_width, slit_rotation, x_begin, x_end, V, stream, "wfs_add_wall_planes");
 }
 
 extern "C" int wfs_add_harmonic(const int n[3], const double dx[3], double omega, double mass,
                                 const double center[3], void* V, void* stream) {
   if (!dims_ok(n) || !dx || !center || !V) { set_last_error("wfs_add_harmonic: bad argument"); return WFS_ERR_INVALID; }
-  Dims d{n[0], n[1], n[2]};
+  const Dims d = full_dims(n);
   const long total = long(n[0]) * n[1] * n[2];
   hipLaunchKernelGGL(harmonic_kernel, dim3(blocks_for(total)), dim3(kGenThreads), 0, static_cast<hipStream_t>(stream),
                      static_cast<float2*>(V), d, dx[0], dx[1], dx[2], 0.5 * mass * omega * omega, center[0],
@@ -407,7 +455,7 @@ extern "C" int wfs_add_harmonic(const int n[3], const double dx[3], double omega
 extern "C" int wfs_add_lattice(const int n[3], const double dx[3], double depth, double period, void* V,
                                void* stream) {
   if (!dims_ok(n) || !dx || !V || period == 0.0) { set_last_error("wfs_add_lattice: bad argument"); return WFS_ERR_INVALID; }
-  Dims d{n[0], n[1], n[2]};
+  const Dims d = full_dims(n);
   const long total = long(n[0]) * n[1] * n[2];
   hipLaunchKernelGGL(lattice_kernel, dim3(blocks_for(total)), dim3(kGenThreads), 0, static_cast<hipStream_t>(stream),
                      static_cast<float2*>(V), d, dx[0], dx[1], dx[2], depth, period);
@@ -418,7 +466,7 @@ extern "C" int wfs_add_lattice(const int n[3], const double dx[3], double depth,
 extern "C" int wfs_kinetic_operator(const int n[3], const double dx[3], double dt, double mass, int precision,
                                     void* out, void* stream) {
   if (!dims_ok(n) || !dx || !out || !(mass > 0.0)) { set_last_error("wfs_kinetic_operator: bad argument"); return WFS_ERR_INVALID; }
-  Dims d{n[0], n[1], n[2]};
+  const Dims d = full_dims(n);
   const long total = long(n[0]) * n[1] * n[2];
   const double c = dt / (4.0 * mass);
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -245,8 +245,9 @@ __attribute__((amdgpu_waves_per_eu(ZB == 8 && NX / split_of<NX>::n2 <= 16 ? 4 : 
 
 // flags[p] = 1 when x plane p equals plane p - 1 of the same packet bit for bit, else 0
 __global__ __launch_bounds__(256) void plane_equal_kernel(const unsigned long long* __restrict__ table,
-                                                          long plane_elems, int nx, int* __restrict__ flags) {
-  const long p = blockIdx.x;
+                                                          long plane_elems, int nx, long first,
+                                                          int* __restrict__ flags) {
+  const long p = first + blockIdx.x;
   int differ = 0;
   if (p % nx == 0) {
     differ = 1;
@@ -344,14 +345,15 @@ int launch_x_slab_ny(int nx, int variant, void* data, const XSlabParams& p, hipS
 
 }  // namespace
 
-int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets, int* flags, int* src, hipStream_t s) {
+int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets, long first, long count, int* flags,
+                       int* src, hipStream_t s) {
   const long planes = long(nx) * packets;
-  if (planes < 1 || planes > 0x7fffffffL || plane_elems < 1) {
+  if (planes < 1 || planes > 0x7fffffffL || plane_elems < 1 || first < 0 || count < 1 || first + count > planes) {
     set_last_error("potential plane sharing: bad sizes");
     return WFS_ERR_INVALID;
   }
-  WFS_LAUNCH(plane_equal_kernel, dim3(unsigned(planes)), dim3(256), 0, s,
-                     static_cast<const unsigned long long*>(table), plane_elems, nx, flags);
+  WFS_LAUNCH(plane_equal_kernel, dim3(unsigned(count)), dim3(256), 0, s,
+                     static_cast<const unsigned long long*>(table), plane_elems, nx, first, flags);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
   WFS_LAUNCH(plane_runs_kernel, dim3(unsigned((packets + 63) / 64)), dim3(64), 0, s, flags, nx, packets, src);

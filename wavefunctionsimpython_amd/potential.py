@@ -71,7 +71,8 @@ class AbsorbingBoundaryCondition:
             dtype=np.float64)
         self.enable = bool(try_read_param(config, "absorbing_boundary_condition.enable", True))
 
-    def add_potential(self, V: torch.Tensor, delta_x_3) -> torch.Tensor:
+    def add_potential(self, V: torch.Tensor, delta_x_3, planes=None) -> torch.Tensor:
+        """planes=(x_begin, x_end): only those x planes of V get the absorber."""
         if not self.enable:
             return V
         _abi.add_cap(tuple(V.shape), delta_x_3, self.boundary_bottom_corner_bohr_radii_3,
@@ -79,7 +80,7 @@ class AbsorbingBoundaryCondition:
                      self.outer_potential_in_positive_xyz_direction_hartree_3,
                      self.outer_potential_in_negative_xyz_direction_hartree_3,
                      self.slope_exponent_in_positive_xyz_direction_3,
-                     self.slope_exponent_in_negative_xyz_direction_3, V)
+                     self.slope_exponent_in_negative_xyz_direction_3, V, planes=planes)
         return V
 
 
@@ -116,14 +117,36 @@ class PotentialWall:
         self.visible = try_read_param(wall_config, "visible", True)
         self.enable = bool(try_read_param(wall_config, "enable", True))
 
-    def add_potential(self, V: torch.Tensor, delta_x_3) -> torch.Tensor:
+    def add_potential(self, V: torch.Tensor, delta_x_3, planes=None) -> torch.Tensor:
+        """planes=(x_begin, x_end): only those x planes of V get the wall."""
         if not self.enable:
             return V
         _abi.add_wall(tuple(V.shape), delta_x_3, self.potential_hartree, self.center_bohr_radii_3,
                       self.normal_vector_3, self.plateau_thickness_bohr_radii, self.slope_thickness_bohr_radii,
                       self.slope_exponent, self.slit_count, self.slit_spacing_bohr_radii,
-                      self.slit_width_bohr_radii, self.slit_rotation_radian, V)
+                      self.slit_width_bohr_radii, self.slit_rotation_radian, V, planes=planes)
         return V
+
+    def x_planes(self, shape, delta_x_3, center=None):
+        """The x planes [lo, hi) outside which the wall (at ``center``, default its own) adds
+        nothing, or None when its normal is not along x (its slab then crosses every x plane).
+
+        A voxel gets the wall where |n . (r - c)| <= plateau/2 + slope
+        (sources/cuda_kernels/potential_wall.cu:37-38, :76-82); with n = (+-1, 0, 0) that is
+        |r_x - c_x| <= plateau/2 + slope, r_x = dx (i - nx/2).  Two planes of margin on each side
+        cover the kernel's float32 rounding of r and d."""
+        nx = int(shape[0])
+        if not self.enable:
+            return (0, 0)
+        n = self.normal_vector_3
+        if n[1] != 0.0 or n[2] != 0.0 or n[0] == 0.0:
+            return None
+        c = float((self.center_bohr_radii_3 if center is None else center)[0])
+        reach = (0.5 * self.plateau_thickness_bohr_radii + max(self.slope_thickness_bohr_radii, 0.0)) / abs(n[0])
+        d = float(delta_x_3[0])
+        lo = int(np.floor(nx / 2 + (c - reach) / d)) - 2
+        hi = int(np.ceil(nx / 2 + (c + reach) / d)) + 3
+        return (min(max(lo, 0), nx), min(max(hi, 0), nx))
 
 
 class HarmonicPotential:
@@ -162,13 +185,18 @@ class OpticalLattice:
 
 def generate_potential_from_walls_and_drain(V: torch.Tensor, V_vis: torch.Tensor, delta_x_3,
                                             absorbing_pontetial: AbsorbingBoundaryCondition,
-                                            walls: List[PotentialWall]):
+                                            walls: List[PotentialWall], planes=None):
     """Dynamic-mode regeneration: V = 0, + walls, + absorbing boundary.
 
-    sources/potential.py:508-522 (V_vis is returned unchanged there too).
+    sources/potential.py:508-522 (V_vis is returned unchanged there too).  planes=(x_begin,
+    x_end) regenerates those x planes only, with the values the whole-grid regeneration gives
+    them; the other planes keep theirs.
     """
-    V.zero_()
+    if planes is None:
+        V.zero_()
+    else:
+        V[planes[0]:planes[1]].zero_()
     for w in walls:
-        w.add_potential(V, delta_x_3)
-    absorbing_pontetial.add_potential(V, delta_x_3)
+        w.add_potential(V, delta_x_3, planes=planes)
+    absorbing_pontetial.add_potential(V, delta_x_3, planes=planes)
     return V, V_vis

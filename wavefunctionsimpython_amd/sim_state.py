@@ -120,6 +120,7 @@ class SimState:
         self.__localised_potential_to_visualize_hartree: torch.Tensor | None = None
         self.__plan: _abi.Plan | None = None
         self.__potential_dirty = True   # the plan's phase table must be refreshed from V
+        self.__dirty_planes = None      # else: only these x planes of it, (lo, hi)
 
     # ------------------------------------------------------------------ set-up
     def _shape(self):
@@ -323,6 +324,7 @@ class SimState:
         if not self.__is_dynamic_potential_mode:
             return
         changed = False
+        moved = []      # (wall, centre before the move)
         for w in self.__potential_walls:
             old = w.center_bohr_radii_3.copy()
             w.center_bohr_radii_3 = w.center_bohr_radii_3 + (w.velocity_bohr_radius_hartree_per_h_bar_3
@@ -331,17 +333,53 @@ class SimState:
                 w.center_bohr_radii_3[0] = 0.0
             if not np.array_equal(old, w.center_bohr_radii_3):
                 changed = True
+                moved.append((w, old))
         if not changed and self.__localised_potential_hartree is not None and getattr(self, "_regenerated", False):
             return
+        planes = self._swept_planes(moved)
         self.__localised_potential_hartree, self.__localised_potential_to_visualize_hartree = \
             potential.generate_potential_from_walls_and_drain(
                 V=self.__localised_potential_hartree, V_vis=self.__localised_potential_to_visualize_hartree,
                 delta_x_3=self.__delta_x_bohr_radii_3, absorbing_pontetial=self.__absorbing_boundary_condition,
-                walls=self.__potential_walls)
-        self.__harmonic.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
-        self.__lattice.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
+                walls=self.__potential_walls, planes=planes)
+        if planes is None:
+            self.__harmonic.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
+            self.__lattice.add_potential(self.__localised_potential_hartree, self.__delta_x_bohr_radii_3)
+            self.__potential_dirty = True
+        elif not self.__potential_dirty and planes[0] < planes[1]:
+            lo, hi = planes
+            if self.__dirty_planes is not None:
+                lo, hi = min(lo, self.__dirty_planes[0]), max(hi, self.__dirty_planes[1])
+            self.__dirty_planes = (lo, hi)
         self._regenerated = True
-        self.__potential_dirty = True
+
+    # Plane-wise regeneration (no reference counterpart: the reference rebuilds V whole): when
+    # every wall that moved has its normal along x, only the x planes its slab covered before or
+    # covers after the move change, and only those are rebuilt -- V and the plan's table come out
+    # bit for bit as from the whole rebuild.  Used after a first whole rebuild, without the
+    # harmonic well or the lattice (whole-grid terms), and while the planes are at most half of x.
+    partial_regeneration = True
+
+    def _swept_planes(self, moved):
+        if not (self.partial_regeneration and getattr(self, "_regenerated", False)
+                and self.__localised_potential_hartree is not None and moved):
+            return None
+        if self.__harmonic.enable or self.__lattice.enable:
+            return None
+        shape = tuple(self.__localised_potential_hartree.shape)
+        lo, hi = shape[0], 0
+        for w, old in moved:
+            for c in (old, w.center_bohr_radii_3):
+                span = w.x_planes(shape, self.__delta_x_bohr_radii_3, center=c)
+                if span is None:
+                    return None
+                if span[0] < span[1]:
+                    lo, hi = min(lo, span[0]), max(hi, span[1])
+        if hi <= lo:
+            return (0, 0)
+        if 2 * (hi - lo) > shape[0]:
+            return None
+        return (lo, hi)
 
     def potential_is_static(self) -> bool:
         """True when update_potential() can no longer change V."""
@@ -360,6 +398,11 @@ class SimState:
         if self.__potential_dirty:
             self.__plan.set_potential(self.__localised_potential_hartree)
             self.__potential_dirty = False
+            self.__dirty_planes = None
+        elif self.__dirty_planes is not None:
+            lo, hi = self.__dirty_planes
+            self.__plan.set_potential_planes(self.__localised_potential_hartree, lo, hi)
+            self.__dirty_planes = None
 
     def evolve_state(self, steps: int = 1):
         """Advance psi by ``steps`` time steps in place (sources/sim_state.py:754-762)."""
