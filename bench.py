@@ -770,6 +770,7 @@ def dropin_bench(steps, warmup):
     import contextlib
     import shutil
     import tempfile
+    import numpy as np
     import torch
     from wavefunctionsimpython_amd import _abi, core_sim, initialisation
     tmp = tempfile.mkdtemp(prefix="wfs_dropin_")
@@ -811,15 +812,49 @@ def dropin_bench(steps, warmup):
                 it.i = i
                 core_sim.time_step(sim, tools, it)
             torch.cuda.synchronize()
-            t0 = time.perf_counter()
+            # each iteration ends in a device-to-host copy of its observables (a synchronisation), so
+            # the iterations are timed one by one: the median is the loop's cost, the mean adds the
+            # host's occasional pauses (allocator, garbage collector) that a short window may catch
+            per = []
             for i in range(warmup, warmup + loop_steps):
                 it.i = i
+                t0 = time.perf_counter()
                 core_sim.time_step(sim, tools, it)
-            torch.cuda.synchronize()
-            el = time.perf_counter() - t0
-            res["reference_loop_time_step"] = {"value": npts * loop_steps / el, "ms_per_step": el / loop_steps * 1e3,
-                                               "steps": loop_steps}
+                torch.cuda.synchronize()
+                per.append(time.perf_counter() - t0)
+            med = float(np.median(per))
+            res["reference_loop_time_step"] = {"value": npts / med, "ms_per_step": med * 1e3,
+                                               "mean_ms_per_step": float(np.mean(per)) * 1e3, "steps": loop_steps}
             del sim, tools, it
+            # the wall moving along its x normal (velocity 0.5 Bohr per atomic time unit), sampled every
+            # step: V rebuilt every step, plane-wise (only the planes the wall sweeps) or whole
+            moving = {}
+            states = []
+            for plane_wise in (False, True):
+                cfg = dropin_config(tmp, 1)
+                cfg["potential"]["walls"][0]["velocity_bohr_radius_hartree_per_h_bar_3"] = [0.5, 0.0, 0.0]
+                sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
+                sim.partial_regeneration = plane_wise
+                it.total_iteration_count = warmup
+                core_sim.run_iteration(sim, tools, it, observe_every=1, exit_on_quit=False)
+                torch.cuda.synchronize()
+                # run_iteration's loop with a moving wall is time_step's: timed per iteration as above
+                per = []
+                for i in range(warmup, warmup + loop_steps):
+                    it.i = i
+                    t0 = time.perf_counter()
+                    core_sim.time_step(sim, tools, it)
+                    torch.cuda.synchronize()
+                    per.append(time.perf_counter() - t0)
+                med = float(np.median(per))
+                moving["plane_wise" if plane_wise else "whole"] = {"ms_per_step": med * 1e3, "value": npts / med,
+                                                                   "mean_ms_per_step": float(np.mean(per)) * 1e3}
+                states.append(sim.get_wave_function().clone())
+                del sim, tools, it
+            moving["steps"] = loop_steps
+            moving["bit_identical"] = bool(torch.equal(states[0], states[1]))
+            res["moving_wall_observe_every_1"] = moving
+            del states
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return {
