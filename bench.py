@@ -768,6 +768,7 @@ def dropin_bench(steps, warmup):
     slit, sampling every step and every 10 steps, against the reference's own loop shape
     (core_sim.time_step per iteration: measure, evolve one step, update the potential)."""
     import contextlib
+    import gc
     import shutil
     import tempfile
     import numpy as np
@@ -784,6 +785,7 @@ def dropin_bench(steps, warmup):
                 psi0 = sim.get_wave_function().clone()
                 it.total_iteration_count = warmup
                 core_sim.run_iteration(sim, tools, it, observe_every=every, exit_on_quit=False)
+                gc.collect()
                 torch.cuda.synchronize()
                 it.total_iteration_count = warmup + steps
                 t0 = time.perf_counter()
@@ -811,10 +813,12 @@ def dropin_bench(steps, warmup):
             for i in range(warmup):
                 it.i = i
                 core_sim.time_step(sim, tools, it)
+            # the earlier runs' objects (plans, states) are freed now, not inside the timed window
+            gc.collect()
             torch.cuda.synchronize()
             # each iteration ends in a device-to-host copy of its observables (a synchronisation), so
             # the iterations are timed one by one: the median is the loop's cost, the mean adds the
-            # host's occasional pauses (allocator, garbage collector) that a short window may catch
+            # host's occasional pauses that a short window may catch
             per = []
             for i in range(warmup, warmup + loop_steps):
                 it.i = i
@@ -837,6 +841,7 @@ def dropin_bench(steps, warmup):
                 sim.partial_regeneration = plane_wise
                 it.total_iteration_count = warmup
                 core_sim.run_iteration(sim, tools, it, observe_every=1, exit_on_quit=False)
+                gc.collect()
                 torch.cuda.synchronize()
                 # run_iteration's loop with a moving wall is time_step's: timed per iteration as above
                 per = []
