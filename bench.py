@@ -856,8 +856,25 @@ def dropin_bench(steps, warmup):
                                                                    "mean_ms_per_step": float(np.mean(per)) * 1e3}
                 states.append(sim.get_wave_function().clone())
                 del sim, tools, it
+            # run_iteration: stretches of steps with the plane-wise update after every step inside
+            # the call (wfs_step_observe_between), sampled every step
+            cfg = dropin_config(tmp, 1)
+            cfg["potential"]["walls"][0]["velocity_bohr_radius_hartree_per_h_bar_3"] = [0.5, 0.0, 0.0]
+            sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
+            it.total_iteration_count = warmup
+            core_sim.run_iteration(sim, tools, it, observe_every=1, exit_on_quit=False)
+            gc.collect()
+            torch.cuda.synchronize()
+            it.total_iteration_count = warmup + loop_steps
+            t0 = time.perf_counter()
+            core_sim.run_iteration(sim, tools, it, observe_every=1, exit_on_quit=False)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            moving["run_iteration_stretches"] = {"ms_per_step": el / loop_steps * 1e3, "value": npts * loop_steps / el}
+            states.append(sim.get_wave_function().clone())
+            del sim, tools, it
             moving["steps"] = loop_steps
-            moving["bit_identical"] = bool(torch.equal(states[0], states[1]))
+            moving["bit_identical"] = bool(torch.equal(states[0], states[1]) and torch.equal(states[1], states[2]))
             res["moving_wall_observe_every_1"] = moving
             del states
     finally:

@@ -190,6 +190,31 @@ int wfs_step_observe(wfs_plan* plan, void* psi, const void* V, int nsteps, int o
                      const wfs_box* boxes, int nboxes, double* out, void* stream);
 
 /*
+ * Called by wfs_step_observe_between after each step (0-based index `step`), once that step's
+ * kernels are queued on the call's stream and before the next step's are; a nonzero return
+ * stops the call with WFS_ERR_INVALID.
+ */
+typedef int (*wfs_between_steps)(void* ctx, int step);
+
+/*
+ * wfs_step_observe with a callback between steps (no reference counterpart: the reference's
+ * dynamic mode updates the potential after every step, SimState.update_potential between
+ * evolve_state calls, sources/core_sim.py:19-24, sources/sim_state.py:466-509).  The callback
+ * may rewrite V and the plan's table (wfs_set_potential, wfs_set_potential_planes, the
+ * generators) on the same stream; the next step reads what it queued.  It must not touch psi,
+ * which holds an intermediate layout until the call returns, nor wait on the stream.  With the
+ * slab scheme the steps run at the fused rate (no entry and exit launches per step).  With
+ * observe_every = 1 the state and observables are those of one wfs_step_observe call per step
+ * with the same potentials (bit for bit, as a stretch equals its alternating calls); with
+ * larger intervals the kinetic half steps between observations merge as in any stretch (float
+ * rounding).  Without the slab scheme each step is one call, bit for bit those calls.  between
+ * may be NULL (then as wfs_step_observe).
+ */
+int wfs_step_observe_between(wfs_plan* plan, void* psi, const void* V, int nsteps, int observe_every,
+                             const wfs_box* boxes, int nboxes, double* out, wfs_between_steps between, void* ctx,
+                             void* stream);
+
+/*
  * Probability current through a plane (sources/measurement.py:190-302,
  * sources/cuda_kernels/probability_current_density.cu:4-218).  count[2] are
  * the (odd) pixel counts, size[2] the plane's width and height in Bohr radii,

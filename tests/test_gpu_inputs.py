@@ -292,3 +292,91 @@ def test_moving_wall_rebuilds_only_the_planes_it_sweeps(dev, tmp_path, monkeypat
     assert not calls
     assert torch.equal(out[0][1], out[1][1])
     assert torch.equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("precision,shape", [("c64", (64, 128, 128)), ("c128", (16, 16, 16))])
+def test_between_steps_callback_order_and_errors(dev, precision, shape):
+    """wfs_step_observe_between calls back after every step in order (slab scheme and the
+    per-step form), a callback that queues nothing leaves the result of wfs_step_observe bit for
+    bit, and an exception raised in it stops the call and reaches the caller."""
+    from wavefunctionsimpython_amd import _abi
+    prec = _abi.WFS_C64 if precision == "c64" else _abi.WFS_C128
+    dt = torch.complex64 if precision == "c64" else torch.complex128
+    rng = np.random.default_rng(2)
+    V = torch.from_numpy((rng.uniform(0, 1, shape) - 0.05j).astype(np.complex64)).to(dev)
+    psi0 = torch.from_numpy((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.1).to(dev, dt)
+    box = [((2, 4, 4), (10, 8, 8))]                              # (bottom voxel, voxel counts)
+    p = _abi.Plan(shape, (0.3, 0.25, 0.25), 0.01, 1.0, prec, _abi.WFS_METHOD_FOURIER)
+    if precision == "c64":
+        p.set_variant("yz_slab", 2)                              # the fused stretch
+    p.set_potential(V)
+    seen = []
+    p.step_observe(psi0.clone(), None, 7, 3, box, between=seen.append)
+    assert seen == list(range(7))
+    # sampled every step, a callback that queues nothing changes no bit (without the slab scheme
+    # the callback form makes one call per step, so the comparison samples every step)
+    seen.clear()
+    a = psi0.clone()
+    p.set_step_index(0)
+    oa = p.step_observe(a, None, 7, 1, box, between=seen.append)
+    assert seen == list(range(7))
+    b = psi0.clone()
+    p.set_step_index(0)
+    ob = p.step_observe(b, None, 7, 1, box)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b) and torch.equal(oa, ob)
+
+    class Stop(Exception):
+        pass
+
+    def stop_at_2(step):
+        if step == 2:
+            raise Stop()
+
+    with pytest.raises(Stop):
+        p.step_observe(psi0.clone(), None, 5, 1, box, between=stop_at_2)
+    torch.cuda.synchronize()
+
+
+def test_moving_wall_stretches_match_the_reference_loop(dev, tmp_path, monkeypatch):
+    """run_iteration steps a wall that moves every step in stretches with update_potential after
+    every step inside the call (128^3 complex64, slab scheme): sampled every step, the state and
+    the series equal the reference's time_step loop (sources/core_sim.py:19-24, 42-55) bit for
+    bit; sampled every 5 steps, the merged kinetic half steps leave it within float rounding."""
+    from wavefunctionsimpython_amd import _abi
+    used = []
+    original = _abi.Plan.step_observe
+
+    def spy(self, *args, between=None, **kw):
+        used.append(between is not None)
+        return original(self, *args, between=between, **kw)
+
+    monkeypatch.setattr(_abi.Plan, "step_observe", spy)
+    runs = {}
+    for mode in ("loop", "stretch1", "stretch5"):
+        cfg = _sweeping_wall_config(str(tmp_path / mode), 128, False)
+        cfg["simulation"]["total_iteration_count"] = 12
+        cfg["measurement"] = {"expected_location": {"enable_image": True}}
+        sim, tools, it = initialisation.initialize(use_cache=False, config=cfg)
+        if mode == "loop":
+            for k in range(it.total_iteration_count):
+                it.i = k
+                core_sim.time_step(sim, tools, it)
+        else:
+            core_sim.run_iteration(sim, tools, it, observe_every=1 if mode == "stretch1" else 5,
+                                   exit_on_quit=False)
+        torch.cuda.synchronize()
+        runs[mode] = (sim.get_wave_function().clone(), sim.get_potential().clone(),
+                      tools.get_expected_location().get_expected_location_evolution())
+    assert any(used), "no stretch with a callback ran"
+    psi_l, V_l, e_l = runs["loop"]
+    psi_1, V_1, e_1 = runs["stretch1"]
+    assert torch.equal(V_1, V_l)
+    assert torch.equal(psi_1, psi_l)
+    assert e_1.shape == e_l.shape and np.allclose(e_1, e_l, rtol=1e-12, atol=1e-13)
+    psi_5, V_5, e_5 = runs["stretch5"]
+    assert torch.equal(V_5, V_l)
+    err = oracle.relative_l2(psi_5.cpu().numpy(), psi_l.cpu().numpy())
+    print(f"moving wall, stretches sampled every 5 steps vs one call per step: {err:.2e}")
+    assert err < 1e-5
+    assert np.allclose(e_5, e_l[::5], atol=1e-5)

@@ -214,3 +214,36 @@ def test_duplicate_measurement_names_take_the_first_free_suffix(capsys):
     single = Named("screen")
     MeasurementTools._resolve_naming_conflicts(items, single)
     assert single.get_name() == "screen_1"
+
+
+def test_run_iteration_restores_a_sigint_handler_set_outside_python(monkeypatch):
+    """signal.getsignal() reads None for a handler not installed from Python (a profiler's, say);
+    run_iteration then restores the default handler instead of failing on the way out."""
+    import signal
+    from wavefunctionsimpython_amd import core_sim
+    calls = []
+    real = signal.signal
+
+    def fake_signal(sig, handler):
+        calls.append(handler)
+        return None if len(calls) == 1 else real(sig, handler)
+
+    class _Sim:
+        def potential_is_static(self):
+            return True
+
+        def get_delta_time_h_bar_per_hartree(self):
+            return 0.01
+
+    class _Tools:
+        def set_observe_every(self, n):
+            pass
+
+    it = IterData({})
+    it.i = it.total_iteration_count = 0
+    monkeypatch.setattr(signal, "signal", fake_signal)
+    try:
+        core_sim.run_iteration(_Sim(), _Tools(), it, observe_every=1, exit_on_quit=False)
+    finally:
+        real(signal.SIGINT, signal.default_int_handler)
+    assert calls[-1] is signal.SIG_DFL

@@ -31,6 +31,8 @@ SUPPORTED_LENGTHS = (8, 16, 32, 64, 96, 128, 192, 256, 512)
 _c_int3 = ctypes.c_int * 3
 _c_dbl3 = ctypes.c_double * 3
 _vp = ctypes.c_void_p
+# wfs_between_steps: int (*)(void* ctx, int step)
+BETWEEN_STEPS = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int)
 
 
 class WfsBox(ctypes.Structure):
@@ -73,6 +75,8 @@ SIGNATURES = {
                                                ctypes.POINTER(WfsBox), ctypes.c_int, _vp, _vp]),
     "wfs_step_observe": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(WfsBox),
                                         ctypes.c_int, _vp, _vp]),
+    "wfs_step_observe_between": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(WfsBox),
+                                                ctypes.c_int, _vp, BETWEEN_STEPS, _vp, _vp]),
     "wfs_plane_current": (ctypes.c_int, [_vp, _vp, _c_dbl3, _c_dbl3, ctypes.c_double * 2, ctypes.c_int * 2, _vp,
                                          _vp, _vp]),
     "wfs_init_gaussian": (ctypes.c_int, [_c_int3, _c_dbl3, ctypes.c_int, _c_dbl3, _c_dbl3, _c_dbl3, _vp, _vp]),
@@ -306,10 +310,14 @@ class Plan:
         return nb, arr
 
     def step_observe(self, psi: torch.Tensor, V: torch.Tensor | None, nsteps: int, observe_every: int, boxes,
-                     out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+                     out: torch.Tensor | None = None, stream=None, between=None) -> torch.Tensor:
         """nsteps steps, measuring the boxes before steps 0, observe_every, ... (< nsteps):
         device float64 tensor (ceil(nsteps / observe_every), [batch,] nboxes, 7).  Same state
-        and observables as alternating observables() and step(observe_every) calls."""
+        and observables as alternating observables() and step(observe_every) calls.
+
+        between(step): called after each step's kernels are queued, before the next step's
+        (wfs_step_observe_between); it may rewrite V and the table on the current stream, must
+        not touch psi or wait on the stream.  An exception in it stops the call and is re-raised."""
         check_state(psi, self.shape, self.precision)
         if V is None and self.method == WFS_METHOD_POWER_SERIES:
             V = self._V_ref
@@ -327,9 +335,28 @@ class Plan:
         elif (out.dtype != torch.float64 or out.numel() < nobs * 7 * nb * self.batch or not out.is_contiguous()
               or out.device != psi.device):
             raise ValueError("out must be a contiguous float64 device tensor with 7 entries per box and observation")
-        check(lib().wfs_step_observe(self._h, ptr(psi), ptr(V) if V is not None else None, int(nsteps),
-                                     int(observe_every), arr, nb, ptr(out) if nb else None, stream_handle(stream)),
-              "wfs_step_observe")
+        if between is None:
+            check(lib().wfs_step_observe(self._h, ptr(psi), ptr(V) if V is not None else None, int(nsteps),
+                                         int(observe_every), arr, nb, ptr(out) if nb else None,
+                                         stream_handle(stream)), "wfs_step_observe")
+        else:
+            raised = []
+
+            def hook(_ctx, step):
+                try:
+                    between(int(step))
+                    return 0
+                except BaseException as e:   # re-raised below, after the C call unwinds
+                    raised.append(e)
+                    return 1
+
+            fn = BETWEEN_STEPS(hook)
+            rc = lib().wfs_step_observe_between(self._h, ptr(psi), ptr(V) if V is not None else None, int(nsteps),
+                                                int(observe_every), arr, nb, ptr(out) if nb else None, fn, None,
+                                                stream_handle(stream))
+            if raised:
+                raise raised[0]
+            check(rc, "wfs_step_observe_between")
         if V is not None:
             self._potential_set = True
             if self.method == WFS_METHOD_POWER_SERIES:

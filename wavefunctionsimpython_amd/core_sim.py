@@ -66,8 +66,13 @@ def run_iteration(sim_state: SimState, measurement_tools: MeasurementTools, iter
         while i < total:
             t0 = time.time()
             iter_data.i = i
-            if (i % observe_every == 0 and sim_state.potential_is_static()
-                    and not sim_state.is_wave_function_saving() and not measurement_tools.needs_state_per_sample()):
+            stretch = (i % observe_every == 0 and not sim_state.is_wave_function_saving()
+                       and not measurement_tools.needs_state_per_sample())
+            static = sim_state.potential_is_static()
+            dynamic = (stretch and not static and sim_state.dynamic_stretches
+                       and sim_state.potential_changes_every_step()
+                       and bool(measurement_tools.observation_boxes()[0]))
+            if stretch and static:
                 # a stretch of whole sampling intervals in one call
                 span = min(max(STRETCH - STRETCH % observe_every, observe_every), total - i)
                 boxes, owners = measurement_tools.observation_boxes()
@@ -76,6 +81,13 @@ def run_iteration(sim_state: SimState, measurement_tools: MeasurementTools, iter
                                                      owners)
                 else:
                     sim_state.evolve_state(span)
+            elif dynamic:
+                # a potential that changes every step: the stretch runs update_potential after
+                # every step inside the call (time_step's evolve-then-update order)
+                span = min(max(STRETCH - STRETCH % observe_every, observe_every), total - i)
+                boxes, owners = measurement_tools.observation_boxes()
+                measurement_tools.record_samples(
+                    sim_state.evolve_state_observed(span, observe_every, boxes, update_between=True), owners)
             else:
                 if i % observe_every == 0:
                     measurement_tools.measure_and_render(sim_state, iter_data)
@@ -84,7 +96,8 @@ def run_iteration(sim_state: SimState, measurement_tools: MeasurementTools, iter
                 if not (span > 1 and sim_state.potential_is_static() and not sim_state.is_wave_function_saving()):
                     span = 1
                 sim_state.evolve_state(span)
-            sim_state.update_potential()
+            if not dynamic:
+                sim_state.update_potential()   # (a dynamic stretch updated after every step)
             i += span
             iter_data.elapsed_system_time_s += time.time() - t0
             if iter_data.is_quit:
@@ -96,7 +109,8 @@ def run_iteration(sim_state: SimState, measurement_tools: MeasurementTools, iter
                 return sim_state, measurement_tools, iter_data
         iter_data.i = total
     finally:
-        signal.signal(signal.SIGINT, previous)
+        # a handler installed outside Python reads back as None: restore the default then
+        signal.signal(signal.SIGINT, previous if previous is not None else signal.SIG_DFL)
     iter_data.total_simulated_time = sim_state.get_delta_time_h_bar_per_hartree() * iter_data.total_iteration_count
     iter_data.average_iteration_system_time_s = (iter_data.elapsed_system_time_s
                                                  / float(max(1, iter_data.total_iteration_count)))

@@ -611,7 +611,7 @@ int fourier_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t
 // wfs_observables and wfs_step(every) calls.  The slab scheme runs the exit of one call, the
 // observation and the entry of the next as one launch (run_slab_observe).
 int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every, const wfs_box* boxes, int nb,
-                   double* out, hipStream_t s);
+                   double* out, hipStream_t s, wfs_between_steps between = nullptr, void* ctx = nullptr);
 
 int power_series_steps(wfs_plan* p, void* psi, const void* V, int nsteps, hipStream_t s) {
   if (!V) { set_last_error("wfs_step: the power series needs V"); return WFS_ERR_INVALID; }
@@ -858,17 +858,31 @@ int observe(wfs_plan* p, const void* psi, const wfs_box* boxes, int nboxes, doub
   return 0;
 }
 
+// the caller's between-steps callback after step `step` (its kernels queued, the next step's not)
+static int call_between(wfs_between_steps between, void* ctx, int step) {
+  if (!between) return 0;
+  const int rc = between(ctx, step);
+  if (rc) {
+    set_last_error("wfs_step_observe_between: the callback returned %d after step %d", rc, step);
+    return WFS_ERR_INVALID;
+  }
+  return 0;
+}
+
 int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every, const wfs_box* boxes, int nb,
-                   double* out, hipStream_t s) {
+                   double* out, hipStream_t s, wfs_between_steps between, void* ctx) {
   const long per_obs = long(p->batch) * nb * 7;
   const bool fused = p->method == WFS_METHOD_FOURIER && p->variant[WFS_K_YZ_SLAB] != 0 && nb <= kBoxesPerPass &&
                      nb > 0;
   if (!fused) {
-    for (int k = 0; k < nsteps; k += every) {
-      if (nb > 0) WFS_TRY(observe(p, psi, boxes, nb, out + (k / every) * per_obs, s));
-      const int m = std::min(every, nsteps - k);
+    // with a callback, one step per call, so that it runs between every two steps
+    const int chunk = between ? 1 : every;
+    for (int k = 0; k < nsteps; k += chunk) {
+      if (nb > 0 && k % every == 0) WFS_TRY(observe(p, psi, boxes, nb, out + (k / every) * per_obs, s));
+      const int m = std::min(chunk, nsteps - k);
       WFS_TRY(p->method == WFS_METHOD_FOURIER ? fourier_steps(p, psi, k == 0 ? V : nullptr, m, s)
                                               : power_series_steps(p, psi, V, m, s));
+      WFS_TRY(call_between(between, ctx, k));
     }
     return 0;
   }
@@ -888,6 +902,9 @@ int steps_observed(wfs_plan* p, void* psi, const void* V, int nsteps, int every,
     for (int step = 1; step <= m; ++step) {
       WFS_TRY(run_slab_lines(p, LINES_FULL, psi, pv, s, b + k + step));
       WFS_TRY(run_x_slab(p, psi, step == m ? p->khalf : p->kfull, s, b + k + step));
+      // the table is read next by step k + step's y-z launch: what the callback queues on the
+      // stream to rewrite it runs after this step's and before that one
+      WFS_TRY(call_between(between, ctx, k + step - 1));
     }
     if (k + m < nsteps) {
       WFS_TRY(run_slab_observe(p, LINES_OBSERVE, psi, b + k + m, b + k + m, boxes, nb,
@@ -917,6 +934,22 @@ extern "C" int wfs_step_observe(wfs_plan* p, void* psi, const void* V, int nstep
   if (nsteps == 0) return 0;
   if (p->method == WFS_METHOD_POWER_SERIES && !V) { set_last_error("wfs_step_observe: the power series needs V"); return WFS_ERR_INVALID; }
   return steps_observed(p, psi, V, nsteps, observe_every, boxes, nboxes, out, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int wfs_step_observe_between(wfs_plan* p, void* psi, const void* V, int nsteps, int observe_every,
+                                        const wfs_box* boxes, int nboxes, double* out, wfs_between_steps between,
+                                        void* ctx, void* stream) {
+  if (!p || !psi || nsteps < 0 || observe_every < 1 || nboxes < 0 || (nboxes > 0 && (!boxes || !out))) {
+    set_last_error("wfs_step_observe_between: bad argument");
+    return WFS_ERR_INVALID;
+  }
+  if (nsteps == 0) return 0;
+  if (p->method == WFS_METHOD_POWER_SERIES && !V) {
+    set_last_error("wfs_step_observe_between: the power series needs V");
+    return WFS_ERR_INVALID;
+  }
+  return steps_observed(p, psi, V, nsteps, observe_every, boxes, nboxes, out, static_cast<hipStream_t>(stream),
+                        between, ctx);
 }
 
 extern "C" int wfs_plan_set_step_index(wfs_plan* p, long index) {

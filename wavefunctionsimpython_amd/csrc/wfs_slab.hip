@@ -254,21 +254,58 @@ __global__ __launch_bounds__(256) void plane_equal_kernel(const unsigned long lo
   } else {
     const unsigned long long* a = table + p * plane_elems;
     const unsigned long long* b = a - plane_elems;
-    for (long i = threadIdx.x; i < plane_elems && !differ; i += 256) differ = a[i] != b[i];
+    // 16 independent pairs of loads per round, issued together (no bound test between them;
+    // the early exit is tested once a round): a 256^2 plane takes 16 rounds, not 256 dependent
+    // steps.  The last partial round goes one element at a time.
+    constexpr int U = 16;
+    const long full = plane_elems - 256L * (U - 1);   // rounds starting below this hold U elements
+    long i0 = threadIdx.x;
+    for (; i0 < full && !differ; i0 += 256L * U) {
+      unsigned long long v[U], w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        v[u] = a[i0 + 256L * u];
+        w[u] = b[i0 + 256L * u];
+      }
+      unsigned long long d = 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) d |= v[u] ^ w[u];
+      differ = d != 0;
+    }
+    for (; i0 < plane_elems && !differ; i0 += 256) differ = a[i0] != b[i0];
   }
   differ = __syncthreads_or(differ);
   if (threadIdx.x == 0) flags[p] = differ ? 0 : 1;
 }
 
-// src[p] = the first plane of p's run of identical planes (one thread per packet)
-__global__ void plane_runs_kernel(const int* __restrict__ flags, int nx, int packets, int* __restrict__ src) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= packets) return;
-  const long base = long(k) * nx;
-  int first = int(base);
-  for (int x = 0; x < nx; ++x) {
-    if (!flags[base + x]) first = int(base + x);
-    src[base + x] = first;
+// src[p] = the first plane of p's run of identical planes, one block per packet: plane x of the
+// packet maps to the last x' <= x whose flag is 0 (plane 0 of a packet always starts a run).
+// Each thread takes a chunk of planes, the chunks' last run starts are max-scanned across the
+// block, and each thread fills its chunk from the start carried in -- instead of one thread
+// walking the packet's planes one dependent load at a time.
+__global__ __launch_bounds__(256) void plane_runs_kernel(const int* __restrict__ flags, int nx,
+                                                         int* __restrict__ src) {
+  __shared__ int last[256];
+  const long base = long(blockIdx.x) * nx;
+  const int t = threadIdx.x;
+  const int chunk = (nx + 255) / 256;
+  const int x0 = t * chunk < nx ? t * chunk : nx;
+  const int x1 = x0 + chunk < nx ? x0 + chunk : nx;
+  int l = -1;
+  for (int x = x0; x < x1; ++x)
+    if (!flags[base + x]) l = x;
+  last[t] = l;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {   // inclusive max-scan over the chunks
+    const int v = t >= d ? last[t - d] : -1;
+    __syncthreads();
+    if (v > last[t]) last[t] = v;
+    __syncthreads();
+  }
+  int cur = t > 0 ? last[t - 1] : -1;
+  for (int x = x0; x < x1; ++x) {
+    if (!flags[base + x]) cur = x;
+    src[base + x] = int(base) + cur;
   }
 }
 
@@ -356,7 +393,7 @@ int launch_plane_dedup(const void* table, long plane_elems, int nx, int packets,
                      static_cast<const unsigned long long*>(table), plane_elems, nx, first, flags);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return int(e);
-  WFS_LAUNCH(plane_runs_kernel, dim3(unsigned((packets + 63) / 64)), dim3(64), 0, s, flags, nx, packets, src);
+  WFS_LAUNCH(plane_runs_kernel, dim3(unsigned(packets)), dim3(256), 0, s, flags, nx, src);
   return int(hipGetLastError());
 }
 

@@ -392,6 +392,14 @@ class SimState:
                 return False
         return True
 
+    def potential_changes_every_step(self) -> bool:
+        """True once V has been rebuilt in dynamic mode and a wall still moves: every further
+        update_potential() changes V (an x centre clamped at 0 keeps moving in y and z, and an
+        unchanged centre skips the rebuild)."""
+        if not self.__is_dynamic_potential_mode or not getattr(self, "_regenerated", False):
+            return False
+        return any(np.any(w.velocity_bohr_radius_hartree_per_h_bar_3 != 0.0) for w in self.__potential_walls)
+
     def _refresh_potential(self):
         if self.__plan is None:
             raise RuntimeError("initialize_state() must be called before evolve_state()")
@@ -409,12 +417,28 @@ class SimState:
         self._refresh_potential()
         self.__plan.step(self.__wave_tensor, None, int(steps))
 
-    def evolve_state_observed(self, steps: int, observe_every: int, boxes, out: torch.Tensor | None = None):
+    def evolve_state_observed(self, steps: int, observe_every: int, boxes, out: torch.Tensor | None = None,
+                              update_between: bool = False):
         """``steps`` time steps with the box moments of the state before steps 0, observe_every, ...
         (< steps) -- the measure-then-evolve order of core_sim, sources/core_sim.py:19-24 -- in one
-        wfs_step_observe call.  Returns the device float64 samples (n_samples, n_boxes, 7)."""
+        wfs_step_observe call.  Returns the device float64 samples (n_samples, n_boxes, 7).
+
+        update_between: update_potential() and the plan's table refresh run after every step
+        inside the call (wfs_step_observe_between), as time_step's evolve-then-update order
+        does between calls -- the dynamic mode with moving walls at the fused rate."""
         self._refresh_potential()
-        return self.__plan.step_observe(self.__wave_tensor, None, int(steps), int(observe_every), boxes, out=out)
+        between = self._update_between_steps if update_between else None
+        return self.__plan.step_observe(self.__wave_tensor, None, int(steps), int(observe_every), boxes, out=out,
+                                        between=between)
+
+    # whether core_sim.run_iteration steps a potential that changes every step in stretches of
+    # steps with the update between them (evolve_state_observed(update_between=True)) instead of
+    # one evolve_state call per step
+    dynamic_stretches = True
+
+    def _update_between_steps(self, step: int):
+        self.update_potential()
+        self._refresh_potential()
 
     # ------------------------------------------------------------------ text
     def get_sim_state_description_text(self, use_colors=False):
